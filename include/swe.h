@@ -1,0 +1,174 @@
+/*
+ * swe.h -- C-ABI of the MI355X (gfx950) erosion library, libswe.so.
+ *
+ * The library replaces SimpleWindErosion's erosion hot path:
+ *   World::erode(int cycles)   source/world.hpp:21,36-77
+ *   Wind::Wind(vec2), fly()    source/wind.hpp:6,20,28-117
+ *   World::cascade(vec2)       source/world.hpp:22,79-153
+ *   quad::map get/height/normal, quad::_normal
+ *                              source/cellpool.h:200-223,432-480
+ * Plain C types only; every call returns 0 on success or a negative
+ * swe_status, and swe_last_error() describes the last failure of a context.
+ *
+ * Semantics (batched-synchronous): within one snapshot window every particle
+ * reads the same float32 snapshot of the map; pickups are summed as int64
+ * fixed point with 2^-32 resolution; cascade events are counted per cell and
+ * applied once per window (visits x the event's transfers).  Results are
+ * deterministic for given (params, seed, cycle counter) and independent of
+ * thread and launch order.
+ *
+ * Map layout: cell (x, y) lives at index x*ny + y (math::flatten,
+ * source/include/math.h:11-14): x is the slow axis, y the fast axis.
+ * The reference's third axis (pos.z) is the map's y axis.
+ *
+ * Threading: one context per thread; calls on a context are serialized on
+ * its HIP stream.  Two contexts may share a device.
+ */
+#ifndef SWE_H
+#define SWE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct swe_ctx swe_ctx;
+
+typedef enum {
+  SWE_OK = 0,
+  SWE_EINVAL = -1,      /* bad argument */
+  SWE_EHIP = -2,        /* HIP runtime error */
+  SWE_ENOMEM = -3,      /* device allocation failed */
+  SWE_EAUDIT = -4,      /* mass audit mismatch */
+  SWE_ESTATE = -5       /* call not valid in the current state */
+} swe_status;
+
+/*
+ * Parameters.  Replaces the mutable statics Wind::maxAge/boundarylayer/
+ * suspension/gravity (wind.hpp:15-26), Wind::pspeed (wind.hpp:13) and
+ * World::lrate/maxdiff/settling (world.hpp:17-19,30-32), plus the
+ * compile-time quad::mapscale (cellpool.h:184) and the boundary choice
+ * (bounded cellpool.h:82-94,432-445 or periodic cellpool.h:96-113,447-465).
+ * quad::lodsize is fixed at 1 as in the reference.
+ */
+typedef struct swe_params {
+  int32_t max_age;        /* Wind::maxAge = 512 -> 513 particle-steps */
+  float boundarylayer;    /* Wind::boundarylayer = 2.0 */
+  float suspension;       /* Wind::suspension = 0.05 */
+  float gravity;          /* Wind::gravity = 0.1 */
+  float pspeed[3];        /* Wind::pspeed = normalize(vec3(1,0,0)) */
+  float lrate;            /* World::lrate = 0.1 */
+  float maxdiff;          /* World::maxdiff = 0.005 */
+  float settling;         /* World::settling = 1.0 */
+  float abrasion;         /* sediment-layer extension (layers == 2): bedrock pickup rate */
+  float mapscale;         /* quad::mapscale = 80 */
+  int32_t window;         /* fly() calls per snapshot window; 0 = one window per cycle */
+  int32_t periodic;       /* 1 = periodic map, 0 = bounded (reference default) */
+  int32_t track;          /* 1 = massflow/momentum tracking and EMA fields */
+  int32_t layers;         /* 1 = height only (reference); 2 = height + loose sediment */
+  int32_t deposit_on_death; /* 1 = a dying particle drops its load (closes map mass) */
+  int32_t reserved;       /* must be 0 */
+  uint64_t seed;          /* World::SEED: feeds the counter RNG */
+} swe_params;
+
+/* Mass audit: exact in int64 fixed point (2^-32). */
+typedef struct swe_audit_t {
+  int64_t height_sum_lo;  /* sum over cells of height, low 64 bits of an int128 */
+  int64_t height_sum_hi;  /* high 64 bits */
+  int64_t sediment_sum_lo;/* layers == 2 */
+  int64_t sediment_sum_hi;
+  int64_t lost;           /* mass carried off by particles that died loaded */
+  uint64_t cycle;         /* cycles completed (RNG counter) */
+} swe_audit_t;
+
+typedef struct swe_stats_t {
+  uint64_t cycles;        /* map-cycles run on this context */
+  uint64_t particle_steps;/* fly() calls that returned true */
+  double march_ms;        /* HIP-event time of the particle march kernels (timing on) */
+  uint64_t march_launches;
+  double relax_ms;        /* HIP-event time of the relax kernels (timing on) */
+  uint64_t relax_launches;
+} swe_stats_t;
+
+/* Reference defaults (wind.hpp:13,23-26; world.hpp:30-32; cellpool.h:184). */
+void swe_params_default(swe_params* p);
+
+/*
+ * Create a context owning an nx x ny map on HIP device `device`.
+ * Replaces World::map / mappool::pool<cell> (world.hpp:13,28;
+ * SimpleWindErosion.cpp:11,36; cellpool.h:139-162,335-356).
+ * The map starts flat (height 0) with reference default parameters.
+ */
+int swe_create(swe_ctx** out, int32_t nx, int32_t ny, int32_t device);
+int swe_destroy(swe_ctx* ctx);
+const char* swe_last_error(const swe_ctx* ctx);
+
+int swe_set_params(swe_ctx* ctx, const swe_params* p);
+int swe_get_params(const swe_ctx* ctx, swe_params* p);
+
+/*
+ * Host <-> device map transfer (caller-owned host memory, nx*ny floats,
+ * index x*ny + y).  Upload converts to the fixed-point master layers and
+ * refreshes the snapshot; sediment may be NULL (zero thickness).
+ * Download returns the current snapshot of each requested layer; any
+ * pointer may be NULL.  momentum is 3*nx*ny floats (x, y, z planes).
+ */
+int swe_upload_layers(swe_ctx* ctx, const float* height, const float* sediment);
+int swe_download_layers(swe_ctx* ctx, float* height, float* sediment, float* massflow,
+                        float* momentum);
+/* Raw int64 fixed-point master layers (exact state), nx*ny each; NULL skips. */
+int swe_download_master(swe_ctx* ctx, int64_t* height_fx, int64_t* sediment_fx);
+int swe_upload_master(swe_ctx* ctx, const int64_t* height_fx, const int64_t* sediment_fx);
+/* EMA flow fields in: massflow nx*ny, momentum 3*nx*ny; NULL skips. */
+int swe_upload_fields(swe_ctx* ctx, const float* massflow, const float* momentum);
+
+/*
+ * World::erode(int cycles) (world.hpp:36-77): run n_cycles map-cycles, each
+ * spawning `particles_per_cycle` particles (the reference's `cycles`
+ * argument) and marching every one of them to death.  Asynchronous on the
+ * context's stream; swe_synchronize() waits.
+ */
+int swe_erode(swe_ctx* ctx, int64_t particles_per_cycle, int32_t n_cycles);
+int swe_synchronize(swe_ctx* ctx);
+
+/* Exact mass audit of the current state (synchronizes). */
+int swe_audit(swe_ctx* ctx, swe_audit_t* out);
+/* RNG cycle counter (for resume); setting it does not touch the map. */
+int swe_get_cycle(const swe_ctx* ctx, uint64_t* cycle);
+int swe_set_cycle(swe_ctx* ctx, uint64_t cycle);
+/* Carried-off mass counter (fixed point) for resume. */
+int swe_set_lost(swe_ctx* ctx, int64_t lost);
+
+/* Kernel timing with HIP events on the context stream (off by default). */
+int swe_set_timing(swe_ctx* ctx, int32_t on);
+int swe_stats(swe_ctx* ctx, swe_stats_t* out);
+int swe_reset_stats(swe_ctx* ctx);
+
+/*
+ * Surface normals of the current snapshot, quad::_normal (cellpool.h:200-223)
+ * for every cell, as used by updatenode (cellpool.h:302-333).  out: 3*nx*ny
+ * floats, planes x, y, z.
+ */
+int swe_normals(swe_ctx* ctx, float* out);
+
+/*
+ * Terrain generator for the benchmark inputs: seeded fBm gradient noise
+ * (hashed lattice gradients, quintic fade, `octaves` octaves with lacunarity
+ * 2 and gain 0.5, base frequency `base_cycles` periods across the map,
+ * periodic in both axes) scaled to [0,1], plus `n_pyramids` square pyramids
+ * with heights in [pyr_hmin, pyr_hmax] and base widths in [pyr_bmin,
+ * pyr_bmax] cells at seeded positions.  Host-side; out: nx*ny floats.
+ */
+int swe_generate_terrain(int32_t nx, int32_t ny, uint64_t seed, int32_t octaves,
+                         float base_cycles, int32_t n_pyramids, float pyr_hmin,
+                         float pyr_hmax, float pyr_bmin, float pyr_bmax, float* out);
+
+/* Library version string. */
+const char* swe_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_H */

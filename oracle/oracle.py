@@ -1,0 +1,216 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker or the timed CPU baseline.  The product package
+(simplewinderosion_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class OrParams(ctypes.Structure):
+    """or_params; the same layout as swe_params in include/swe.h."""
+
+    _fields_ = [
+        ("max_age", ctypes.c_int32),
+        ("boundarylayer", ctypes.c_float),
+        ("suspension", ctypes.c_float),
+        ("gravity", ctypes.c_float),
+        ("pspeed", ctypes.c_float * 3),
+        ("lrate", ctypes.c_float),
+        ("maxdiff", ctypes.c_float),
+        ("settling", ctypes.c_float),
+        ("abrasion", ctypes.c_float),
+        ("mapscale", ctypes.c_float),
+        ("window", ctypes.c_int32),
+        ("periodic", ctypes.c_int32),
+        ("track", ctypes.c_int32),
+        ("layers", ctypes.c_int32),
+        ("deposit_on_death", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+def reference_params(**kw):
+    """Reference defaults (wind.hpp:13,23-26; world.hpp:30-32; cellpool.h:184), bounded map."""
+    p = OrParams()
+    p.max_age = 512
+    p.boundarylayer = 2.0
+    p.suspension = 0.05
+    p.gravity = 0.1
+    p.pspeed[0], p.pspeed[1], p.pspeed[2] = 1.0, 0.0, 0.0
+    p.lrate = 0.1
+    p.maxdiff = 0.005
+    p.settling = 1.0
+    p.abrasion = 0.0
+    p.mapscale = 80.0
+    p.window = 0
+    p.periodic = 0
+    p.track = 1
+    p.layers = 1
+    p.deposit_on_death = 0
+    p.reserved = 0
+    p.seed = 0
+    for k, v in kw.items():
+        if k == "pspeed":
+            for i in range(3):
+                p.pspeed[i] = float(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_from(swe_params):
+    """Copy a swe_params-like ctypes struct (same fields) into OrParams."""
+    p = OrParams()
+    for name, _ in OrParams._fields_:
+        v = getattr(swe_params, name)
+        if name == "pspeed":
+            for i in range(3):
+                p.pspeed[i] = v[i]
+        else:
+            setattr(p, name, v)
+    return p
+
+
+_lib = None
+_F = ctypes.POINTER(ctypes.c_float)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_PP = ctypes.POINTER(OrParams)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "or_fx": (ctypes.c_int64, [ctypes.c_float]),
+        "or_unfx": (ctypes.c_float, [ctypes.c_int64]),
+        "or_expf": (ctypes.c_float, [ctypes.c_float]),
+        "or_mix64": (ctypes.c_uint64, [ctypes.c_uint64]),
+        "or_particle_key": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
+        "or_spawn": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _F, _F]),
+        "or_turb": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+        "or_normal": (None, [_F, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_float, ctypes.c_int32,
+                             ctypes.c_int32, _F]),
+        "or_cascade_cell": (ctypes.c_int32, [_F, _F, ctypes.c_int32, ctypes.c_int32, _PP, ctypes.c_int32,
+                                             ctypes.c_int32, _I64, _I64]),
+        "or_relax_only": (None, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _U32, _I64, _I64]),
+        "or_trajectory": (ctypes.c_int32, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, ctypes.c_uint64,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_int32, _F]),
+        "or_erode": (ctypes.c_int, [_PP, ctypes.c_int32, ctypes.c_int32, _I64, _I64, _F, _I64,
+                                    ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64, ctypes.c_int32,
+                                    ctypes.c_int32]),
+        "or_srand": (None, [ctypes.c_uint32]),
+        "or_erode_seq": (ctypes.c_int64, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _F, ctypes.c_int64,
+                                          ctypes.c_int32]),
+        "or_params_size": (ctypes.c_int32, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_I64)
+
+
+def fx(v):
+    return load().or_fx(float(v))
+
+
+def unfx(m):
+    return load().or_unfx(int(m))
+
+
+def fx_array(a):
+    lib = load()
+    a = np.asarray(a, np.float32).ravel()
+    return np.array([lib.or_fx(float(v)) for v in a], np.int64)
+
+
+class OracleState:
+    """Map state for the batched oracle: int64 masters, EMA fields, counters."""
+
+    def __init__(self, nx, ny, height, sediment=None, params=None):
+        self.nx, self.ny = int(nx), int(ny)
+        self.p = params if params is not None else reference_params()
+        lib = load()
+        h = np.ascontiguousarray(height, np.float32).ravel()
+        self.zm = np.array([lib.or_fx(float(v)) for v in h], np.int64) if h.size < 5000 else _fx_vec(h)
+        if sediment is not None:
+            s = np.ascontiguousarray(sediment, np.float32).ravel()
+            self.sm = _fx_vec(s)
+        else:
+            self.sm = np.zeros(self.nx * self.ny, np.int64)
+        self.ema = np.zeros(4 * self.nx * self.ny, np.float32)
+        self.lost = ctypes.c_int64(0)
+        self.cycle = ctypes.c_uint64(0)
+
+    def erode(self, particles, n_cycles=1, nthreads=0):
+        rc = load().or_erode(ctypes.byref(self.p), self.nx, self.ny, _i(self.zm),
+                             _i(self.sm) if self.p.layers == 2 else None,
+                             _f(self.ema) if self.p.track else None, ctypes.byref(self.lost),
+                             ctypes.byref(self.cycle), int(particles), int(n_cycles), int(nthreads))
+        if rc != 0:
+            raise RuntimeError("or_erode failed")
+
+    def height(self):
+        return unfx_vec(self.zm).reshape(self.nx, self.ny)
+
+
+def _fx_vec(a):
+    """or_fx over an array (vectorised restatement checked against or_fx in tests)."""
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    out = np.empty(a.size, np.int64)
+    lib = load()
+    # exact for |v| < 2^31 via float64: v*2^32 is exact in float64, rint is RNE
+    d = a.astype(np.float64) * 4294967296.0
+    ok = np.isfinite(d) & (np.abs(d) < 9.2e18)
+    out[ok] = np.rint(d[ok]).astype(np.int64)
+    for i in np.nonzero(~ok)[0]:
+        out[i] = lib.or_fx(float(a[i]))
+    return out
+
+
+def unfx_vec(m):
+    """or_unfx over an array: round(m * 2^-32) to float32 (int64 -> float64 is exact below 2^53)."""
+    m = np.asarray(m, np.int64)
+    big = np.abs(m) >= (1 << 53)
+    out = (m.astype(np.float64) * (2.0 ** -32)).astype(np.float32)
+    if big.any():
+        lib = load()
+        for i in np.nonzero(big.ravel())[0]:
+            out.ravel()[i] = lib.or_unfx(int(m.ravel()[i]))
+    return out
+
+
+def erode_seq(params, nx, ny, height, sediment=None, ema=None, particles=0, n_cycles=1, seed=None):
+    """The reference's sequential algorithm; returns particle-steps executed."""
+    lib = load()
+    if seed is not None:
+        lib.or_srand(int(seed) & 0xFFFFFFFF)
+    return lib.or_erode_seq(ctypes.byref(params), int(nx), int(ny), _f(height), _f(sediment), _f(ema),
+                            int(particles), int(n_cycles))

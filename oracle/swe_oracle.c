@@ -1,0 +1,918 @@
+/*
+ * swe_oracle.c -- THE ORACLE.  TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU restatement of SimpleWindErosion's erosion hot path
+ * (World::erode -> Wind::fly -> World::cascade).  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library, and only as the checker or as the timed CPU baseline; the product
+ * library (simplewinderosion_amd/libswe.so) never links, loads or calls it.
+ *
+ * Two restatements live here:
+ *
+ *   1. or_erode()      -- the batched-synchronous method.  Every particle in a
+ *      snapshot window reads the frozen float32 snapshot of the map; pickups
+ *      are summed as int64 fixed point (2^-32) deltas, cascade events are
+ *      counted per cell and applied once per window as visits x transfer.
+ *      The GPU library must agree with it bit for bit.
+ *
+ *   2. or_erode_seq()  -- the reference's own sequential algorithm: one
+ *      particle at a time writing float heights in place, AoS cells,
+ *      bounds-checked lookups, glibc rand() and libm expf.  This is the
+ *      "reference CPU path" that bench.py times as the CPU baseline and that
+ *      the statistical comparison uses.
+ *
+ * Pinning (what checks this file independently of the GPU library) is listed
+ * per function; the checks live in tests/test_oracle_*.py.
+ *
+ * Build: oracle/Makefile (gcc -O2 -std=c11 -ffp-contract=off -fopenmp).
+ * Floating point: no contraction, no fast-math, IEEE denormals, so each
+ * float operation below is one correctly-rounded IEEE operation.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Parameters: same layout as swe_params in include/swe.h                    */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  int32_t max_age;        /* Wind::maxAge         wind.hpp:23 (512)          */
+  float boundarylayer;    /* Wind::boundarylayer  wind.hpp:24 (2.0)          */
+  float suspension;       /* Wind::suspension     wind.hpp:25 (0.05)         */
+  float gravity;          /* Wind::gravity        wind.hpp:26 (0.1)          */
+  float pspeed[3];        /* Wind::pspeed         wind.hpp:13 normalize(1,0,0)*/
+  float lrate;            /* World::lrate         world.hpp:30 (0.1)         */
+  float maxdiff;          /* World::maxdiff       world.hpp:31 (0.005)       */
+  float settling;         /* World::settling      world.hpp:32 (1.0)         */
+  float abrasion;         /* two-layer extension: bedrock pickup rate        */
+  float mapscale;         /* quad::mapscale       cellpool.h:184 (80)        */
+  int32_t window;         /* fly() calls per snapshot window, 0 = per cycle  */
+  int32_t periodic;       /* 1: wrap (cellpool.h:96-113), 0: bounded         */
+  int32_t track;          /* flow tracking + EMA fields (world.hpp:38-44,69-76)*/
+  int32_t layers;         /* 1: height only (reference), 2: + sediment layer */
+  int32_t deposit_on_death;
+  int32_t reserved;
+  uint64_t seed;
+} or_params;
+
+/* ------------------------------------------------------------------------ */
+/* Bit helpers                                                              */
+/* ------------------------------------------------------------------------ */
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/*
+ * Float -> fixed point: round_half_even(v * 2^32) as int64, by integer
+ * operations on the IEEE encoding (no library rounding mode involved).
+ * NaN -> 0; |v * 2^32| >= 2^63 saturates.
+ * Pinned by: exact rational arithmetic in Python (fractions.Fraction) over
+ * random, boundary and tie cases.
+ */
+int64_t or_fx(float v)
+{
+  uint32_t u = f2u(v);
+  uint32_t sign = u >> 31;
+  int e = (int)((u >> 23) & 0xffu);
+  uint64_t m = u & 0x7fffffu;
+  uint64_t r;
+  if (e == 0xff) {
+    if (m) return 0;                                   /* NaN */
+    return sign ? INT64_MIN : INT64_MAX;               /* +-inf */
+  }
+  if (e == 0) return 0;                                /* |v| < 2^-126: rounds to 0 */
+  m |= 0x800000u;                                      /* 24-bit significand */
+  int sh = e - 118;                                    /* v*2^32 = m * 2^(e-150+32) */
+  if (sh >= 0) {
+    if (sh > 39) return sign ? INT64_MIN : INT64_MAX;
+    r = m << sh;
+    if (r > (uint64_t)INT64_MAX) return sign ? INT64_MIN : INT64_MAX;
+  } else {
+    int k = -sh;
+    if (k > 25) return 0;                              /* < 0.5 ulp of fixed point */
+    uint64_t q = m >> k;
+    uint64_t rem = m & ((1ull << k) - 1ull);
+    uint64_t half = 1ull << (k - 1);
+    if (rem > half || (rem == half && (q & 1ull))) q += 1;
+    r = q;
+  }
+  return sign ? -(int64_t)r : (int64_t)r;
+}
+
+/*
+ * Fixed point -> float: round_half_even(m * 2^-32) to float32 by integer
+ * operations.  Pinned by: exact rational rounding in Python and by the
+ * compiler's int64->float conversion (x86 cvtsi2ss, RNE) times 2^-32.
+ */
+float or_unfx(int64_t m)
+{
+  if (m == 0) return 0.0f;
+  uint32_t sign = m < 0;
+  uint64_t a = sign ? (uint64_t)0 - (uint64_t)m : (uint64_t)m;
+  int msb = 63 - __builtin_clzll(a);
+  uint64_t q;
+  int ex;                                              /* value = q * 2^(ex) */
+  if (msb <= 23) {
+    q = a;
+    ex = -32;
+  } else {
+    int k = msb - 23;
+    q = a >> k;
+    uint64_t rem = a & ((1ull << k) - 1ull);
+    uint64_t half = 1ull << (k - 1);
+    if (rem > half || (rem == half && (q & 1ull))) q += 1;   /* q <= 2^24 */
+    ex = k - 32;
+  }
+  /* q < 2^25 and q*2^ex is a normal float: build it exactly */
+  float fq = (float)(uint32_t)q;                       /* exact: q <= 2^24 */
+  float sc = u2f((uint32_t)(ex + 127) << 23);          /* 2^ex, ex in [-32, 31] */
+  float r = fq * sc;                                   /* exact power-of-two scale */
+  return sign ? -r : r;
+}
+
+/*
+ * The project's expf, shared by specification (not by code) with the GPU
+ * library: Cody-Waite range reduction x = k*ln2 + r with a two-constant
+ * ln2 split, e^r by the degree-7 Taylor polynomial in Horner form, then an
+ * exact power-of-two scale (two steps below 2^-125 so that the only rounding
+ * into the subnormal range is the final multiply).  Every step is one IEEE
+ * float operation, so two IEEE implementations give identical bits.
+ * Used by wind.hpp:50 (argument <= 0 after the ground clamp at :42-43).
+ * Pinned by: max error <= 2 ulp against double-precision exp over a dense
+ * sweep, and exact values at 0 and the underflow threshold.
+ */
+float or_expf(float x)
+{
+  if (x != x) return x;                                /* NaN */
+  if (x > 88.72283935546875f) return u2f(0x7f800000u); /* +inf */
+  if (x < -104.0f) return 0.0f;
+  const float log2e = 1.44269502162933349609375f;      /* 0x3FB8AA3B */
+  const float ln2_hi = 0.693145751953125f;             /* 0x3F317200 */
+  const float ln2_lo = 1.428606765330187045e-06f;      /* 0x35BFBE8E */
+  float kf = rintf(x * log2e);
+  float r = (x - kf * ln2_hi) - kf * ln2_lo;
+  float p = 1.0f / 5040.0f;
+  p = p * r + 1.0f / 720.0f;
+  p = p * r + 1.0f / 120.0f;
+  p = p * r + 1.0f / 24.0f;
+  p = p * r + 1.0f / 6.0f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  int k = (int)kf;
+  if (k < -125) {
+    p = p * u2f((uint32_t)(127 - 64) << 23);           /* * 2^-64, exact */
+    k += 64;
+  }
+  return p * u2f((uint32_t)(k + 127) << 23);
+}
+
+/*
+ * Counter-based RNG: the SplitMix64 output function (Steele, Lea, Flood,
+ * "Fast splittable pseudorandom number generators", OOPSLA 2014) applied to
+ * (seed, cycle, particle, call) counters.  It replaces glibc rand() at
+ * world.hpp:52,54 (spawn) and wind.hpp:83 (turbulence) so that particles are
+ * order-free.  Pinned by: the published SplitMix64 test sequence for seed 0
+ * and the key schedule restated in Python.
+ */
+#define OR_G 0x9E3779B97F4A7C15ull
+#define OR_P 0xD1B54A32D192ED03ull
+uint64_t or_mix64(uint64_t z)
+{
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+uint64_t or_particle_key(uint64_t seed, uint64_t cycle, uint64_t particle)
+{
+  uint64_t ck = or_mix64(seed + OR_G * (cycle + 1));
+  return or_mix64(ck ^ (OR_P * (particle + 1)));
+}
+/* world.hpp:52-54: integer cell + thousandths, restated on the counter RNG */
+void or_spawn(uint64_t pk, int32_t nx, int32_t ny, float* px, float* pz)
+{
+  uint64_t a = or_mix64(pk + 1 * OR_G);
+  uint64_t b = or_mix64(pk + 2 * OR_G);
+  int32_t ix = (int32_t)((uint32_t)a % (uint32_t)nx);
+  int32_t iy = (int32_t)((uint32_t)(a >> 32) % (uint32_t)ny);
+  int32_t fx = (int32_t)((uint32_t)b % 1000u);
+  int32_t fy = (int32_t)((uint32_t)(b >> 32) % 1000u);
+  *px = (float)ix + (float)fx / 1000.0f;
+  *pz = (float)iy + (float)fy / 1000.0f;
+}
+/* wind.hpp:83: three draws in [0,1000] for the fly() call with age k */
+void or_turb(uint64_t pk, int32_t k, int32_t out[3])
+{
+  uint64_t t = or_mix64(pk + (uint64_t)(k + 3) * OR_G);
+  out[0] = (int32_t)((t & 0x1fffffull) % 1001ull);
+  out[1] = (int32_t)(((t >> 21) & 0x1fffffull) % 1001ull);
+  out[2] = (int32_t)(((t >> 42) & 0x1fffffull) % 1001ull);
+}
+
+/* ------------------------------------------------------------------------ */
+/* glm arithmetic, in glm 0.9.9's operation order                           */
+/* ------------------------------------------------------------------------ */
+typedef struct { float x, y, z; } v3;
+static inline v3 V3(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vmul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 smul(float s, v3 a) { return V3(s * a.x, s * a.y, s * a.z); }
+static inline v3 mulsv(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 vdivs(v3 a, float s) { return V3(a.x / s, a.y / s, a.z / s); }
+static inline v3 vsubs(v3 a, float s) { return V3(a.x - s, a.y - s, a.z - s); }
+/* glm::dot: (a.x*b.x + a.y*b.y) + a.z*b.z */
+static inline float dot3(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+/* glm::cross */
+static inline v3 cross3(v3 a, v3 b)
+{
+  return V3(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+/* glm::length = sqrt(dot(v,v)); glm::normalize = v * (1/sqrt(dot(v,v))) */
+static inline float len3(v3 a) { return sqrtf(dot3(a, a)); }
+static inline v3 norm3(v3 a) { return mulsv(a, 1.0f / sqrtf(dot3(a, a))); }
+/* glm::mix(x, y, a) = x*(1-a) + y*a */
+static inline v3 mix3(v3 x, v3 y, float a) { return vadd(mulsv(x, 1.0f - a), mulsv(y, a)); }
+
+/* ------------------------------------------------------------------------ */
+/* Map geometry                                                             */
+/* ------------------------------------------------------------------------ */
+static inline int32_t wrapi(int32_t v, int32_t n) { int32_t r = v % n; return r < 0 ? r + n : r; }
+static inline int oob(int32_t x, int32_t y, int32_t nx, int32_t ny, int periodic)
+{
+  if (periodic) return 0;
+  return x < 0 || y < 0 || x >= nx || y >= ny;       /* cellpool.h:432-438 */
+}
+/* map::height (cellpool.h:472-476), periodic variant per cellpool.h:447-465 */
+static inline float hgt(const float* S, int32_t x, int32_t y, int32_t nx, int32_t ny, int periodic)
+{
+  if (periodic) return S[(int64_t)wrapi(x, nx) * ny + wrapi(y, ny)];
+  if (x < 0 || y < 0 || x >= nx || y >= ny) return 0.0f;
+  return S[(int64_t)x * ny + y];                       /* math::flatten, math.h:11-14 */
+}
+
+/*
+ * quad::_normal (cellpool.h:200-223): sum of four cross products of scaled
+ * edge vectors, each gated by its diagonal being in bounds, normalized when
+ * the length is positive.  Pinned by: flat map -> (0,1,0); the plane
+ * h = a*x -> normalize(-mapscale*a, 1, 0) and h = b*y -> (0,1,-mapscale*b);
+ * bounded corners use the documented partial plane sets.
+ */
+void or_normal(const float* S, int32_t nx, int32_t ny, int periodic, float mapscale,
+               int32_t x, int32_t y, float out[3])
+{
+  v3 n = V3(0.0f, 0.0f, 0.0f);
+  const v3 s = V3(1.0f, mapscale, 1.0f);
+  float hp = hgt(S, x, y, nx, ny, periodic);
+  if (!oob(x + 1, y + 1, nx, ny, periodic))
+    n = vadd(n, cross3(vmul(s, V3(0.0f, hgt(S, x, y + 1, nx, ny, periodic) - hp, 1.0f)),
+                       vmul(s, V3(1.0f, hgt(S, x + 1, y, nx, ny, periodic) - hp, 0.0f))));
+  if (!oob(x - 1, y - 1, nx, ny, periodic))
+    n = vadd(n, cross3(vmul(s, V3(0.0f, hgt(S, x, y - 1, nx, ny, periodic) - hp, -1.0f)),
+                       vmul(s, V3(-1.0f, hgt(S, x - 1, y, nx, ny, periodic) - hp, 0.0f))));
+  if (!oob(x + 1, y - 1, nx, ny, periodic))
+    n = vadd(n, cross3(vmul(s, V3(1.0f, hgt(S, x + 1, y, nx, ny, periodic) - hp, 0.0f)),
+                       vmul(s, V3(0.0f, hgt(S, x, y - 1, nx, ny, periodic) - hp, -1.0f))));
+  if (!oob(x - 1, y + 1, nx, ny, periodic))
+    n = vadd(n, cross3(vmul(s, V3(-1.0f, hgt(S, x - 1, y, nx, ny, periodic) - hp, 0.0f)),
+                       vmul(s, V3(0.0f, hgt(S, x, y + 1, nx, ny, periodic) - hp, 1.0f))));
+  if (len3(n) > 0.0f) n = norm3(n);
+  out[0] = n.x; out[1] = n.y; out[2] = n.z;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cascade event (world.hpp:79-153) against a frozen snapshot                */
+/* ------------------------------------------------------------------------ */
+static const int32_t OFFX[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+static const int32_t OFFY[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+typedef struct { int64_t cell; float h; float s; float d; } or_point;
+
+/*
+ * One cascade event at cell (x,y): gather the in-bounds neighbours in the
+ * fixed offset order (world.hpp:83-114), sort by height descending with a
+ * stable insertion sort (libstdc++ std::sort on 8 elements, world.hpp:118),
+ * then walk them with the centre height evolving (world.hpp:122-151).
+ * Writes, per transfer, the neighbour cell and the signed fixed-point amount
+ * the neighbour receives; the centre receives the negated sum.  With
+ * layers == 2 only loose sediment moves: a transfer is capped by the giver's
+ * sediment snapshot (the centre's evolves within the event).
+ * Returns the number of transfers.
+ * Pinned by: hand-computed 3x3 cases (pair transfer settling*(|dh|-d*maxdiff)/2,
+ * diagonal d = sqrtf(2), tie order = offset order, evolving centre).
+ */
+int32_t or_cascade_event(const float* S, const float* SS, int32_t nx, int32_t ny,
+                         const or_params* p, int32_t x, int32_t y,
+                         int64_t nb_cell[8], int64_t nb_fx[8])
+{
+  or_point sn[8];
+  int num = 0;
+  for (int k = 0; k < 8; k++) {
+    int32_t qx = x + OFFX[k], qy = y + OFFY[k];
+    if (oob(qx, qy, nx, ny, p->periodic)) continue;
+    if (p->periodic) { qx = wrapi(qx, nx); qy = wrapi(qy, ny); }
+    int64_t c = (int64_t)qx * ny + qy;
+    float fxk = (float)OFFX[k], fyk = (float)OFFY[k];
+    sn[num].cell = c;
+    sn[num].h = S[c];
+    sn[num].s = SS ? SS[c] : 0.0f;
+    sn[num].d = sqrtf(fxk * fxk + fyk * fyk);          /* length(vec2(nn)) */
+    num++;
+  }
+  /* stable insertion sort, key h descending */
+  for (int i = 1; i < num; i++) {
+    or_point v = sn[i];
+    int j = i - 1;
+    while (j >= 0 && v.h > sn[j].h) { sn[j + 1] = sn[j]; j--; }
+    sn[j + 1] = v;
+  }
+  int64_t cc = (int64_t)(p->periodic ? wrapi(x, nx) : x) * ny + (p->periodic ? wrapi(y, ny) : y);
+  float hc = S[cc];
+  float sc = SS ? SS[cc] : 0.0f;
+  int32_t cnt = 0;
+  for (int i = 0; i < num; i++) {
+    float diff = hc - sn[i].h;
+    if (diff == 0.0f) continue;
+    float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;   /* * lodsize */
+    if (excess <= 0.0f) continue;
+    float t = p->settling * excess / 2.0f;
+    if (p->layers == 2) {
+      float avail = diff > 0.0f ? sc : sn[i].s;
+      if (t > avail) t = avail;
+      if (!(t > 0.0f)) continue;
+    }
+    int64_t tf = or_fx(t);
+    if (diff > 0.0f) {
+      hc -= t; sc -= t;
+      nb_cell[cnt] = sn[i].cell; nb_fx[cnt] = tf;
+    } else {
+      hc += t; sc += t;
+      nb_cell[cnt] = sn[i].cell; nb_fx[cnt] = -tf;
+    }
+    cnt++;
+  }
+  return cnt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Batched-synchronous erosion                                              */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+  float px, py, pz, sx, sy, sz, sed;
+  int32_t age;            /* < 0: dead */
+  int64_t ledger;         /* fixed-point sediment carried */
+  int64_t last_cell;
+  uint64_t key;
+} or_wind;
+
+typedef struct {
+  const or_params* p;
+  int32_t nx, ny;
+  const float* S;         /* height snapshot */
+  const float* SS;        /* sediment snapshot (layers == 2) */
+  int64_t* D;             /* height deltas */
+  int64_t* DS;            /* sediment deltas (layers == 2) */
+  uint32_t* V;            /* cascade events per cell */
+  int64_t* T;             /* 4 tracks: massflow, momentum x, y, z */
+  int64_t* lost;          /* mass carried off by dying particles */
+  v3 ps, nps;             /* pspeed and normalize(pspeed) */
+} or_march;
+
+static inline void add64(int64_t* a, int64_t v)
+{
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+  *a += v;
+}
+static inline void add32(uint32_t* a, uint32_t v)
+{
+#ifdef _OPENMP
+#pragma omp atomic
+#endif
+  *a += v;
+}
+
+static void or_die(or_march* m, or_wind* w, int64_t cell)
+{
+  if (m->p->deposit_on_death && w->ledger != 0) {
+    int64_t c = cell >= 0 ? cell : w->last_cell;
+    add64(&m->D[c], w->ledger);
+    if (m->p->layers == 2) add64(&m->DS[c], w->ledger);
+  } else if (w->ledger != 0) {
+    add64(m->lost, w->ledger);
+  }
+  w->age = -1;
+}
+
+/*
+ * Wind::fly (wind.hpp:28-117) against the snapshot.  Returns 1 when the call
+ * is a particle-step (fly() == true), 0 when the particle dies.
+ */
+static int or_fly(or_march* m, or_wind* w)
+{
+  const or_params* p = m->p;
+  const int32_t nx = m->nx, ny = m->ny;
+  /* wind.hpp:30 -- glm::round is round-half-away-from-zero */
+  float rx = roundf(w->px), rz = roundf(w->pz);
+  if (!(fabsf(rx) < 1073741824.0f) || !(fabsf(rz) < 1073741824.0f)) { or_die(m, w, -1); return 0; }
+  int32_t ix = (int32_t)rx, iy = (int32_t)rz;
+  /* wind.hpp:32-38 -- bounded map: leaving the map ends the particle */
+  if (oob(ix, iy, nx, ny, p->periodic)) { or_die(m, w, -1); return 0; }
+  int32_t cx = p->periodic ? wrapi(ix, nx) : ix;
+  int32_t cy = p->periodic ? wrapi(iy, ny) : iy;
+  int64_t cell = (int64_t)cx * ny + cy;
+  float nn[3];
+  or_normal(m->S, nx, ny, p->periodic, p->mapscale, cx, cy, nn);   /* wind.hpp:40 */
+  v3 n = V3(nn[0], nn[1], nn[2]);
+  float h = m->S[cell];
+  if (w->age == 0 || w->py < h) w->py = h;                          /* :42-43 */
+  int32_t age = w->age;
+  w->age = age + 1;
+  if (age > p->max_age) { or_die(m, w, cell); return 0; }         /* :45-46 */
+
+  float hfac = or_expf(-(w->py - h) / p->boundarylayer);            /* :50-52 */
+  if (hfac < 0.0f) hfac = 0.0f;
+
+  float shadow = dot3(m->nps, n);                                   /* :56-59 */
+  if (shadow < 0.0f) shadow = 0.0f;
+  shadow = 1.0f - shadow;
+
+  v3 sp = V3(w->sx, w->sy, w->sz);
+  sp = vadd(sp, smul(0.05f, vsub(mulsv(m->ps, 0.1f + 0.9f * shadow), sp)));  /* :61 */
+  /* (0.1f+0.9f*shadow)*pspeed: glm's scalar*vec multiplies each component */
+
+  if (w->py > h) sp.y -= p->gravity * w->sed;                        /* :65-66 */
+
+  float collision = -dot3(norm3(sp), n);                             /* :70-71 */
+  if (collision < 0.0f) collision = 0.0f;
+
+  v3 rspeed = cross3(n, cross3(smul(1.0f - collision, sp), n));     /* :75 */
+
+  sp = vadd(sp, smul(0.9f, vsub(smul(shadow, mix3(m->ps, rspeed, shadow * hfac)), sp)));  /* :79 */
+
+  int32_t r[3];                                                      /* :83 */
+  or_turb(w->key, age, r);
+  {
+    float k = 0.1f * hfac * collision;
+    v3 t = vsubs(V3((float)r[0], (float)r[1], (float)r[2]), 500.0f);
+    sp = vadd(sp, vdivs(smul(k, t), 500.0f));
+  }
+
+  {                                                                  /* :87 */
+    float drag = (float)(1.0 - 0.3 * (double)w->sed);
+    sp = mulsv(sp, drag);
+  }
+
+  w->px += sp.x; w->py += sp.y; w->pz += sp.z;                       /* :91 */
+
+  if (p->track) {                                                    /* :93-96 */
+    add64(&m->T[0 * (int64_t)nx * ny + cell], or_fx(w->sed));
+    add64(&m->T[1 * (int64_t)nx * ny + cell], or_fx(sp.x));
+    add64(&m->T[2 * (int64_t)nx * ny + cell], or_fx(sp.y));
+    add64(&m->T[3 * (int64_t)nx * ny + cell], or_fx(sp.z));
+  }
+
+  float force = -dot3(norm3(sp), n) * len3(sp);                      /* :100-102 */
+  if (force < 0.0f) force = 0.0f;
+  float lift = (1.0f - collision) * len3(sp);                        /* :104 */
+  float capacity = force * hfac + 0.02f * lift * hfac;              /* :106 */
+
+  float diff = capacity - w->sed;                                    /* :110-112 */
+  float q;
+  int64_t qf;
+  if (p->layers == 2 && diff > 0.0f && m->SS[cell] <= 0.0f) {
+    q = p->abrasion * diff;                   /* bare bedrock: abrasion rate */
+    qf = or_fx(q);
+    add64(&m->D[cell], -qf);
+  } else {
+    q = p->suspension * diff;
+    qf = or_fx(q);
+    add64(&m->D[cell], -qf);
+    if (p->layers == 2) add64(&m->DS[cell], -qf);
+  }
+  w->sed += q;
+  w->ledger += qf;
+  w->sx = sp.x; w->sy = sp.y; w->sz = sp.z;
+  w->last_cell = cell;
+
+  add32(&m->V[cell], 1u);                                            /* :114 */
+  return 1;
+}
+
+static void or_relax(const or_params* p, int32_t nx, int32_t ny, const float* S, const float* SS,
+                     const uint32_t* V, int64_t* D, int64_t* DS)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64)
+#endif
+  for (int64_t c = 0; c < ncell; c++) {
+    uint32_t v = V[c];
+    if (!v) continue;
+    int64_t nbc[8], nbf[8];
+    int32_t cnt = or_cascade_event(S, SS, nx, ny, p, (int32_t)(c / ny), (int32_t)(c % ny), nbc, nbf);
+    int64_t centre = 0;
+    for (int32_t i = 0; i < cnt; i++) {
+      int64_t a = (int64_t)v * nbf[i];
+      add64(&D[nbc[i]], a);
+      if (p->layers == 2) add64(&DS[nbc[i]], a);
+      centre -= a;
+    }
+    if (cnt) {
+      add64(&D[c], centre);
+      if (p->layers == 2) add64(&DS[c], centre);
+    }
+  }
+}
+
+static int64_t or_windows(const or_params* p, int32_t* R)
+{
+  int64_t calls = (int64_t)p->max_age + 2;           /* 513 steps + the dying call */
+  if (calls < 1) calls = 1;
+  int32_t r = p->window > 0 ? p->window : (int32_t)calls;
+  *R = r;
+  return (calls + r - 1) / r;
+}
+
+/*
+ * World::erode(cycles) (world.hpp:36-77) in batched-synchronous form, run
+ * n_cycles times with `particles` particles each.
+ *   zm       int64[nx*ny]   height, fixed point 2^-32 (in/out)
+ *   sm       int64[nx*ny]   sediment thickness (layers == 2; in/out) or NULL
+ *   ema      float[4*nx*ny] massflow, momentum x/y/z EMA fields (track) or NULL
+ *   lost     int64          mass carried off by dying particles (in/out)
+ *   cycle    uint64         cycle counter feeding the RNG (in/out)
+ * Returns 0, or -1 on bad arguments / allocation failure.
+ */
+int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* sm,
+             float* ema, int64_t* lost, uint64_t* cycle, int64_t particles, int32_t n_cycles,
+             int32_t nthreads)
+{
+  if (nx < 1 || ny < 1 || particles < 0 || n_cycles < 0 || !zm || !lost || !cycle) return -1;
+  if (p->layers == 2 && !sm) return -1;
+  if (p->track && !ema) return -1;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  const int64_t ncell = (int64_t)nx * ny;
+  float* S = (float*)malloc(sizeof(float) * ncell);
+  float* SS = p->layers == 2 ? (float*)malloc(sizeof(float) * ncell) : NULL;
+  int64_t* D = (int64_t*)calloc(ncell, sizeof(int64_t));
+  int64_t* DS = p->layers == 2 ? (int64_t*)calloc(ncell, sizeof(int64_t)) : NULL;
+  uint32_t* V = (uint32_t*)calloc(ncell, sizeof(uint32_t));
+  int64_t* T = p->track ? (int64_t*)calloc(4 * ncell, sizeof(int64_t)) : NULL;
+  or_wind* W = (or_wind*)malloc(sizeof(or_wind) * (particles > 0 ? particles : 1));
+  if (!S || !D || !V || !W || (p->layers == 2 && (!SS || !DS)) || (p->track && !T)) {
+    free(S); free(SS); free(D); free(DS); free(V); free(T); free(W);
+    return -1;
+  }
+  or_march m;
+  m.p = p; m.nx = nx; m.ny = ny; m.S = S; m.SS = SS; m.D = D; m.DS = DS; m.V = V; m.T = T;
+  m.lost = lost;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);                                 /* normalize(pspeed), wind.hpp:56 */
+
+  int32_t R;
+  int64_t nwin = or_windows(p, &R);
+
+  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+    for (int64_t c = 0; c < ncell; c++) S[c] = or_unfx(zm[c]);
+    if (SS) for (int64_t c = 0; c < ncell; c++) SS[c] = or_unfx(sm[c]);
+    if (T) memset(T, 0, sizeof(int64_t) * 4 * ncell);  /* world.hpp:38-44 */
+
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t i = 0; i < particles; i++) {          /* world.hpp:52-62 */
+      or_wind* w = &W[i];
+      w->key = or_particle_key(p->seed, *cycle, (uint64_t)i);
+      or_spawn(w->key, nx, ny, &w->px, &w->pz);
+      w->py = 0.0f; w->sx = w->sy = w->sz = 0.0f; w->sed = 0.0f;
+      w->age = 0; w->ledger = 0; w->last_cell = 0;
+    }
+
+    for (int64_t win = 0; win < nwin; win++) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256)
+#endif
+      for (int64_t i = 0; i < particles; i++) {
+        or_wind* w = &W[i];
+        for (int32_t s = 0; s < R && w->age >= 0; s++) or_fly(&m, w);
+      }
+      or_relax(p, nx, ny, S, SS, V, D, DS);
+      for (int64_t c = 0; c < ncell; c++) {
+        zm[c] += D[c]; D[c] = 0; V[c] = 0;
+        S[c] = or_unfx(zm[c]);
+      }
+      if (SS) for (int64_t c = 0; c < ncell; c++) {
+        int64_t v = sm[c] + DS[c];
+        sm[c] = v < 0 ? 0 : v;
+        DS[c] = 0;
+        SS[c] = or_unfx(sm[c]);
+      }
+    }
+    /* particles still alive after the last window cannot exist: every
+       particle dies within max_age + 2 calls */
+    if (T) {                                             /* world.hpp:69-76 */
+      const float a = 1.0f - p->lrate;
+      for (int f = 0; f < 4; f++)
+        for (int64_t c = 0; c < ncell; c++) {
+          float tr = or_unfx(T[f * ncell + c]);
+          ema[f * ncell + c] = a * ema[f * ncell + c] + p->lrate * tr;
+        }
+    }
+    *cycle += 1;
+  }
+  free(S); free(SS); free(D); free(DS); free(V); free(T); free(W);
+  return 0;
+}
+
+/* Small entry points for known-answer tests of the pieces */
+int32_t or_cascade_cell(const float* S, const float* SS, int32_t nx, int32_t ny, const or_params* p,
+                        int32_t x, int32_t y, int64_t* nb_cell, int64_t* nb_fx)
+{
+  return or_cascade_event(S, SS, nx, ny, p, x, y, nb_cell, nb_fx);
+}
+
+/*
+ * Batched relax of one window in isolation: given a snapshot and visit
+ * counts, return the cascade deltas (height, and sediment if layers == 2).
+ */
+void or_relax_only(const or_params* p, int32_t nx, int32_t ny, const float* S, const float* SS,
+                   const uint32_t* V, int64_t* D, int64_t* DS)
+{
+  or_relax(p, nx, ny, S, SS, V, D, DS);
+}
+
+/*
+ * Single particle trajectory against a fixed snapshot (no map feedback):
+ * runs up to max_calls fly() calls and records per step the position,
+ * speed, sediment and pickup.  out: float[max_calls * 8]; returns steps.
+ */
+int32_t or_trajectory(const or_params* p, int32_t nx, int32_t ny, const float* S, const float* SS,
+                      uint64_t key, float px, float pz, int32_t max_calls, float* out)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  int64_t* D = (int64_t*)calloc(ncell, sizeof(int64_t));
+  int64_t* DS = (int64_t*)calloc(ncell, sizeof(int64_t));
+  uint32_t* V = (uint32_t*)calloc(ncell, sizeof(uint32_t));
+  int64_t* T = (int64_t*)calloc(4 * ncell, sizeof(int64_t));
+  int64_t lost = 0;
+  or_march m;
+  m.p = p; m.nx = nx; m.ny = ny; m.S = S; m.SS = SS; m.D = D; m.DS = DS; m.V = V; m.T = T;
+  m.lost = &lost;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);
+  or_wind w;
+  w.key = key; w.px = px; w.pz = pz; w.py = 0.0f; w.sx = w.sy = w.sz = 0.0f; w.sed = 0.0f;
+  w.age = 0; w.ledger = 0; w.last_cell = 0;
+  int32_t n = 0;
+  for (int32_t i = 0; i < max_calls; i++) {
+    float sed0 = w.sed;
+    if (!or_fly(&m, &w)) break;
+    float* o = out + (int64_t)n * 8;
+    o[0] = w.px; o[1] = w.py; o[2] = w.pz;
+    o[3] = w.sx; o[4] = w.sy; o[5] = w.sz;
+    o[6] = w.sed; o[7] = w.sed - sed0;
+    n++;
+  }
+  free(D); free(DS); free(V); free(T);
+  return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sequential restatement of the reference (the CPU baseline)               */
+/* ------------------------------------------------------------------------ */
+/* quad::cell (cellpool.h:226-240): nine interleaved floats */
+typedef struct {
+  float height;
+  float massflow, momentumx, momentumy, momentumz;
+  float massflow_track, momentumx_track, momentumy_track, momentumz_track;
+} or_cell;
+
+typedef struct {
+  const or_params* p;
+  int32_t nx, ny;
+  or_cell* cells;
+  float* sed;             /* sediment layer (layers == 2) */
+  v3 ps, nps;
+} or_seqmap;
+
+static inline or_cell* seq_get(or_seqmap* m, int32_t x, int32_t y)
+{
+  if (m->p->periodic) { x = wrapi(x, m->nx); y = wrapi(y, m->ny); }
+  else if (x < 0 || y < 0 || x >= m->nx || y >= m->ny) return NULL;   /* slice::get */
+  return &m->cells[(int64_t)x * m->ny + y];
+}
+static inline float seq_h(or_seqmap* m, int32_t x, int32_t y)
+{
+  or_cell* c = seq_get(m, x, y);
+  return c ? c->height : 0.0f;
+}
+
+static v3 seq_normal(or_seqmap* m, int32_t x, int32_t y)
+{
+  v3 n = V3(0.0f, 0.0f, 0.0f);
+  const v3 s = V3(1.0f, m->p->mapscale, 1.0f);
+  const int per = m->p->periodic;
+  const int32_t nx = m->nx, ny = m->ny;
+  if (!oob(x + 1, y + 1, nx, ny, per))
+    n = vadd(n, cross3(vmul(s, V3(0.0f, seq_h(m, x, y + 1) - seq_h(m, x, y), 1.0f)),
+                       vmul(s, V3(1.0f, seq_h(m, x + 1, y) - seq_h(m, x, y), 0.0f))));
+  if (!oob(x - 1, y - 1, nx, ny, per))
+    n = vadd(n, cross3(vmul(s, V3(0.0f, seq_h(m, x, y - 1) - seq_h(m, x, y), -1.0f)),
+                       vmul(s, V3(-1.0f, seq_h(m, x - 1, y) - seq_h(m, x, y), 0.0f))));
+  if (!oob(x + 1, y - 1, nx, ny, per))
+    n = vadd(n, cross3(vmul(s, V3(1.0f, seq_h(m, x + 1, y) - seq_h(m, x, y), 0.0f)),
+                       vmul(s, V3(0.0f, seq_h(m, x, y - 1) - seq_h(m, x, y), -1.0f))));
+  if (!oob(x - 1, y + 1, nx, ny, per))
+    n = vadd(n, cross3(vmul(s, V3(-1.0f, seq_h(m, x - 1, y) - seq_h(m, x, y), 0.0f)),
+                       vmul(s, V3(0.0f, seq_h(m, x, y + 1) - seq_h(m, x, y), 1.0f))));
+  if (len3(n) > 0.0f) n = norm3(n);
+  return n;
+}
+
+/* World::cascade (world.hpp:79-153), writing float heights in place */
+static void seq_cascade(or_seqmap* m, int32_t x, int32_t y)
+{
+  struct { int32_t x, y; float h; float d; } sn[8];
+  int num = 0;
+  const or_params* p = m->p;
+  for (int k = 0; k < 8; k++) {
+    int32_t qx = x + OFFX[k], qy = y + OFFY[k];
+    if (oob(qx, qy, m->nx, m->ny, p->periodic)) continue;
+    float fxk = (float)OFFX[k], fyk = (float)OFFY[k];
+    sn[num].x = qx; sn[num].y = qy;
+    sn[num].h = seq_get(m, qx, qy)->height;
+    sn[num].d = sqrtf(fxk * fxk + fyk * fyk);
+    num++;
+  }
+  for (int i = 1; i < num; i++) {
+    __typeof__(sn[0]) v = sn[i];
+    int j = i - 1;
+    while (j >= 0 && v.h > sn[j].h) { sn[j + 1] = sn[j]; j--; }
+    sn[j + 1] = v;
+  }
+  or_cell* c = seq_get(m, x, y);
+  int64_t ci = c - m->cells;
+  for (int i = 0; i < num; i++) {
+    float diff = seq_get(m, x, y)->height - sn[i].h;
+    if (diff == 0.0f) continue;
+    float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;
+    if (excess <= 0.0f) continue;
+    float t = p->settling * excess / 2.0f;
+    or_cell* nc = seq_get(m, sn[i].x, sn[i].y);
+    int64_t ni = nc - m->cells;
+    if (p->layers == 2) {
+      float avail = diff > 0.0f ? m->sed[ci] : m->sed[ni];
+      if (t > avail) t = avail;
+      if (!(t > 0.0f)) continue;
+    }
+    if (diff > 0.0f) {
+      c->height -= t; nc->height += t;
+      if (p->layers == 2) { m->sed[ci] -= t; m->sed[ni] += t; }
+    } else {
+      c->height += t; nc->height -= t;
+      if (p->layers == 2) { m->sed[ci] += t; m->sed[ni] -= t; }
+    }
+  }
+}
+
+typedef struct { v3 pos, speed; float sediment; int32_t age; } or_seqwind;
+
+/* Wind::fly (wind.hpp:28-117), sequential: glibc rand(), libm expf */
+static int seq_fly(or_seqmap* m, or_seqwind* w)
+{
+  const or_params* p = m->p;
+  float rx = roundf(w->pos.x), rz = roundf(w->pos.z);
+  if (!(fabsf(rx) < 1073741824.0f) || !(fabsf(rz) < 1073741824.0f)) return 0;
+  int32_t ix = (int32_t)rx, iy = (int32_t)rz;
+  or_cell* cell = seq_get(m, ix, iy);
+  if (cell == NULL) return 0;
+  v3 n = seq_normal(m, ix, iy);
+  if (w->age == 0 || w->pos.y < cell->height) w->pos.y = cell->height;
+  if (w->age++ > p->max_age) return 0;
+  float hfac = expf(-(w->pos.y - cell->height) / p->boundarylayer);
+  if (hfac < 0.0f) hfac = 0.0f;
+  float shadow = dot3(m->nps, n);
+  if (shadow < 0.0f) shadow = 0.0f;
+  shadow = 1.0f - shadow;
+  v3 sp = w->speed;
+  sp = vadd(sp, smul(0.05f, vsub(mulsv(m->ps, 0.1f + 0.9f * shadow), sp)));
+  if (w->pos.y > cell->height) sp.y -= p->gravity * w->sediment;
+  float collision = -dot3(norm3(sp), n);
+  if (collision < 0.0f) collision = 0.0f;
+  v3 rspeed = cross3(n, cross3(smul(1.0f - collision, sp), n));
+  sp = vadd(sp, smul(0.9f, vsub(smul(shadow, mix3(m->ps, rspeed, shadow * hfac)), sp)));
+  {
+    /* vec3(rand()%1001, rand()%1001, rand()%1001): drawn left to right here */
+    int32_t r0 = rand() % 1001, r1 = rand() % 1001, r2 = rand() % 1001;
+    float k = 0.1f * hfac * collision;
+    v3 t = vsubs(V3((float)r0, (float)r1, (float)r2), 500.0f);
+    sp = vadd(sp, vdivs(smul(k, t), 500.0f));
+  }
+  sp = mulsv(sp, (float)(1.0 - 0.3 * (double)w->sediment));
+  w->pos = vadd(w->pos, sp);
+  cell->momentumx_track += sp.x;
+  cell->momentumy_track += sp.y;
+  cell->momentumz_track += sp.z;
+  cell->massflow_track += w->sediment;
+  float force = -dot3(norm3(sp), n) * len3(sp);
+  if (force < 0.0f) force = 0.0f;
+  float lift = (1.0f - collision) * len3(sp);
+  float capacity = force * hfac + 0.02f * lift * hfac;
+  float diff = capacity - w->sediment;
+  int64_t ci = cell - m->cells;
+  if (p->layers == 2 && diff > 0.0f && m->sed[ci] <= 0.0f) {
+    float q = p->abrasion * diff;
+    cell->height -= q;
+    w->sediment += q;
+  } else {
+    cell->height -= p->suspension * diff;
+    if (p->layers == 2) {
+      m->sed[ci] -= p->suspension * diff;
+      if (m->sed[ci] < 0.0f) m->sed[ci] = 0.0f;
+    }
+    w->sediment += p->suspension * diff;
+  }
+  w->speed = sp;
+  seq_cascade(m, ix, iy);
+  return 1;
+}
+
+void or_srand(uint32_t seed) { srand(seed); }
+
+/*
+ * World::erode(cycles) as the reference runs it (world.hpp:36-77): one
+ * particle after another, each marching to death on the live float map.
+ *   height   float[nx*ny]  in/out
+ *   sed      float[nx*ny]  sediment layer (layers == 2) or NULL
+ *   ema      float[4*nx*ny] massflow, momentum x/y/z EMA fields or NULL
+ * Returns the number of particle-steps executed.  Call or_srand(seed) first,
+ * as SimpleWindErosion.cpp:27-34 calls srand.
+ */
+int64_t or_erode_seq(const or_params* p, int32_t nx, int32_t ny, float* height, float* sed,
+                     float* ema, int64_t particles, int32_t n_cycles)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  or_cell* cells = (or_cell*)calloc(ncell, sizeof(or_cell));
+  if (!cells) return -1;
+  for (int64_t c = 0; c < ncell; c++) {
+    cells[c].height = height[c];
+    if (ema) {
+      cells[c].massflow = ema[c];
+      cells[c].momentumx = ema[ncell + c];
+      cells[c].momentumy = ema[2 * ncell + c];
+      cells[c].momentumz = ema[3 * ncell + c];
+    }
+  }
+  or_seqmap m;
+  m.p = p; m.nx = nx; m.ny = ny; m.cells = cells; m.sed = sed;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);
+  int64_t steps = 0;
+  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+    for (int64_t c = 0; c < ncell; c++) {
+      cells[c].massflow_track = 0; cells[c].momentumx_track = 0;
+      cells[c].momentumy_track = 0; cells[c].momentumz_track = 0;
+    }
+    for (int64_t i = 0; i < particles; i++) {
+      int32_t x = rand() % nx;
+      int32_t y = rand() % ny;
+      int32_t fx = rand() % 1000;
+      int32_t fy = rand() % 1000;
+      or_seqwind w;
+      w.pos = V3((float)x + (float)fx / 1000.0f, 0.0f, (float)y + (float)fy / 1000.0f);
+      w.speed = V3(0.0f, 0.0f, 0.0f);
+      w.sediment = 0.0f;
+      w.age = 0;
+      while (seq_fly(&m, &w)) steps++;
+    }
+    for (int64_t c = 0; c < ncell; c++) {
+      or_cell* e = &cells[c];
+      e->massflow = (1.0f - p->lrate) * e->massflow + p->lrate * e->massflow_track;
+      e->momentumx = (1.0f - p->lrate) * e->momentumx + p->lrate * e->momentumx_track;
+      e->momentumy = (1.0f - p->lrate) * e->momentumy + p->lrate * e->momentumy_track;
+      e->momentumz = (1.0f - p->lrate) * e->momentumz + p->lrate * e->momentumz_track;
+    }
+  }
+  for (int64_t c = 0; c < ncell; c++) {
+    height[c] = cells[c].height;
+    if (ema) {
+      ema[c] = cells[c].massflow;
+      ema[ncell + c] = cells[c].momentumx;
+      ema[2 * ncell + c] = cells[c].momentumy;
+      ema[3 * ncell + c] = cells[c].momentumz;
+    }
+  }
+  free(cells);
+  return steps;
+}
+
+int32_t or_params_size(void) { return (int32_t)sizeof(or_params); }

@@ -1,0 +1,65 @@
+// swe_kernels.h -- argument blocks and launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/swe.h"
+
+namespace swe {
+
+// carried-off mass is spread over this many counters to avoid one hot address
+constexpr int kLostSlots = 256;
+
+struct MarchArgs {
+  const float* S;      // height snapshot of the window
+  const float* SS;     // sediment snapshot (layers == 2)
+  int64_t* D;          // height pickup deltas
+  int64_t* DS;         // sediment pickup deltas (layers == 2)
+  uint32_t* V;         // cascade events per cell
+  int64_t* T;          // 4 x ncell tracks (track)
+  float4* P0;          // pos.xyz, sediment
+  float4* P1;          // speed.xyz, age bits
+  int64_t* L;          // fixed-point load
+  int32_t* LC;         // last pre-move cell
+  int64_t* lost;       // kLostSlots counters
+  const uint64_t* cycle;
+  int64_t n;           // particles
+  int64_t ncell;
+  int32_t nx, ny;
+  float rcp_nx, rcp_ny;
+  int32_t R;           // fly() calls in this window
+  float mapscale;
+  float ps[3];         // pspeed
+  float nps[3];        // normalize(pspeed)
+  swe_params p;
+};
+
+struct RelaxArgs {
+  const float* S;      // window snapshot (read, with halo)
+  const float* SS;
+  const uint32_t* V;   // window event counts (read, with halo)
+  uint32_t* v_clear;   // the other event buffer, zeroed for the next window
+  int64_t* zm;         // master height (read-modify-write)
+  int64_t* dz;         // pickup deltas (read, then zeroed)
+  float* zs_out;       // next snapshot
+  int64_t* sm;
+  int64_t* ds;
+  float* ss_out;
+  int32_t nx, ny;
+  float rcp_nx, rcp_ny;
+  float maxdiff, settling;
+};
+
+void launch_march(const MarchArgs& a, bool first, hipStream_t s);
+void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
+void launch_zero64(int64_t* p, int64_t n, hipStream_t s);
+void launch_zero32(uint32_t* p, int64_t n, hipStream_t s);
+void launch_from_f32(const float* in, int64_t* master, float* snap, int64_t n, hipStream_t s);
+void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s);
+void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s);
+void launch_bump(uint64_t* cycle, hipStream_t s);
+void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic,
+                    hipStream_t s);
+
+}  // namespace swe

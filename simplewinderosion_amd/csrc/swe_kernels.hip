@@ -1,0 +1,570 @@
+// swe_kernels.hip -- gfx950 kernels of the erosion hot path.
+//
+//   k_march   one particle per lane: up to R Wind::fly() calls
+//             (wind.hpp:28-117) against the window's float32 snapshot,
+//             int64 fixed-point pickup deltas and per-cell cascade-event
+//             counts by device-scope atomics.
+//   k_relax   LDS-tiled stencil: every cascade event of the window
+//             (world.hpp:79-153) as visits x transfers, summed in int64,
+//             then master += deltas and the next snapshot, in one pass.
+//   k_ema     World::erode's EMA flow fields (world.hpp:69-76).
+//   small fills/conversions for upload, download and zeroing.
+//
+// Layout in HBM (all SoA, cell (x,y) at x*ny + y):
+//   zm int64 master height, zs[2] float32 snapshot (ping-pong),
+//   dz int64 window pickup deltas, vis[2] uint32 cascade events (ping-pong),
+//   sm/ss/ds the same for the loose-sediment layer (layers == 2),
+//   trk 4 x int64 tracks, ema 4 x float32 fields,
+//   particles: P0 float4 (pos.x, pos.y, pos.z, sediment),
+//              P1 float4 (speed.x, speed.y, speed.z, age bits),
+//              ledger int64 (fixed-point load), lastc int32.
+#include "swe_kernels.h"
+
+#include "swe_device.hpp"
+
+namespace swe {
+
+// ---------------------------------------------------------------------------
+// Particle march
+// ---------------------------------------------------------------------------
+template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool FIRST>
+__global__ __launch_bounds__(256) void k_march(MarchArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int64_t lost_here = 0;
+  if (i < a.n) {
+    const int32_t nx = a.nx, ny = a.ny;
+    const uint64_t pk = particle_key(cycle_key(a.p.seed, *a.cycle), (uint64_t)i);
+    float px, py, pz, sed, sx, sy, sz;
+    int32_t age;
+    int64_t ledger;
+    int32_t lastc;
+    if (FIRST) {
+      // world.hpp:52-54,62 / wind.hpp:6-13 on the counter RNG
+      const uint64_t ra = mix64(pk + 1 * kGolden);
+      const uint64_t rb = mix64(pk + 2 * kGolden);
+      const int32_t ix = (int32_t)((uint32_t)ra % (uint32_t)nx);
+      const int32_t iy = (int32_t)((uint32_t)(ra >> 32) % (uint32_t)ny);
+      const int32_t fxp = (int32_t)((uint32_t)rb % 1000u);
+      const int32_t fyp = (int32_t)((uint32_t)(rb >> 32) % 1000u);
+      px = (float)ix + (float)fxp / 1000.0f;
+      pz = (float)iy + (float)fyp / 1000.0f;
+      py = 0.0f;
+      sed = 0.0f;
+      sx = sy = sz = 0.0f;
+      age = 0;
+      ledger = 0;
+      lastc = 0;
+    } else {
+      const float4 q0 = a.P0[i];
+      const float4 q1 = a.P1[i];
+      px = q0.x; py = q0.y; pz = q0.z; sed = q0.w;
+      sx = q1.x; sy = q1.y; sz = q1.z;
+      age = (int32_t)f2u(q1.w);
+      ledger = a.L[i];
+      lastc = a.LC[i];
+    }
+    const f3 ps = mk(a.ps[0], a.ps[1], a.ps[2]);
+    const f3 nps = mk(a.nps[0], a.nps[1], a.nps[2]);
+    for (int32_t s = 0; s < a.R && age >= 0; s++) {
+      // wind.hpp:30 ipos = round(vec2(pos.x, pos.z)), half away from zero
+      const float rx = __builtin_roundf(px), rz = __builtin_roundf(pz);
+      int64_t dead_cell = -1;
+      bool die = false;
+      if (!(__builtin_fabsf(rx) < 1073741824.0f) || !(__builtin_fabsf(rz) < 1073741824.0f)) {
+        die = true;
+      }
+      int32_t ix = die ? 0 : (int32_t)rx, iy = die ? 0 : (int32_t)rz;
+      if (!PERIODIC && !die && (ix < 0 || iy < 0 || ix >= nx || iy >= ny)) die = true;  // :32-38
+      int32_t cx = 0, cy = 0;
+      f3 n = mk(0.0f, 0.0f, 0.0f);
+      float h = 0.0f;
+      if (!die) {
+        if (PERIODIC) {
+          cx = wrap(ix, nx, a.rcp_nx);
+          cy = wrap(iy, ny, a.rcp_ny);
+        } else {
+          cx = ix;
+          cy = iy;
+        }
+        // five-point stencil of quad::_normal (cellpool.h:200-223)
+        const int64_t row = (int64_t)cx * ny;
+        h = a.S[row + cy];
+        float hyp = 0.0f, hym = 0.0f, hxp = 0.0f, hxm = 0.0f;
+        bool g1, g2, g3, g4;
+        if (PERIODIC) {
+          const int32_t xp = cx + 1 == nx ? 0 : cx + 1;
+          const int32_t xm = cx == 0 ? nx - 1 : cx - 1;
+          const int32_t yp = cy + 1 == ny ? 0 : cy + 1;
+          const int32_t ym = cy == 0 ? ny - 1 : cy - 1;
+          hyp = a.S[row + yp];
+          hym = a.S[row + ym];
+          hxp = a.S[(int64_t)xp * ny + cy];
+          hxm = a.S[(int64_t)xm * ny + cy];
+          g1 = g2 = g3 = g4 = true;
+        } else {
+          const bool bxp = cx + 1 < nx, bxm = cx > 0, byp = cy + 1 < ny, bym = cy > 0;
+          g1 = bxp && byp;
+          g2 = bxm && bym;
+          g3 = bxp && bym;
+          g4 = bxm && byp;
+          if (byp) hyp = a.S[row + cy + 1];
+          if (bym) hym = a.S[row + cy - 1];
+          if (bxp) hxp = a.S[row + ny + cy];
+          if (bxm) hxm = a.S[row - ny + cy];
+        }
+        n = surface_normal(h, hyp, hxp, hym, hxm, g1, g2, g3, g4, a.mapscale);
+        if (age == 0 || py < h) py = h;  // :42-43
+        if (age > a.p.max_age) {         // :45-46
+          die = true;
+          dead_cell = row + cy;
+        }
+      }
+      if (die) {
+        if (DEPOSIT) {
+          if (ledger != 0) {
+            const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)lastc;
+            atomicAdd((unsigned long long*)&a.D[c], (unsigned long long)ledger);
+            if (LAYERS == 2) atomicAdd((unsigned long long*)&a.DS[c], (unsigned long long)ledger);
+          }
+        } else {
+          lost_here += ledger;
+        }
+        age = -1;
+        break;
+      }
+      const int32_t call = age;
+      age = age + 1;
+      const int64_t cell = (int64_t)cx * ny + cy;
+
+      float hfac = exp_f32(-(py - h) / a.p.boundarylayer);  // :50-52
+      if (hfac < 0.0f) hfac = 0.0f;
+
+      float shadow = dot(nps, n);  // :56-59
+      if (shadow < 0.0f) shadow = 0.0f;
+      shadow = 1.0f - shadow;
+
+      f3 sp = mk(sx, sy, sz);
+      sp = add(sp, smul(0.05f, sub(muls(ps, 0.1f + 0.9f * shadow), sp)));  // :61
+
+      if (py > h) sp.y -= a.p.gravity * sed;  // :65-66
+
+      float collision = -dot(normalize(sp), n);  // :70-71
+      if (collision < 0.0f) collision = 0.0f;
+
+      const f3 rspeed = cross(n, cross(smul(1.0f - collision, sp), n));  // :75
+
+      sp = add(sp, smul(0.9f, sub(smul(shadow, mix(ps, rspeed, shadow * hfac)), sp)));  // :79
+
+      {  // :83 turbulence, three draws in [0,1000] for this call
+        const uint64_t t = mix64(pk + (uint64_t)(call + 3) * kGolden);
+        const float r0 = (float)(int32_t)((t & 0x1fffffull) % 1001ull);
+        const float r1 = (float)(int32_t)(((t >> 21) & 0x1fffffull) % 1001ull);
+        const float r2 = (float)(int32_t)(((t >> 42) & 0x1fffffull) % 1001ull);
+        const float k = 0.1f * hfac * collision;
+        sp = add(sp, divs(smul(k, subs(mk(r0, r1, r2), 500.0f)), 500.0f));
+      }
+
+      sp = muls(sp, (float)(1.0 - 0.3 * (double)sed));  // :87 (double, then float)
+
+      px += sp.x;  // :91
+      py += sp.y;
+      pz += sp.z;
+
+      if (TRACK) {  // :93-96 on the pre-move cell
+        atomicAdd((unsigned long long*)&a.T[cell], (unsigned long long)fx_from_f32(sed));
+        atomicAdd((unsigned long long*)&a.T[a.ncell + cell], (unsigned long long)fx_from_f32(sp.x));
+        atomicAdd((unsigned long long*)&a.T[2 * a.ncell + cell], (unsigned long long)fx_from_f32(sp.y));
+        atomicAdd((unsigned long long*)&a.T[3 * a.ncell + cell], (unsigned long long)fx_from_f32(sp.z));
+      }
+
+      float force = -dot(normalize(sp), n) * length(sp);  // :100-102
+      if (force < 0.0f) force = 0.0f;
+      const float lift = (1.0f - collision) * length(sp);  // :104
+      const float capacity = force * hfac + 0.02f * lift * hfac;  // :106
+
+      const float diff = capacity - sed;  // :110-112
+      float q;
+      if (LAYERS == 2 && diff > 0.0f && a.SS[cell] <= 0.0f) {
+        q = a.p.abrasion * diff;  // bare bedrock: abrasion rate, sediment layer untouched
+        const int64_t qf = fx_from_f32(q);
+        atomicAdd((unsigned long long*)&a.D[cell], (unsigned long long)(-qf));
+        ledger += qf;
+      } else {
+        q = a.p.suspension * diff;
+        const int64_t qf = fx_from_f32(q);
+        atomicAdd((unsigned long long*)&a.D[cell], (unsigned long long)(-qf));
+        if (LAYERS == 2) atomicAdd((unsigned long long*)&a.DS[cell], (unsigned long long)(-qf));
+        ledger += qf;
+      }
+      sed += q;
+      sx = sp.x;
+      sy = sp.y;
+      sz = sp.z;
+      lastc = (int32_t)cell;
+      atomicAdd(&a.V[cell], 1u);  // :114 World::cascade(ipos), applied by k_relax
+    }
+    a.P0[i] = make_float4(px, py, pz, sed);
+    a.P1[i] = make_float4(sx, sy, sz, u2f((uint32_t)age));
+    a.L[i] = ledger;
+    a.LC[i] = lastc;
+  }
+  if (!DEPOSIT) {
+    // carried-off mass: one atomic per wave into one of kLostSlots counters
+    for (int off = 32; off > 0; off >>= 1) lost_here += __shfl_xor(lost_here, off, 64);
+    if ((threadIdx.x & 63) == 0 && lost_here != 0) {
+      const uint32_t slot = (blockIdx.x * 4 + (threadIdx.x >> 6)) & (kLostSlots - 1);
+      atomicAdd((unsigned long long*)&a.lost[slot], (unsigned long long)lost_here);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cascade relax + apply (LDS-tiled stencil)
+// ---------------------------------------------------------------------------
+// Tile of kTX x kTY cells (y contiguous).  LDS holds the snapshot with a
+// 2-cell halo, the event counts with a 1-cell halo, and for every cell of the
+// 1-ring region the fixed-point amount each of its 8 neighbours receives
+// from its window's cascade events (visits x per-event transfer).
+constexpr int kTX = 8;
+constexpr int kTY = 64;
+constexpr int kHX = kTX + 4, kHY = kTY + 4;  // snapshot region
+constexpr int kRX = kTX + 2, kRY = kTY + 2;  // ring region
+constexpr int kRelaxThreads = 256;
+
+// world.hpp:83-92 neighbour order
+constexpr int32_t c_offx[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+constexpr int32_t c_offy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+// One cascade event (world.hpp:79-153) on the 3x3 neighbourhood h[] (index
+// 4 = centre, neighbour k at offset k of the fixed order), valid[k] false for
+// out-of-map neighbours.  amt[k] = signed fixed-point amount neighbour k
+// receives.  s[] is the sediment snapshot (LAYERS == 2).
+template <int LAYERS>
+__device__ __forceinline__ void cascade_event(const float hn[8], const float sn[8], float hc, float sc,
+                                              uint32_t validmask, float maxdiff, float settling,
+                                              int64_t amt[8]) {
+  // stable sort by height descending == sort on (h desc, offset asc);
+  // invalid neighbours carry -inf and are skipped.
+  float key[8];
+  int32_t idx[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
+    idx[k] = k;
+    amt[k] = 0;
+  }
+#define SWE_CAS(i, j)                                                                  \
+  {                                                                                    \
+    const bool sw = key[j] > key[i] || (key[j] == key[i] && idx[j] < idx[i]);          \
+    const float tk = sw ? key[j] : key[i];                                             \
+    const int32_t ti = sw ? idx[j] : idx[i];                                           \
+    key[j] = sw ? key[i] : key[j];                                                     \
+    idx[j] = sw ? idx[i] : idx[j];                                                     \
+    key[i] = tk;                                                                       \
+    idx[i] = ti;                                                                       \
+  }
+  SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
+  SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
+  SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
+  SWE_CAS(2, 4) SWE_CAS(3, 5)
+  SWE_CAS(1, 4) SWE_CAS(3, 6)
+  SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
+#undef SWE_CAS
+  const float dd = __builtin_sqrtf(2.0f);
+#pragma unroll
+  for (int r = 0; r < 8; r++) {
+    const int32_t k = idx[r];
+    if (!((validmask >> k) & 1u)) continue;
+    const float hk = key[r];
+    const float diff = hc - hk;
+    if (diff == 0.0f) continue;
+    const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
+    const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
+    if (excess <= 0.0f) continue;
+    float t = settling * excess / 2.0f;
+    if (LAYERS == 2) {
+      float sk = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 8; j++) sk = (j == k) ? sn[j] : sk;
+      const float avail = diff > 0.0f ? sc : sk;
+      if (t > avail) t = avail;
+      if (!(t > 0.0f)) continue;
+    }
+    const int64_t tf = fx_from_f32(t);
+    int64_t v;
+    if (diff > 0.0f) {
+      hc -= t;
+      sc -= t;
+      v = tf;
+    } else {
+      hc += t;
+      sc += t;
+      v = -tf;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++) amt[j] = (j == k) ? v : amt[j];
+  }
+}
+
+template <bool PERIODIC, int LAYERS>
+__global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
+  __shared__ float sh[kHX * kHY];
+  __shared__ float shs[LAYERS == 2 ? kHX * kHY : 1];
+  __shared__ uint32_t sv[kRX * kRY];
+  __shared__ int64_t sx[kRX * kRY * 8];
+
+  const int32_t nx = a.nx, ny = a.ny;
+  // XCD-aware tile order: consecutive workgroups land on different XCDs, so
+  // hand each XCD a contiguous band of tiles (shares halo lines in its L2).
+  const uint32_t nb = gridDim.x;
+  const uint32_t b = blockIdx.x;
+  const uint32_t per = (nb + 7) / 8;
+  uint32_t t = (b % 8) * per + b / 8;
+  if (t >= nb) t = b;  // uneven tail: keep the identity mapping
+  const int32_t tiles_y = (ny + kTY - 1) / kTY;
+  const int32_t x0 = (int32_t)(t / tiles_y) * kTX;
+  const int32_t y0 = (int32_t)(t % tiles_y) * kTY;
+
+  // load snapshot (2-halo) and event counts (1-halo)
+  for (int e = threadIdx.x; e < kHX * kHY; e += kRelaxThreads) {
+    const int32_t lx = e / kHY, ly = e % kHY;
+    int32_t gx = x0 + lx - 2, gy = y0 + ly - 2;
+    float v = 0.0f, vs = 0.0f;
+    if (PERIODIC) {
+      gx = wrap(gx, nx, a.rcp_nx);
+      gy = wrap(gy, ny, a.rcp_ny);
+      v = a.S[(int64_t)gx * ny + gy];
+      if (LAYERS == 2) vs = a.SS[(int64_t)gx * ny + gy];
+    } else if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
+      v = a.S[(int64_t)gx * ny + gy];
+      if (LAYERS == 2) vs = a.SS[(int64_t)gx * ny + gy];
+    }
+    sh[e] = v;
+    if (LAYERS == 2) shs[e] = vs;
+  }
+  for (int e = threadIdx.x; e < kRX * kRY; e += kRelaxThreads) {
+    const int32_t lx = e / kRY, ly = e % kRY;
+    int32_t gx = x0 + lx - 1, gy = y0 + ly - 1;
+    uint32_t v = 0;
+    if (PERIODIC) {
+      gx = wrap(gx, nx, a.rcp_nx);
+      gy = wrap(gy, ny, a.rcp_ny);
+      v = a.V[(int64_t)gx * ny + gy];
+    } else if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) {
+      v = a.V[(int64_t)gx * ny + gy];
+    }
+    sv[e] = v;
+  }
+  __syncthreads();
+
+  // cascade events of every ring cell with visits
+  for (int e = threadIdx.x; e < kRX * kRY; e += kRelaxThreads) {
+    const uint32_t v = sv[e];
+    int64_t* out = &sx[e * 8];
+    if (v == 0) {
+#pragma unroll
+      for (int k = 0; k < 8; k++) out[k] = 0;
+      continue;
+    }
+    const int32_t lx = e / kRY, ly = e % kRY;  // ring coords; snapshot coords +1
+    const int32_t gx = x0 + lx - 1, gy = y0 + ly - 1;
+    float hn[8], sn[8];
+    uint32_t valid = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const int32_t hx = lx + 1 + c_offx[k], hy = ly + 1 + c_offy[k];
+      hn[k] = sh[hx * kHY + hy];
+      sn[k] = LAYERS == 2 ? shs[hx * kHY + hy] : 0.0f;
+      bool ok = true;
+      if (!PERIODIC) {
+        const int32_t qx = gx + c_offx[k], qy = gy + c_offy[k];
+        ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
+      }
+      valid |= (ok ? 1u : 0u) << k;
+    }
+    const float hc = sh[(lx + 1) * kHY + ly + 1];
+    const float sc = LAYERS == 2 ? shs[(lx + 1) * kHY + ly + 1] : 0.0f;
+    int64_t amt[8];
+    cascade_event<LAYERS>(hn, sn, hc, sc, valid, a.maxdiff, a.settling, amt);
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[k] = (int64_t)v * amt[k];
+  }
+  __syncthreads();
+
+  // gather: own events' centre share + what each neighbour's events sent here
+  for (int e = threadIdx.x; e < kTX * kTY; e += kRelaxThreads) {
+    const int32_t lx = e / kTY, ly = e % kTY;
+    const int32_t gx = x0 + lx, gy = y0 + ly;
+    if (gx >= nx || gy >= ny) continue;
+    const int32_t rc = (lx + 1) * kRY + (ly + 1);
+    int64_t cas = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) cas -= sx[rc * 8 + k];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      // neighbour y = x - off_k sends through its offset k to x
+      const int32_t ry = rc - c_offx[k] * kRY - c_offy[k];
+      cas += sx[ry * 8 + k];
+    }
+    const int64_t c = (int64_t)gx * ny + gy;
+    const int64_t z = a.zm[c] + a.dz[c] + cas;
+    a.zm[c] = z;
+    a.zs_out[c] = f32_from_fx(z);
+    a.dz[c] = 0;
+    a.v_clear[c] = 0;
+    if (LAYERS == 2) {
+      int64_t s = a.sm[c] + a.ds[c] + cas;
+      if (s < 0) s = 0;
+      a.sm[c] = s;
+      a.ss_out[c] = f32_from_fx(s);
+      a.ds[c] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ void k_zero64(int64_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+__global__ void k_zero32(uint32_t* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+// float -> fixed-point master + snapshot
+__global__ void k_from_f32(const float* in, int64_t* master, float* snap, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = fx_from_f32(in[i]);
+    master[i] = m;
+    snap[i] = f32_from_fx(m);
+  }
+}
+__global__ void k_snap(const int64_t* master, float* snap, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    snap[i] = f32_from_fx(master[i]);
+}
+// world.hpp:69-76: f = (1-lrate)*f + lrate*track, four fields
+__global__ void k_ema(float* ema, const int64_t* trk, int64_t n4, float lrate) {
+  const float a = 1.0f - lrate;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    ema[i] = a * ema[i] + lrate * f32_from_fx(trk[i]);
+}
+__global__ void k_bump(uint64_t* cycle) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) cycle[0] = cycle[0] + 1;
+}
+// quad::_normal for every cell of the snapshot (cellpool.h:200-223,478-480)
+template <bool PERIODIC>
+__global__ void k_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale) {
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t cx = (int32_t)(c / ny), cy = (int32_t)(c % ny);
+    const int64_t row = (int64_t)cx * ny;
+    const float h = S[c];
+    float hyp = 0.0f, hym = 0.0f, hxp = 0.0f, hxm = 0.0f;
+    bool g1, g2, g3, g4;
+    if (PERIODIC) {
+      const int32_t xp = cx + 1 == nx ? 0 : cx + 1, xm = cx == 0 ? nx - 1 : cx - 1;
+      const int32_t yp = cy + 1 == ny ? 0 : cy + 1, ym = cy == 0 ? ny - 1 : cy - 1;
+      hyp = S[row + yp];
+      hym = S[row + ym];
+      hxp = S[(int64_t)xp * ny + cy];
+      hxm = S[(int64_t)xm * ny + cy];
+      g1 = g2 = g3 = g4 = true;
+    } else {
+      const bool bxp = cx + 1 < nx, bxm = cx > 0, byp = cy + 1 < ny, bym = cy > 0;
+      g1 = bxp && byp;
+      g2 = bxm && bym;
+      g3 = bxp && bym;
+      g4 = bxm && byp;
+      if (byp) hyp = S[row + cy + 1];
+      if (bym) hym = S[row + cy - 1];
+      if (bxp) hxp = S[row + ny + cy];
+      if (bxm) hxm = S[row - ny + cy];
+    }
+    const f3 nn = surface_normal(h, hyp, hxp, hym, hxm, g1, g2, g3, g4, mapscale);
+    out[c] = nn.x;
+    out[n + c] = nn.y;
+    out[2 * n + c] = nn.z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int threads) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+template <bool P, bool T, int L, bool D>
+static void march_first(const MarchArgs& a, bool first, hipStream_t s) {
+  const unsigned g = (unsigned)((a.n + 255) / 256);
+  if (first)
+    hipLaunchKernelGGL((k_march<P, T, L, D, true>), dim3(g), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_march<P, T, L, D, false>), dim3(g), dim3(256), 0, s, a);
+}
+template <bool P, bool T, int L>
+static void march_dep(const MarchArgs& a, bool first, bool dep, hipStream_t s) {
+  if (dep) march_first<P, T, L, true>(a, first, s);
+  else march_first<P, T, L, false>(a, first, s);
+}
+template <bool P, bool T>
+static void march_lay(const MarchArgs& a, bool first, bool dep, int layers, hipStream_t s) {
+  if (layers == 2) march_dep<P, T, 2>(a, first, dep, s);
+  else march_dep<P, T, 1>(a, first, dep, s);
+}
+template <bool P>
+static void march_trk(const MarchArgs& a, bool first, bool dep, bool trk, int layers, hipStream_t s) {
+  if (trk) march_lay<P, true>(a, first, dep, layers, s);
+  else march_lay<P, false>(a, first, dep, layers, s);
+}
+
+void launch_march(const MarchArgs& a, bool first, hipStream_t s) {
+  if (a.n <= 0) return;
+  const bool dep = a.p.deposit_on_death != 0, trk = a.p.track != 0;
+  if (a.p.periodic) march_trk<true>(a, first, dep, trk, a.p.layers, s);
+  else march_trk<false>(a, first, dep, trk, a.p.layers, s);
+}
+
+void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s) {
+  const unsigned tiles = (unsigned)(((a.nx + kTX - 1) / kTX) * ((a.ny + kTY - 1) / kTY));
+  if (periodic) {
+    if (layers == 2) hipLaunchKernelGGL((k_relax<true, 2>), dim3(tiles), dim3(kRelaxThreads), 0, s, a);
+    else hipLaunchKernelGGL((k_relax<true, 1>), dim3(tiles), dim3(kRelaxThreads), 0, s, a);
+  } else {
+    if (layers == 2) hipLaunchKernelGGL((k_relax<false, 2>), dim3(tiles), dim3(kRelaxThreads), 0, s, a);
+    else hipLaunchKernelGGL((k_relax<false, 1>), dim3(tiles), dim3(kRelaxThreads), 0, s, a);
+  }
+}
+
+void launch_zero64(int64_t* p, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_zero64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+}
+void launch_zero32(uint32_t* p, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_zero32, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
+}
+void launch_from_f32(const float* in, int64_t* master, float* snap, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_from_f32, dim3(grid_for(n, 256)), dim3(256), 0, s, in, master, snap, n);
+}
+void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_snap, dim3(grid_for(n, 256)), dim3(256), 0, s, master, snap, n);
+}
+void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s) {
+  if (n4 > 0) hipLaunchKernelGGL(k_ema, dim3(grid_for(n4, 256)), dim3(256), 0, s, ema, trk, n4, lrate);
+}
+void launch_bump(uint64_t* cycle, hipStream_t s) {
+  hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, cycle);
+}
+void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic,
+                    hipStream_t s) {
+  const int64_t n = (int64_t)nx * ny;
+  if (periodic) hipLaunchKernelGGL((k_normals<true>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
+  else hipLaunchKernelGGL((k_normals<false>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
+}
+
+}  // namespace swe

@@ -1,0 +1,91 @@
+"""Named workloads of the erosion hot path (inputs and parameters).
+
+``reference``  the reference demo: 512x512 bounded map, reference defaults,
+               4096 particles per World::erode call
+               (SimpleWindErosion.cpp:217, cellpool.h:184-198).
+``cpu256``     256x256 periodic map, fBm height + 0.05 sediment layer,
+               wind normalize(1,0,1), suspension 1e-4, abrasion 1e-4,
+               roughness (maxdiff) 0.005, settling 0.01; 2,000 particles x
+               100 cycles; one snapshot window per cycle.
+``gpu1024``    1024x1024 periodic map, same recipe plus 8 square pyramids
+               (height 0.3-0.6, base 40-120 cells) in the height layer;
+               1,000,000 particles x 200 cycles; snapshot window of 16 fly()
+               calls (cascade gain 2 * visits-per-window * settling = 0.32).
+
+Surface height = fBm (+ pyramids) + sediment thickness: the library's
+height layer is the surface the reference's cell->height describes
+(cellpool.h:228), the sediment layer the loose part on top of bedrock.
+"""
+from .world import World, Wind, generate_terrain, glm_normalize
+
+WORKLOADS = {
+    "reference": dict(nx=512, ny=512, particles=4096, cycles=1, periodic=False, layers=1, window=0,
+                      params=dict(), terrain=None, sediment=None),
+    "cpu256": dict(nx=256, ny=256, particles=2000, cycles=100, periodic=True, layers=2, window=0,
+                   params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
+                               pspeed=glm_normalize((1.0, 0.0, 1.0))),
+                   terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=0), sediment=0.05),
+    "gpu1024": dict(nx=1024, ny=1024, particles=1_000_000, cycles=200, periodic=True, layers=2, window=16,
+                    params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
+                                pspeed=glm_normalize((1.0, 0.0, 1.0))),
+                    terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=8), sediment=0.05),
+}
+
+
+def make_inputs(name, seed=1, nx=None, ny=None):
+    """Surface height and sediment arrays (float32, [nx, ny]) of a workload."""
+    import numpy as np
+
+    w = WORKLOADS[name]
+    nx = nx or w["nx"]
+    ny = ny or w["ny"]
+    if w["terrain"] is None:
+        # cellpool.h:380-385,416-428: Gaussian bump at (W/4, H/2), scaled to [0, 0.5]
+        x = np.arange(nx, dtype=np.float32)[:, None] / np.float32(nx)
+        y = np.arange(ny, dtype=np.float32)[None, :] / np.float32(ny)
+        d2 = (x - np.float32(0.25)) ** 2 + (y - np.float32(0.5)) ** 2
+        h = np.exp(-d2 * np.float32(nx * 0.2)).astype(np.float32)
+        lo = min(0.0, float(h.min()))
+        h = (0.5 * (h - lo) / (float(h.max()) - lo)).astype(np.float32)
+        return h, None
+    t = w["terrain"]
+    base = generate_terrain(nx, ny, seed, octaves=t["octaves"], base_cycles=t["base_cycles"],
+                            n_pyramids=t["n_pyramids"])
+    sed = np.full((nx, ny), np.float32(w["sediment"]), np.float32)
+    return (base + sed).astype(np.float32), sed
+
+
+def apply(name, seed=1, **over):
+    """Set the Wind/World statics of a workload; return the per-map World options."""
+    w = WORKLOADS[name]
+    p = dict(w["params"])
+    p.update(over.pop("params", {}))
+    for k in ("suspension", "gravity", "boundarylayer", "pspeed"):
+        if k in p:
+            setattr(Wind, k, p.pop(k))
+    for k in ("lrate", "maxdiff", "settling"):
+        if k in p:
+            setattr(World, k, p.pop(k))
+    World.SEED = seed
+    opts = dict(periodic=w["periodic"], layers=w["layers"], window=w["window"], abrasion=p.pop("abrasion", 0.0))
+    opts.update(over)
+    return opts
+
+
+def params(name, seed=1, **over):
+    """swe_params of a workload (sets the statics; no device needed)."""
+    from .world import build_params
+
+    return build_params(**apply(name, seed, **over))
+
+
+def make_world(name, seed=1, nx=None, ny=None, device=0, **over):
+    """A World loaded with the workload's inputs and parameters."""
+    w = WORKLOADS[name]
+    nx = nx or w["nx"]
+    ny = ny or w["ny"]
+    opts = apply(name, seed, **over)
+    world = World(nx, ny, device=device, **opts)
+    h, s = make_inputs(name, seed, nx, ny)
+    world.upload(h, s)
+    return world

@@ -1,0 +1,212 @@
+"""Python mirror of the reference's simulation interface over libswe.so.
+
+``Wind`` carries the parameter statics of ``struct Wind`` (wind.hpp:13,15-26)
+and ``World`` those of ``class World`` (world.hpp:12-32) plus
+``World.erode(cycles)`` (world.hpp:21,36-77).  As in the reference the
+parameters are class-level statics read at each ``erode`` call; the map and
+the extensions (periodic boundary, sediment layer, snapshot window) belong to
+one ``World`` instance, which owns one device context.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+
+
+class Wind:
+    """Statics of struct Wind (wind.hpp:15-26); pspeed is the member default (wind.hpp:13)."""
+
+    maxAge = 512
+    boundarylayer = 2.0
+    suspension = 0.05
+    gravity = 0.1
+    pspeed = (1.0, 0.0, 0.0)  # glm::normalize(vec3(1,0,0))
+
+
+def glm_normalize(v):
+    """glm::normalize in float32 with glm's operation order: v * (1/sqrt(dot(v,v)))."""
+    v = np.asarray(v, dtype=np.float32)
+    d = np.float32(np.float32(v[0] * v[0]) + np.float32(v[1] * v[1])) + np.float32(v[2] * v[2])
+    r = np.float32(1.0) / np.sqrt(np.float32(d))
+    return tuple(float(np.float32(c * r)) for c in v)
+
+
+class SweError(RuntimeError):
+    pass
+
+
+def build_params(abrasion=0.0, mapscale=80.0, window=0, periodic=False, track=True, layers=1,
+                 deposit_on_death=False):
+    """swe_params from the Wind/World statics plus per-map options (no device needed)."""
+    p = capi.SweParams()
+    capi.load().swe_params_default(ctypes.byref(p))
+    p.update(
+        max_age=int(Wind.maxAge), boundarylayer=float(Wind.boundarylayer),
+        suspension=float(Wind.suspension), gravity=float(Wind.gravity), pspeed=Wind.pspeed,
+        lrate=float(World.lrate), maxdiff=float(World.maxdiff), settling=float(World.settling),
+        abrasion=float(abrasion), mapscale=float(mapscale), window=int(window), periodic=int(bool(periodic)),
+        track=int(bool(track)), layers=int(layers), deposit_on_death=int(bool(deposit_on_death)),
+        seed=int(World.SEED) & 0xFFFFFFFFFFFFFFFF,
+    )
+    return p
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)) if a is not None else None
+
+
+class World:
+    """class World (world.hpp:9-24) bound to one libswe context on one GPU."""
+
+    SEED = 0
+    lrate = 0.1
+    maxdiff = 0.005
+    settling = 1.0
+
+    def __init__(self, nx=512, ny=512, device=0, periodic=False, track=True, layers=1, window=0,
+                 abrasion=0.0, deposit_on_death=False, mapscale=80.0):
+        self.lib = capi.load()
+        self.nx, self.ny = int(nx), int(ny)
+        self.periodic = bool(periodic)
+        self.track = bool(track)
+        self.layers = int(layers)
+        self.window = int(window)
+        self.abrasion = float(abrasion)
+        self.deposit_on_death = bool(deposit_on_death)
+        self.mapscale = float(mapscale)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.swe_create(ctypes.byref(self.ctx), self.nx, self.ny, int(device))
+        if rc != 0:
+            raise SweError("swe_create(%d, %d, device=%d) failed with %d" % (nx, ny, device, rc))
+        self._push()
+
+    # -- parameters ---------------------------------------------------------
+    def options(self):
+        return dict(abrasion=self.abrasion, mapscale=self.mapscale, window=self.window,
+                    periodic=self.periodic, track=self.track, layers=self.layers,
+                    deposit_on_death=self.deposit_on_death)
+
+    def params(self):
+        return build_params(**self.options())
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.swe_last_error(self.ctx)
+            raise SweError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+    def _push(self):
+        p = self.params()
+        self._check(self.lib.swe_set_params(self.ctx, ctypes.byref(p)), "swe_set_params")
+
+    # -- World::erode --------------------------------------------------------
+    def erode(self, cycles, n_cycles=1):
+        """World::erode(cycles): `cycles` particles per map-cycle (world.hpp:36-77)."""
+        self._push()
+        self._check(self.lib.swe_erode(self.ctx, int(cycles), int(n_cycles)), "swe_erode")
+
+    def synchronize(self):
+        self._check(self.lib.swe_synchronize(self.ctx), "swe_synchronize")
+
+    # -- map transfer ----------------------------------------------------------
+    def upload(self, height, sediment=None):
+        h = np.ascontiguousarray(height, dtype=np.float32).reshape(self.nx * self.ny)
+        s = None if sediment is None else np.ascontiguousarray(sediment, dtype=np.float32).reshape(self.nx * self.ny)
+        self._check(self.lib.swe_upload_layers(self.ctx, _fp(h), _fp(s)), "swe_upload_layers")
+
+    def upload_master(self, height_fx, sediment_fx=None):
+        h = np.ascontiguousarray(height_fx, dtype=np.int64).reshape(self.nx * self.ny)
+        s = None if sediment_fx is None else np.ascontiguousarray(sediment_fx, dtype=np.int64).reshape(-1)
+        self._check(self.lib.swe_upload_master(self.ctx, _ip(h), _ip(s)), "swe_upload_master")
+
+    def upload_fields(self, massflow=None, momentum=None):
+        m = None if massflow is None else np.ascontiguousarray(massflow, dtype=np.float32).reshape(-1)
+        mo = None if momentum is None else np.ascontiguousarray(momentum, dtype=np.float32).reshape(-1)
+        self._check(self.lib.swe_upload_fields(self.ctx, _fp(m), _fp(mo)), "swe_upload_fields")
+
+    def download(self):
+        n = self.nx * self.ny
+        h = np.empty(n, np.float32)
+        s = np.empty(n, np.float32)
+        mf = np.empty(n, np.float32)
+        mo = np.empty(3 * n, np.float32)
+        self._check(self.lib.swe_download_layers(self.ctx, _fp(h), _fp(s), _fp(mf), _fp(mo)), "swe_download_layers")
+        shp = (self.nx, self.ny)
+        return {"height": h.reshape(shp), "sediment": s.reshape(shp), "massflow": mf.reshape(shp),
+                "momentum": mo.reshape((3,) + shp)}
+
+    def master(self):
+        n = self.nx * self.ny
+        h = np.empty(n, np.int64)
+        s = np.empty(n, np.int64)
+        self._check(self.lib.swe_download_master(self.ctx, _ip(h), _ip(s)), "swe_download_master")
+        return h.reshape(self.nx, self.ny), s.reshape(self.nx, self.ny)
+
+    def normals(self):
+        out = np.empty(3 * self.nx * self.ny, np.float32)
+        self._check(self.lib.swe_normals(self.ctx, _fp(out)), "swe_normals")
+        return out.reshape(3, self.nx, self.ny)
+
+    # -- audit, counters, timing -------------------------------------------------
+    def audit(self):
+        a = capi.SweAudit()
+        self._check(self.lib.swe_audit(self.ctx, ctypes.byref(a)), "swe_audit")
+        return {"height_sum": a.height_sum, "sediment_sum": a.sediment_sum, "lost": a.lost, "cycle": a.cycle}
+
+    @property
+    def cycle(self):
+        c = ctypes.c_uint64()
+        self._check(self.lib.swe_get_cycle(self.ctx, ctypes.byref(c)), "swe_get_cycle")
+        return c.value
+
+    @cycle.setter
+    def cycle(self, v):
+        self._check(self.lib.swe_set_cycle(self.ctx, int(v)), "swe_set_cycle")
+
+    def set_lost(self, v):
+        self._check(self.lib.swe_set_lost(self.ctx, int(v)), "swe_set_lost")
+
+    def set_timing(self, on):
+        self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")
+
+    def stats(self):
+        s = capi.SweStats()
+        self._check(self.lib.swe_stats(self.ctx, ctypes.byref(s)), "swe_stats")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def reset_stats(self):
+        self._check(self.lib.swe_reset_stats(self.ctx), "swe_reset_stats")
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.swe_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def generate_terrain(nx, ny, seed, octaves=6, base_cycles=4.0, n_pyramids=0, pyr_h=(0.3, 0.6),
+                     pyr_base=(40.0, 120.0)):
+    """Seeded periodic fBm terrain in [0,1] plus optional square pyramids (host-side)."""
+    lib = capi.load()
+    out = np.empty(int(nx) * int(ny), np.float32)
+    rc = lib.swe_generate_terrain(int(nx), int(ny), int(seed) & 0xFFFFFFFFFFFFFFFF, int(octaves),
+                                  float(base_cycles), int(n_pyramids), float(pyr_h[0]), float(pyr_h[1]),
+                                  float(pyr_base[0]), float(pyr_base[1]), _fp(out))
+    if rc != 0:
+        raise SweError("swe_generate_terrain failed (%d)" % rc)
+    return out.reshape(int(nx), int(ny))

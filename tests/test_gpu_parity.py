@@ -1,0 +1,160 @@
+"""GPU parity: libswe.so (through its C-ABI) against the batched oracle, bit for
+bit on the int64 master layers, the EMA fields and the carried-off mass."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from simplewinderosion_amd import World, Wind
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=None, **opts):
+    """Run the same workload on the GPU and in the oracle; return both states."""
+    rng = np.random.default_rng(map_seed)
+    h = height if height is not None else (rng.random((nx, ny)) * 0.5).astype(np.float32)
+    w = World(nx, ny, **{k: v for k, v in opts.items() if k in
+                         ("periodic", "track", "layers", "window", "abrasion", "deposit_on_death")})
+    w.upload(h, sed)
+    p = w.params()
+    p.seed = seed
+    # the World mirror pushes statics at erode(); apply seed through the static
+    World.SEED = seed
+    op = O.params_from(w.params())
+    st = O.OracleState(nx, ny, h, sediment=sed, params=op)
+    st.erode(particles, cycles)
+    w.erode(particles, n_cycles=cycles)
+    w.synchronize()
+    return w, st
+
+
+def assert_same(w, st):
+    zm, sm = w.master()
+    assert np.array_equal(zm.ravel(), st.zm), "height masters differ: %d cells" % int((zm.ravel() != st.zm).sum())
+    if st.p.layers == 2:
+        assert np.array_equal(sm.ravel(), st.sm), "sediment masters differ"
+    d = w.download()
+    assert np.array_equal(d["height"].ravel(), O.unfx_vec(st.zm))
+    if st.p.track:
+        n = w.nx * w.ny
+        assert np.array_equal(d["massflow"].ravel(), st.ema[:n]), "massflow differs"
+        assert np.array_equal(d["momentum"].ravel(), st.ema[n:]), "momentum differs"
+    a = w.audit()
+    assert a["lost"] == st.lost.value
+    assert a["height_sum"] == int(st.zm.sum())
+    assert a["cycle"] == st.cycle.value
+
+
+CASES = [
+    # name, nx, ny, particles, cycles, opts
+    ("periodic-ref", 32, 32, 200, 2, dict(periodic=True)),
+    ("bounded-ref", 48, 40, 300, 2, dict(periodic=False, window=16)),
+    ("bounded-window1", 20, 24, 100, 1, dict(periodic=False, window=1)),
+    ("periodic-notrack", 40, 33, 300, 2, dict(periodic=True, track=False, window=9)),
+    ("deposit", 32, 32, 300, 2, dict(periodic=True, deposit_on_death=True, window=16)),
+    ("rect-nonpow2", 37, 129, 700, 2, dict(periodic=True, window=5)),
+    ("tall-bounded", 130, 9, 400, 1, dict(periodic=False, window=32)),
+]
+
+
+@pytest.mark.parametrize("name,nx,ny,particles,cycles,opts", CASES, ids=[c[0] for c in CASES])
+def test_parity_single_layer(statics, name, nx, ny, particles, cycles, opts):
+    say("case", name, nx, ny, particles, cycles, opts)
+    w, st = run_pair(nx, ny, particles, cycles, seed=17, **opts)
+    assert_same(w, st)
+    w.close()
+
+
+def test_parity_two_layer_abrasion(statics):
+    say("case two-layer 64x64")
+    Wind.suspension = 1e-2
+    World.settling = 0.1
+    sed = np.full((64, 64), 0.003, np.float32)
+    w, st = run_pair(64, 64, 800, 3, seed=5, sed=sed, periodic=True, layers=2, abrasion=1e-3, window=8)
+    assert_same(w, st)
+    zm, sm = w.master()
+    assert sm.min() >= 0 and (sm == 0).any()
+    w.close()
+
+
+@pytest.mark.parametrize("nx,ny,periodic", [(1, 1, True), (1, 1, False), (2, 3, True), (3, 2, False), (1, 70, True)])
+def test_parity_tiny_maps(statics, nx, ny, periodic):
+    say("case tiny", nx, ny, periodic)
+    w, st = run_pair(nx, ny, 50, 2, seed=3, periodic=periodic, window=4)
+    assert_same(w, st)
+    w.close()
+
+
+def test_parity_cpu256_workload(statics):
+    """The 256x256 two-layer workload (fBm + sediment 0.05, wind (1,0,1)) for 3 cycles."""
+    from simplewinderosion_amd import workloads
+
+    say("case cpu256 3 cycles")
+    w = workloads.make_world("cpu256", seed=4)
+    h, s = workloads.make_inputs("cpu256", seed=4)
+    st = O.OracleState(256, 256, h, sediment=s, params=O.params_from(w.params()))
+    st.erode(2000, 3)
+    w.erode(2000, n_cycles=3)
+    w.synchronize()
+    assert_same(w, st)
+    w.close()
+
+
+def test_repeatable_and_split_calls_equal_one_call(statics):
+    say("case split calls")
+    rng = np.random.default_rng(1)
+    h = (rng.random((50, 50)) * 0.5).astype(np.float32)
+    outs = []
+    for split in (False, True, False):
+        w = World(50, 50, periodic=True, window=16)
+        w.upload(h)
+        if split:
+            w.erode(400)
+            w.erode(400)
+        else:
+            w.erode(400, n_cycles=2)
+        outs.append(w.master()[0])
+        w.close()
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_normals_match_oracle(statics):
+    import ctypes
+    say("case normals")
+    rng = np.random.default_rng(2)
+    for periodic in (False, True):
+        h = (rng.random((17, 23)) * 0.2).astype(np.float32)
+        w = World(17, 23, periodic=periodic)
+        w.upload(h)
+        n = w.normals()
+        S = O.unfx_vec(O._fx_vec(h)).reshape(17, 23)
+        out = np.zeros(3, np.float32)
+        for x in range(17):
+            for y in range(23):
+                O.load().or_normal(S.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 17, 23, int(periodic),
+                                   80.0, x, y, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+                assert np.array_equal(n[:, x, y], out), (x, y)
+        w.close()
+
+
+def test_full_size_one_cycle(statics):
+    """1024x1024 two-layer map with pyramids, 1,000,000 particles, one cycle,
+    16-call snapshot windows: bit-exact against the oracle on all host cores."""
+    from simplewinderosion_amd import workloads
+
+    say("case gpu1024 1 cycle")
+    w = workloads.make_world("gpu1024", seed=1)
+    h, s = workloads.make_inputs("gpu1024", seed=1)
+    w.erode(1_000_000, n_cycles=1)
+    w.synchronize()
+    st = O.OracleState(1024, 1024, h, sediment=s, params=O.params_from(w.params()))
+    st.erode(1_000_000, 1)
+    assert_same(w, st)
+    w.close()

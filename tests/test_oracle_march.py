@@ -1,0 +1,153 @@
+"""Pins of the oracle's particle march (wind.hpp:28-117) and batched erode
+(world.hpp:36-77) by a closed-form flat-map trajectory and exact invariants."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+F32 = np.float32
+
+
+def trajectory(S, p, key=7, px=10.3, pz=20.7, calls=600, SS=None):
+    S = np.ascontiguousarray(S, np.float32)
+    out = np.zeros((calls, 8), np.float32)
+    ss = None if SS is None else np.ascontiguousarray(SS, np.float32)
+    n = O.load().or_trajectory(ctypes.byref(p), S.shape[0], S.shape[1],
+                               S.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               None if ss is None else ss.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               key, px, pz, calls, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out[:n]
+
+
+@pytest.mark.parametrize("susp", [0.05, 1e-4])
+def test_flat_map_closed_form(susp):
+    """On a frozen flat map n = (0,1,0): shadow = 1, collision = 0, hfac = 1, no
+    turbulence and no gravity, so (wind.hpp:61,79,87,104-112)
+        v' = (1 - 0.3 s) (v + 0.05 (p - v)),   s' = s + susp (0.02 |v'| - s)
+    for 513 steps (maxAge = 512) on a periodic map."""
+    S = np.full((64, 64), 0.3, np.float32)
+    p = O.reference_params(periodic=1, suspension=susp)
+    tr = trajectory(S, p)
+    assert len(tr) == 513
+    v = np.zeros(3)
+    s = 0.0
+    pos = np.array([F32(10.0) + F32(300) / F32(1000), 0.3, F32(20.0) + F32(700) / F32(1000)], np.float64)
+    pos[0], pos[2] = 10.3, 20.7
+    ps = np.array([1.0, 0.0, 0.0])
+    for k in range(513):
+        v = (1.0 - 0.3 * s) * (v + 0.05 * (ps - v))
+        s = s + susp * (0.02 * np.linalg.norm(v) - s)
+        pos = pos + v
+        assert np.allclose(tr[k, 3:6], v, rtol=2e-5, atol=1e-7), k
+        assert abs(tr[k, 6] - s) <= 2e-5 * abs(s) + 1e-9, k
+    assert np.allclose(tr[-1, 0], pos[0], rtol=1e-5)
+    assert tr[:, 1].max() == F32(0.3) and tr[:, 4].max() == 0.0  # stays on the ground
+
+
+def test_flat_map_diagonal_wind():
+    S = np.zeros((32, 32), np.float32)
+    ps = (F32(1) * (F32(1) / np.sqrt(F32(2))), 0.0, F32(1) * (F32(1) / np.sqrt(F32(2))))
+    p = O.reference_params(periodic=1, pspeed=ps)
+    tr = trajectory(S, p)
+    assert len(tr) == 513
+    assert np.allclose(tr[:, 3], tr[:, 5]) and np.all(tr[:, 4] == 0)
+
+
+def test_bounded_particle_dies_at_the_edge():
+    S = np.zeros((16, 16), np.float32)
+    p = O.reference_params(periodic=0)
+    tr = trajectory(S, p, px=3.2, pz=5.0)
+    # it moves in +x until round(pos.x) reaches 16
+    assert 0 < len(tr) < 513
+    assert np.floor(tr[-1, 0] + 0.5) >= 16 or np.floor(tr[-2, 0] + 0.5) <= 15
+
+
+def test_slope_facing_wind_raises_collision():
+    # a ramp rising along +x faces the wind: the particle is pushed upward
+    nx = 64
+    S = (np.arange(nx, dtype=np.float32)[:, None] * F32(1 / 256) + np.zeros((1, nx), np.float32)).astype(np.float32)
+    p = O.reference_params(periodic=0)
+    tr = trajectory(S, p, px=5.0, pz=30.0)
+    assert tr[:, 4].max() > 0.0
+
+
+def _run(p, n=24, ny=None, particles=300, cycles=2, nthreads=0, seed_map=0, sed=None):
+    ny = ny or n
+    rng = np.random.default_rng(seed_map)
+    h = (rng.random((n, ny)) * 0.5).astype(np.float32)
+    st = O.OracleState(n, ny, h, sediment=sed, params=p)
+    z0 = st.zm.copy()
+    st.erode(particles, cycles, nthreads=nthreads)
+    return st, z0
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_mass_ledger_exact(periodic):
+    p = O.reference_params(periodic=periodic, seed=3)
+    st, z0 = _run(p)
+    assert int(st.zm.sum()) + st.lost.value == int(z0.sum())
+    assert st.lost.value != 0
+
+
+def test_deposit_on_death_closes_map_mass():
+    p = O.reference_params(periodic=1, seed=3, deposit_on_death=1)
+    st, z0 = _run(p)
+    assert st.lost.value == 0 and int(st.zm.sum()) == int(z0.sum())
+
+
+def test_two_layer_mass_and_nonnegative_sediment():
+    p = O.reference_params(periodic=1, seed=5, layers=2, suspension=1e-2, abrasion=1e-3, settling=0.1)
+    sed = np.full((24, 24), 0.002, np.float32)
+    st, z0 = _run(p, sed=sed, cycles=3)
+    assert int(st.zm.sum()) + st.lost.value == int(z0.sum())
+    assert st.sm.min() >= 0
+    assert (st.sm == 0).any()  # some cells eroded down to bedrock
+
+
+def test_thread_count_independence():
+    p = O.reference_params(periodic=1, seed=11, window=7)
+    a, _ = _run(p, nthreads=1)
+    b, _ = _run(p, nthreads=8)
+    assert np.array_equal(a.zm, b.zm) and np.array_equal(a.ema, b.ema) and a.lost.value == b.lost.value
+
+
+def test_zero_particles_is_identity_and_ema_decays():
+    p = O.reference_params(periodic=1, seed=1)
+    rng = np.random.default_rng(0)
+    h = (rng.random((8, 8)) * 0.5).astype(np.float32)
+    st = O.OracleState(8, 8, h, params=p)
+    st.ema[:] = rng.random(st.ema.size).astype(np.float32)
+    e0 = st.ema.copy()
+    z0 = st.zm.copy()
+    st.erode(0, 1)
+    assert np.array_equal(st.zm, z0)
+    assert np.array_equal(st.ema, (F32(1.0) - F32(0.1)) * e0 + F32(0.1) * F32(0.0))
+    assert st.cycle.value == 1
+
+
+def test_window_at_least_cycle_length_equals_per_cycle():
+    a, _ = _run(O.reference_params(periodic=1, seed=2, window=0))
+    b, _ = _run(O.reference_params(periodic=1, seed=2, window=514))
+    c, _ = _run(O.reference_params(periodic=1, seed=2, window=100))
+    assert np.array_equal(a.zm, b.zm)
+    assert not np.array_equal(a.zm, c.zm)
+
+
+def test_rectangular_non_pow2_map():
+    p = O.reference_params(periodic=1, seed=9, window=16)
+    st, z0 = _run(p, n=12, ny=37, particles=500)
+    assert int(st.zm.sum()) + st.lost.value == int(z0.sum())
+
+
+def test_sequential_port_runs_and_loses_only_airborne_mass():
+    p = O.reference_params(periodic=1)
+    rng = np.random.default_rng(0)
+    h = (rng.random((32, 32)) * 0.5).astype(np.float32)
+    h0 = h.copy()
+    ema = np.zeros(4 * 32 * 32, np.float32)
+    steps = O.erode_seq(p, 32, 32, h, None, ema, particles=50, n_cycles=1, seed=1)
+    assert steps == 50 * 513
+    assert not np.array_equal(h, h0)
+    assert abs(float(h.astype(np.float64).sum()) - float(h0.astype(np.float64).sum())) < 5.0
