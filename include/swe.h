@@ -11,11 +11,15 @@
  * swe_status, and swe_last_error() describes the last failure of a context.
  *
  * Semantics (batched-synchronous): within one snapshot window every particle
- * reads the same float32 snapshot of the map; pickups are summed as int64
- * fixed point with 2^-32 resolution; cascade events are counted per cell and
- * applied once per window (visits x the event's transfers).  Results are
- * deterministic for given (params, seed, cycle counter) and independent of
- * thread and launch order.
+ * makes `window` fly() calls reading the same float32 snapshot of the map;
+ * pickups are summed as int64 fixed point with 2^-32 resolution; cascade
+ * events are counted per cell, and at the end of the window each cell's
+ * events run one after another on a private copy of its 3x3 neighbourhood
+ * taken from the snapshot, their net transfers summed in the same fixed
+ * point.  The default window of 1 makes the method step-synchronous: every
+ * particle takes one step against the same map, then the map is updated.
+ * Results are deterministic for given (params, seed, cycle counter) and
+ * independent of thread and launch order.
  *
  * Map layout: cell (x, y) lives at index x*ny + y (math::flatten,
  * source/include/math.h:11-14): x is the slow axis, y the fast axis.
@@ -63,7 +67,7 @@ typedef struct swe_params {
   float settling;         /* World::settling = 1.0 */
   float abrasion;         /* sediment-layer extension (layers == 2): bedrock pickup rate */
   float mapscale;         /* quad::mapscale = 80 */
-  int32_t window;         /* fly() calls per snapshot window; 0 = one window per cycle */
+  int32_t window;         /* fly() calls per snapshot window (default 1); 0 = one window per cycle */
   int32_t periodic;       /* 1 = periodic map, 0 = bounded (reference default) */
   int32_t track;          /* 1 = massflow/momentum tracking and EMA fields */
   int32_t layers;         /* 1 = height only (reference); 2 = height + loose sediment */

@@ -51,7 +51,7 @@ def reference_params(**kw):
     p.settling = 1.0
     p.abrasion = 0.0
     p.mapscale = 80.0
-    p.window = 0
+    p.window = 1
     p.periodic = 0
     p.track = 1
     p.layers = 1
@@ -109,7 +109,7 @@ def load():
         "or_normal": (None, [_F, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_float, ctypes.c_int32,
                              ctypes.c_int32, _F]),
         "or_cascade_cell": (ctypes.c_int32, [_F, _F, ctypes.c_int32, ctypes.c_int32, _PP, ctypes.c_int32,
-                                             ctypes.c_int32, _I64, _I64]),
+                                             ctypes.c_int32, ctypes.c_uint32, _I64, _I64]),
         "or_relax_only": (None, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _U32, _I64, _I64]),
         "or_trajectory": (ctypes.c_int32, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, ctypes.c_uint64,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_int32, _F]),

@@ -11,8 +11,10 @@
  *
  *   1. or_erode()      -- the batched-synchronous method.  Every particle in a
  *      snapshot window reads the frozen float32 snapshot of the map; pickups
- *      are summed as int64 fixed point (2^-32) deltas, cascade events are
- *      counted per cell and applied once per window as visits x transfer.
+ *      are summed as int64 fixed point (2^-32) deltas; cascade events are
+ *      counted per cell and, at the end of the window, each cell's events run
+ *      in sequence on a private copy of its 3x3 neighbourhood taken from the
+ *      snapshot, their net transfers summed in the same fixed point.
  *      The GPU library must agree with it bit for bit.
  *
  *   2. or_erode_seq()  -- the reference's own sequential algorithm: one
@@ -292,21 +294,27 @@ static const int32_t OFFY[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 typedef struct { int64_t cell; float h; float s; float d; } or_point;
 
 /*
- * One cascade event at cell (x,y): gather the in-bounds neighbours in the
- * fixed offset order (world.hpp:83-114), sort by height descending with a
- * stable insertion sort (libstdc++ std::sort on 8 elements, world.hpp:118),
- * then walk them with the centre height evolving (world.hpp:122-151).
- * Writes, per transfer, the neighbour cell and the signed fixed-point amount
- * the neighbour receives; the centre receives the negated sum.  With
- * layers == 2 only loose sediment moves: a transfer is capped by the giver's
- * sediment snapshot (the centre's evolves within the event).
- * Returns the number of transfers.
- * Pinned by: hand-computed 3x3 cases (pair transfer settling*(|dh|-d*maxdiff)/2,
- * diagonal d = sqrtf(2), tie order = offset order, evolving centre).
+ * The cascade events of one cell in one snapshot window.  Each event is
+ * World::cascade (world.hpp:79-153) verbatim: gather the in-bounds
+ * neighbours in the fixed offset order (:83-114), stable-sort them by height
+ * descending (libstdc++ std::sort on <= 16 elements is an insertion sort,
+ * :118-120), then walk them with the centre height re-read, i.e. evolving,
+ * at every transfer (:122-151).  The `events` events of the window run one
+ * after another on a private copy of the cell's 3x3 neighbourhood taken from
+ * the snapshot, so later events see the transfers of earlier ones, as
+ * consecutive cascades at one cell do in the reference.  With layers == 2
+ * only loose sediment moves: a transfer is capped by the giver's sediment.
+ * Writes the neighbour cells in offset order (in-bounds ones only) and the
+ * fixed-point amount each received in total (the centre loses the sum).
+ * Returns the number of neighbours.
+ * Pinned by: hand-computed single-event cases (pair transfer
+ * settling*(|dh|-d*maxdiff)/2, diagonal d = sqrtf(2), ties in offset order,
+ * evolving centre) and multi-event cases (geometric approach to the
+ * threshold of an isolated pair).
  */
-int32_t or_cascade_event(const float* S, const float* SS, int32_t nx, int32_t ny,
-                         const or_params* p, int32_t x, int32_t y,
-                         int64_t nb_cell[8], int64_t nb_fx[8])
+int32_t or_cascade_events(const float* S, const float* SS, int32_t nx, int32_t ny,
+                          const or_params* p, int32_t x, int32_t y, uint32_t events,
+                          int64_t nb_cell[8], int64_t nb_fx[8])
 {
   or_point sn[8];
   int num = 0;
@@ -320,41 +328,50 @@ int32_t or_cascade_event(const float* S, const float* SS, int32_t nx, int32_t ny
     sn[num].h = S[c];
     sn[num].s = SS ? SS[c] : 0.0f;
     sn[num].d = sqrtf(fxk * fxk + fyk * fyk);          /* length(vec2(nn)) */
+    nb_cell[num] = c;
+    nb_fx[num] = 0;
     num++;
-  }
-  /* stable insertion sort, key h descending */
-  for (int i = 1; i < num; i++) {
-    or_point v = sn[i];
-    int j = i - 1;
-    while (j >= 0 && v.h > sn[j].h) { sn[j + 1] = sn[j]; j--; }
-    sn[j + 1] = v;
   }
   int64_t cc = (int64_t)(p->periodic ? wrapi(x, nx) : x) * ny + (p->periodic ? wrapi(y, ny) : y);
   float hc = S[cc];
   float sc = SS ? SS[cc] : 0.0f;
-  int32_t cnt = 0;
-  for (int i = 0; i < num; i++) {
-    float diff = hc - sn[i].h;
-    if (diff == 0.0f) continue;
-    float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;   /* * lodsize */
-    if (excess <= 0.0f) continue;
-    float t = p->settling * excess / 2.0f;
-    if (p->layers == 2) {
-      float avail = diff > 0.0f ? sc : sn[i].s;
-      if (t > avail) t = avail;
-      if (!(t > 0.0f)) continue;
+  for (uint32_t e = 0; e < events; e++) {
+    int ord[8];
+    for (int i = 0; i < num; i++) ord[i] = i;
+    for (int i = 1; i < num; i++) {                  /* stable insertion sort, h descending */
+      int v = ord[i];
+      int j = i - 1;
+      while (j >= 0 && sn[v].h > sn[ord[j]].h) { ord[j + 1] = ord[j]; j--; }
+      ord[j + 1] = v;
     }
-    int64_t tf = or_fx(t);
-    if (diff > 0.0f) {
-      hc -= t; sc -= t;
-      nb_cell[cnt] = sn[i].cell; nb_fx[cnt] = tf;
-    } else {
-      hc += t; sc += t;
-      nb_cell[cnt] = sn[i].cell; nb_fx[cnt] = -tf;
+    int moved = 0;
+    for (int r = 0; r < num; r++) {
+      const int i = ord[r];
+      float diff = hc - sn[i].h;
+      if (diff == 0.0f) continue;
+      float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;   /* * lodsize */
+      if (excess <= 0.0f) continue;
+      float t = p->settling * excess / 2.0f;
+      if (p->layers == 2) {
+        float avail = diff > 0.0f ? sc : sn[i].s;
+        if (t > avail) t = avail;
+        if (!(t > 0.0f)) continue;
+      }
+      int64_t tf = or_fx(t);
+      if (diff > 0.0f) {
+        hc -= t; sn[i].h += t;
+        if (p->layers == 2) { sc -= t; sn[i].s += t; }
+        nb_fx[i] += tf;
+      } else {
+        hc += t; sn[i].h -= t;
+        if (p->layers == 2) { sc += t; sn[i].s -= t; }
+        nb_fx[i] -= tf;
+      }
+      moved = 1;
     }
-    cnt++;
+    if (!moved) break;   /* nothing changed: every later event is identical */
   }
-  return cnt;
+  return num;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -514,15 +531,15 @@ static void or_relax(const or_params* p, int32_t nx, int32_t ny, const float* S,
     uint32_t v = V[c];
     if (!v) continue;
     int64_t nbc[8], nbf[8];
-    int32_t cnt = or_cascade_event(S, SS, nx, ny, p, (int32_t)(c / ny), (int32_t)(c % ny), nbc, nbf);
+    int32_t cnt = or_cascade_events(S, SS, nx, ny, p, (int32_t)(c / ny), (int32_t)(c % ny), v, nbc, nbf);
     int64_t centre = 0;
     for (int32_t i = 0; i < cnt; i++) {
-      int64_t a = (int64_t)v * nbf[i];
-      add64(&D[nbc[i]], a);
-      if (p->layers == 2) add64(&DS[nbc[i]], a);
-      centre -= a;
+      if (!nbf[i]) continue;
+      add64(&D[nbc[i]], nbf[i]);
+      if (p->layers == 2) add64(&DS[nbc[i]], nbf[i]);
+      centre -= nbf[i];
     }
-    if (cnt) {
+    if (centre) {
       add64(&D[c], centre);
       if (p->layers == 2) add64(&DS[c], centre);
     }
@@ -633,11 +650,11 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
   return 0;
 }
 
-/* Small entry points for known-answer tests of the pieces */
+/* Small entry point for known-answer tests of the cascade */
 int32_t or_cascade_cell(const float* S, const float* SS, int32_t nx, int32_t ny, const or_params* p,
-                        int32_t x, int32_t y, int64_t* nb_cell, int64_t* nb_fx)
+                        int32_t x, int32_t y, uint32_t events, int64_t* nb_cell, int64_t* nb_fx)
 {
-  return or_cascade_event(S, SS, nx, ny, p, x, y, nb_cell, nb_fx);
+  return or_cascade_events(S, SS, nx, ny, p, x, y, events, nb_cell, nb_fx);
 }
 
 /*

@@ -212,7 +212,7 @@ void swe_params_default(swe_params* p) {
   p->settling = 1.0f;
   p->abrasion = 0.0f;
   p->mapscale = 80.0f;
-  p->window = 0;
+  p->window = 1;
   p->periodic = 0;
   p->track = 1;
   p->layers = 1;

@@ -4,9 +4,10 @@
 //             (wind.hpp:28-117) against the window's float32 snapshot,
 //             int64 fixed-point pickup deltas and per-cell cascade-event
 //             counts by device-scope atomics.
-//   k_relax   LDS-tiled stencil: every cascade event of the window
-//             (world.hpp:79-153) as visits x transfers, summed in int64,
-//             then master += deltas and the next snapshot, in one pass.
+//   k_relax   LDS-tiled stencil: each cell's cascade events of the window
+//             (world.hpp:79-153) run in sequence on a private copy of its
+//             3x3 neighbourhood, net transfers summed in int64, then
+//             master += deltas and the next snapshot, in one pass.
 //   k_ema     World::erode's EMA flow fields (world.hpp:69-76).
 //   small fills/conversions for upload, download and zeroing.
 //
@@ -224,7 +225,7 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
 // Tile of kTX x kTY cells (y contiguous).  LDS holds the snapshot with a
 // 2-cell halo, the event counts with a 1-cell halo, and for every cell of the
 // 1-ring region the fixed-point amount each of its 8 neighbours receives
-// from its window's cascade events (visits x per-event transfer).
+// from that cell's cascade events of the window.
 constexpr int kTX = 8;
 constexpr int kTY = 64;
 constexpr int kHX = kTX + 4, kHY = kTY + 4;  // snapshot region
@@ -235,24 +236,29 @@ constexpr int kRelaxThreads = 256;
 constexpr int32_t c_offx[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int32_t c_offy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 
-// One cascade event (world.hpp:79-153) on the 3x3 neighbourhood h[] (index
-// 4 = centre, neighbour k at offset k of the fixed order), valid[k] false for
-// out-of-map neighbours.  amt[k] = signed fixed-point amount neighbour k
-// receives.  s[] is the sediment snapshot (LAYERS == 2).
+// The cascade events of one cell in one window (world.hpp:79-153), run one
+// after another on a private copy of the cell's 3x3 neighbourhood: hn[k] /
+// sn[k] are neighbour k's height / sediment in the fixed offset order
+// (world.hpp:83-92), hc / sc the centre's, bit k of validmask set for
+// in-map neighbours.  Each event sorts the neighbours by height descending
+// (ties in offset order, as the stable insertion sort std::sort uses on
+// eight elements) with a sorting network on (height, offset) keys, then
+// walks them with the centre evolving.  amt[k] accumulates the fixed-point
+// amount neighbour k received.
 template <int LAYERS>
-__device__ __forceinline__ void cascade_event(const float hn[8], const float sn[8], float hc, float sc,
-                                              uint32_t validmask, float maxdiff, float settling,
-                                              int64_t amt[8]) {
-  // stable sort by height descending == sort on (h desc, offset asc);
-  // invalid neighbours carry -inf and are skipped.
-  float key[8];
-  int32_t idx[8];
+__device__ __forceinline__ void cascade_events(float hn[8], float sn[8], float hc, float sc, uint32_t validmask,
+                                               uint32_t events, float maxdiff, float settling, int64_t amt[8]) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
-    idx[k] = k;
-    amt[k] = 0;
-  }
+  for (int k = 0; k < 8; k++) amt[k] = 0;
+  const float dd = __builtin_sqrtf(2.0f);
+  for (uint32_t e = 0; e < events; e++) {
+    float key[8];
+    int32_t idx[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
+      idx[k] = k;
+    }
 #define SWE_CAS(i, j)                                                                  \
   {                                                                                    \
     const bool sw = key[j] > key[i] || (key[j] == key[i] && idx[j] < idx[i]);          \
@@ -263,46 +269,53 @@ __device__ __forceinline__ void cascade_event(const float hn[8], const float sn[
     key[i] = tk;                                                                       \
     idx[i] = ti;                                                                       \
   }
-  SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
-  SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
-  SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
-  SWE_CAS(2, 4) SWE_CAS(3, 5)
-  SWE_CAS(1, 4) SWE_CAS(3, 6)
-  SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
+    SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
+    SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
+    SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
+    SWE_CAS(2, 4) SWE_CAS(3, 5)
+    SWE_CAS(1, 4) SWE_CAS(3, 6)
+    SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
 #undef SWE_CAS
-  const float dd = __builtin_sqrtf(2.0f);
+    bool moved = false;
 #pragma unroll
-  for (int r = 0; r < 8; r++) {
-    const int32_t k = idx[r];
-    if (!((validmask >> k) & 1u)) continue;
-    const float hk = key[r];
-    const float diff = hc - hk;
-    if (diff == 0.0f) continue;
-    const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
-    const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
-    if (excess <= 0.0f) continue;
-    float t = settling * excess / 2.0f;
-    if (LAYERS == 2) {
+    for (int r = 0; r < 8; r++) {
+      const int32_t k = idx[r];
+      if (!((validmask >> k) & 1u)) continue;
+      const float hk = key[r];
+      const float diff = hc - hk;
+      if (diff == 0.0f) continue;
+      const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
+      const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
+      if (excess <= 0.0f) continue;
+      float t = settling * excess / 2.0f;
       float sk = 0.0f;
+      if (LAYERS == 2) {
 #pragma unroll
-      for (int j = 0; j < 8; j++) sk = (j == k) ? sn[j] : sk;
-      const float avail = diff > 0.0f ? sc : sk;
-      if (t > avail) t = avail;
-      if (!(t > 0.0f)) continue;
-    }
-    const int64_t tf = fx_from_f32(t);
-    int64_t v;
-    if (diff > 0.0f) {
-      hc -= t;
-      sc -= t;
-      v = tf;
-    } else {
-      hc += t;
-      sc += t;
-      v = -tf;
-    }
+        for (int j = 0; j < 8; j++) sk = (j == k) ? sn[j] : sk;
+        const float avail = diff > 0.0f ? sc : sk;
+        if (t > avail) t = avail;
+        if (!(t > 0.0f)) continue;
+      }
+      const int64_t tf = fx_from_f32(t);
+      const float nh = diff > 0.0f ? hk + t : hk - t;
+      const float ns = diff > 0.0f ? sk + t : sk - t;
+      const int64_t v = diff > 0.0f ? tf : -tf;
+      if (diff > 0.0f) {
+        hc -= t;
+        sc -= t;
+      } else {
+        hc += t;
+        sc += t;
+      }
 #pragma unroll
-    for (int j = 0; j < 8; j++) amt[j] = (j == k) ? v : amt[j];
+      for (int j = 0; j < 8; j++) {
+        amt[j] = (j == k) ? amt[j] + v : amt[j];
+        hn[j] = (j == k) ? nh : hn[j];
+        if (LAYERS == 2) sn[j] = (j == k) ? ns : sn[j];
+      }
+      moved = true;
+    }
+    if (!moved) break;  // nothing changed: every later event is identical
   }
 }
 
@@ -316,11 +329,13 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
   const int32_t nx = a.nx, ny = a.ny;
   // XCD-aware tile order: consecutive workgroups land on different XCDs, so
   // hand each XCD a contiguous band of tiles (shares halo lines in its L2).
+  // Workgroup b runs on XCD b % 8 as that XCD's (b / 8)-th workgroup; XCD k
+  // owns the contiguous tile range starting at k*q + min(k, r).  Bijective
+  // for any grid size.
   const uint32_t nb = gridDim.x;
   const uint32_t b = blockIdx.x;
-  const uint32_t per = (nb + 7) / 8;
-  uint32_t t = (b % 8) * per + b / 8;
-  if (t >= nb) t = b;  // uneven tail: keep the identity mapping
+  const uint32_t q = nb / 8, rr = nb % 8, xcd = b % 8;
+  const uint32_t t = xcd * q + (xcd < rr ? xcd : rr) + b / 8;
   const int32_t tiles_y = (ny + kTY - 1) / kTY;
   const int32_t x0 = (int32_t)(t / tiles_y) * kTX;
   const int32_t y0 = (int32_t)(t % tiles_y) * kTY;
@@ -385,9 +400,9 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
     const float hc = sh[(lx + 1) * kHY + ly + 1];
     const float sc = LAYERS == 2 ? shs[(lx + 1) * kHY + ly + 1] : 0.0f;
     int64_t amt[8];
-    cascade_event<LAYERS>(hn, sn, hc, sc, valid, a.maxdiff, a.settling, amt);
+    cascade_events<LAYERS>(hn, sn, hc, sc, valid, v, a.maxdiff, a.settling, amt);
 #pragma unroll
-    for (int k = 0; k < 8; k++) out[k] = (int64_t)v * amt[k];
+    for (int k = 0; k < 8; k++) out[k] = amt[k];
   }
   __syncthreads();
 

@@ -6,11 +6,15 @@
 ``cpu256``     256x256 periodic map, fBm height + 0.05 sediment layer,
                wind normalize(1,0,1), suspension 1e-4, abrasion 1e-4,
                roughness (maxdiff) 0.005, settling 0.01; 2,000 particles x
-               100 cycles; one snapshot window per cycle.
+               100 cycles.
 ``gpu1024``    1024x1024 periodic map, same recipe plus 8 square pyramids
                (height 0.3-0.6, base 40-120 cells) in the height layer;
-               1,000,000 particles x 200 cycles; snapshot window of 16 fly()
-               calls (cascade gain 2 * visits-per-window * settling = 0.32).
+               1,000,000 particles x 200 cycles.
+
+All three use the step-synchronous window (one fly() call per snapshot
+window): particles in this dynamics crowd into lee-side cells (tens per cell
+per step at one particle per cell), and longer windows let those cells'
+cascades overshoot.
 
 Surface height = fBm (+ pyramids) + sediment thickness: the library's
 height layer is the surface the reference's cell->height describes
@@ -19,13 +23,13 @@ height layer is the surface the reference's cell->height describes
 from .world import World, Wind, generate_terrain, glm_normalize
 
 WORKLOADS = {
-    "reference": dict(nx=512, ny=512, particles=4096, cycles=1, periodic=False, layers=1, window=0,
+    "reference": dict(nx=512, ny=512, particles=4096, cycles=1, periodic=False, layers=1, window=1,
                       params=dict(), terrain=None, sediment=None),
-    "cpu256": dict(nx=256, ny=256, particles=2000, cycles=100, periodic=True, layers=2, window=0,
+    "cpu256": dict(nx=256, ny=256, particles=2000, cycles=100, periodic=True, layers=2, window=1,
                    params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
                                pspeed=glm_normalize((1.0, 0.0, 1.0))),
                    terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=0), sediment=0.05),
-    "gpu1024": dict(nx=1024, ny=1024, particles=1_000_000, cycles=200, periodic=True, layers=2, window=16,
+    "gpu1024": dict(nx=1024, ny=1024, particles=1_000_000, cycles=200, periodic=True, layers=2, window=1,
                     params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
                                 pspeed=glm_normalize((1.0, 0.0, 1.0))),
                     terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=8), sediment=0.05),

@@ -36,7 +36,7 @@ class SweError(RuntimeError):
     pass
 
 
-def build_params(abrasion=0.0, mapscale=80.0, window=0, periodic=False, track=True, layers=1,
+def build_params(abrasion=0.0, mapscale=80.0, window=1, periodic=False, track=True, layers=1,
                  deposit_on_death=False):
     """swe_params from the Wind/World statics plus per-map options (no device needed)."""
     p = capi.SweParams()
@@ -68,7 +68,7 @@ class World:
     maxdiff = 0.005
     settling = 1.0
 
-    def __init__(self, nx=512, ny=512, device=0, periodic=False, track=True, layers=1, window=0,
+    def __init__(self, nx=512, ny=512, device=0, periodic=False, track=True, layers=1, window=1,
                  abrasion=0.0, deposit_on_death=False, mapscale=80.0):
         self.lib = capi.load()
         self.nx, self.ny = int(nx), int(ny)

@@ -16,16 +16,20 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=None, **opts):
+def exact_sum(a):
+    a = np.asarray(a, np.int64).ravel()
+    return int((a >> 32).sum()) * 2 ** 32 + int((a & 0xFFFFFFFF).sum())
+
+
+def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=None, settling=None, **opts):
     """Run the same workload on the GPU and in the oracle; return both states."""
+    if settling is not None:
+        World.settling = settling
     rng = np.random.default_rng(map_seed)
     h = height if height is not None else (rng.random((nx, ny)) * 0.5).astype(np.float32)
     w = World(nx, ny, **{k: v for k, v in opts.items() if k in
                          ("periodic", "track", "layers", "window", "abrasion", "deposit_on_death")})
     w.upload(h, sed)
-    p = w.params()
-    p.seed = seed
-    # the World mirror pushes statics at erode(); apply seed through the static
     World.SEED = seed
     op = O.params_from(w.params())
     st = O.OracleState(nx, ny, h, sediment=sed, params=op)
@@ -48,19 +52,24 @@ def assert_same(w, st):
         assert np.array_equal(d["momentum"].ravel(), st.ema[n:]), "momentum differs"
     a = w.audit()
     assert a["lost"] == st.lost.value
-    assert a["height_sum"] == int(st.zm.sum())
+    assert a["height_sum"] == exact_sum(st.zm)
+    assert np.abs(O.unfx_vec(st.zm)).max() < 100.0, "unstable parameters: the map diverged"
     assert a["cycle"] == st.cycle.value
 
 
+# Cascade gain per window is about 2 * (particles * window / cells) * settling;
+# each case keeps it below 0.5 so that the map stays bounded.
 CASES = [
     # name, nx, ny, particles, cycles, opts
-    ("periodic-ref", 32, 32, 200, 2, dict(periodic=True)),
-    ("bounded-ref", 48, 40, 300, 2, dict(periodic=False, window=16)),
-    ("bounded-window1", 20, 24, 100, 1, dict(periodic=False, window=1)),
-    ("periodic-notrack", 40, 33, 300, 2, dict(periodic=True, track=False, window=9)),
-    ("deposit", 32, 32, 300, 2, dict(periodic=True, deposit_on_death=True, window=16)),
-    ("rect-nonpow2", 37, 129, 700, 2, dict(periodic=True, window=5)),
-    ("tall-bounded", 130, 9, 400, 1, dict(periodic=False, window=32)),
+    ("periodic-ref-params", 32, 32, 20, 2, dict(periodic=True, window=1)),
+    ("bounded-ref-params", 64, 64, 40, 2, dict(periodic=False, window=1)),
+    ("bounded-w16", 48, 40, 300, 2, dict(periodic=False, window=16, settling=0.05)),
+    ("bounded-window1", 20, 24, 100, 1, dict(periodic=False, window=1, settling=0.2)),
+    ("periodic-notrack", 40, 33, 300, 2, dict(periodic=True, track=False, window=9, settling=0.05)),
+    ("deposit", 32, 32, 300, 2, dict(periodic=True, deposit_on_death=True, window=16, settling=0.02)),
+    ("rect-nonpow2", 37, 129, 700, 2, dict(periodic=True, window=5, settling=0.1)),
+    ("tall-bounded", 130, 9, 400, 1, dict(periodic=False, window=32, settling=0.01)),
+    ("per-cycle-window", 96, 80, 100, 2, dict(periodic=True, window=0, settling=0.01)),
 ]
 
 
@@ -77,7 +86,7 @@ def test_parity_two_layer_abrasion(statics):
     Wind.suspension = 1e-2
     World.settling = 0.1
     sed = np.full((64, 64), 0.003, np.float32)
-    w, st = run_pair(64, 64, 800, 3, seed=5, sed=sed, periodic=True, layers=2, abrasion=1e-3, window=8)
+    w, st = run_pair(64, 64, 800, 3, seed=5, sed=sed, periodic=True, layers=2, abrasion=1e-3, window=8, settling=0.1)
     assert_same(w, st)
     zm, sm = w.master()
     assert sm.min() >= 0 and (sm == 0).any()
@@ -87,7 +96,7 @@ def test_parity_two_layer_abrasion(statics):
 @pytest.mark.parametrize("nx,ny,periodic", [(1, 1, True), (1, 1, False), (2, 3, True), (3, 2, False), (1, 70, True)])
 def test_parity_tiny_maps(statics, nx, ny, periodic):
     say("case tiny", nx, ny, periodic)
-    w, st = run_pair(nx, ny, 50, 2, seed=3, periodic=periodic, window=4)
+    w, st = run_pair(nx, ny, 5, 2, seed=3, periodic=periodic, window=4, settling=0.01)
     assert_same(w, st)
     w.close()
 
@@ -112,6 +121,7 @@ def test_repeatable_and_split_calls_equal_one_call(statics):
     rng = np.random.default_rng(1)
     h = (rng.random((50, 50)) * 0.5).astype(np.float32)
     outs = []
+    World.settling = 0.02
     for split in (False, True, False):
         w = World(50, 50, periodic=True, window=16)
         w.upload(h)

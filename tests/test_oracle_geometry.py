@@ -78,7 +78,7 @@ def test_periodic_wrap_equals_shifted_interior():
 
 # -- cascade -----------------------------------------------------------------
 
-def cascade(S, x, y, SS=None, **kw):
+def cascade(S, x, y, SS=None, events=1, **kw):
     p = O.reference_params(**kw)
     S = np.ascontiguousarray(S, np.float32)
     nbc = np.zeros(8, np.int64)
@@ -86,10 +86,10 @@ def cascade(S, x, y, SS=None, **kw):
     ss = None if SS is None else np.ascontiguousarray(SS, np.float32)
     n = O.load().or_cascade_cell(S.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
                                  None if ss is None else ss.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
-                                 S.shape[0], S.shape[1], ctypes.byref(p), int(x), int(y),
+                                 S.shape[0], S.shape[1], ctypes.byref(p), int(x), int(y), int(events),
                                  nbc.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                  nbf.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
-    return [(int(nbc[i]), int(nbf[i])) for i in range(n)]
+    return [(int(nbc[i]), int(nbf[i])) for i in range(n) if nbf[i] != 0]
 
 
 def test_pair_transfer():
@@ -181,3 +181,26 @@ def test_periodic_neighbours_wrap():
     # eight neighbours, all lower: the first (offset (-1,-1)) is cell (3,3)
     assert len(got) == 8
     assert set(got) == {15, 12, 13, 3, 1, 7, 4, 5}
+
+
+@pytest.mark.parametrize("settling,events", [(1.0, 3), (0.5, 7), (0.1, 40)])
+def test_repeated_events_on_isolated_pair_follow_geometric_law(settling, events):
+    """V events at one cell of an isolated pair: each event moves half of
+    settling*(gap - maxdiff) across, so gap - maxdiff shrinks by (1 - settling)
+    per event and the total moved is (a - maxdiff)(1 - (1-settling)^V)/2."""
+    a = 0.3
+    S = np.array([[a, 0.0]], np.float32)
+    got = cascade(S, 0, 0, events=events, settling=settling)
+    moved = O.unfx(got[0][1]) if got else 0.0
+    want = (a - 0.005) * (1 - (1 - settling) ** events) / 2
+    assert abs(moved - want) < 1e-6 * events
+    # one event equals the single-event formula exactly
+    one = cascade(S, 0, 0, events=1, settling=settling)
+    assert one[0][1] == O.fx(F32(F32(settling) * (F32(a) - F32(0.005))) / F32(2.0))
+
+
+def test_events_stop_once_within_threshold():
+    S = np.array([[0.3, 0.0]], np.float32)
+    a = cascade(S, 0, 0, events=200)
+    b = cascade(S, 0, 0, events=100000)
+    assert a == b  # settling 1: the pair reaches the threshold after one event
