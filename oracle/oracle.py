@@ -102,6 +102,7 @@ def load():
         "or_fx": (ctypes.c_int64, [ctypes.c_float]),
         "or_unfx": (ctypes.c_float, [ctypes.c_int64]),
         "or_expf": (ctypes.c_float, [ctypes.c_float]),
+        "or_settle": (ctypes.c_float, [ctypes.c_float, ctypes.c_uint32]),
         "or_mix64": (ctypes.c_uint64, [ctypes.c_uint64]),
         "or_particle_key": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
         "or_spawn": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _F, _F]),

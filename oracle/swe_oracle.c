@@ -12,9 +12,10 @@
  *   1. or_erode()      -- the batched-synchronous method.  Every particle in a
  *      snapshot window reads the frozen float32 snapshot of the map; pickups
  *      are summed as int64 fixed point (2^-32) deltas; cascade events are
- *      counted per cell and, at the end of the window, each cell's events run
- *      in sequence on a private copy of its 3x3 neighbourhood taken from the
- *      snapshot, their net transfers summed in the same fixed point.
+ *      counted per cell and, at the end of the window, each visited cell runs
+ *      one cascade against the snapshot whose settling factor is that of its
+ *      V events, 1 - (1 - settling)^V; transfers are summed in the same
+ *      fixed point.
  *      The GPU library must agree with it bit for bit.
  *
  *   2. or_erode_seq()  -- the reference's own sequential algorithm: one
@@ -294,23 +295,41 @@ static const int32_t OFFY[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 typedef struct { int64_t cell; float h; float s; float d; } or_point;
 
 /*
- * The cascade events of one cell in one snapshot window.  Each event is
- * World::cascade (world.hpp:79-153) verbatim: gather the in-bounds
+ * Settling factor of `events` cascade events at one cell in one window:
+ * settling itself for one event, else 1 - (1 - settling)^events, the
+ * fraction of an isolated pair's excess that that many consecutive events
+ * remove (each removes `settling` of what is left).  The power is taken by
+ * binary exponentiation, least significant bit first, in float.
+ * Pinned by: the isolated-pair geometric law and exact small cases.
+ */
+float or_settle(float settling, uint32_t events)
+{
+  if (events <= 1u) return settling;
+  float b = 1.0f - settling, r = 1.0f;
+  uint32_t e = events;
+  while (e) {
+    if (e & 1u) r = r * b;
+    b = b * b;
+    e >>= 1;
+  }
+  return 1.0f - r;
+}
+
+/*
+ * The cascade of one cell in one snapshot window: World::cascade
+ * (world.hpp:79-153) against the snapshot -- gather the in-bounds
  * neighbours in the fixed offset order (:83-114), stable-sort them by height
  * descending (libstdc++ std::sort on <= 16 elements is an insertion sort,
  * :118-120), then walk them with the centre height re-read, i.e. evolving,
- * at every transfer (:122-151).  The `events` events of the window run one
- * after another on a private copy of the cell's 3x3 neighbourhood taken from
- * the snapshot, so later events see the transfers of earlier ones, as
- * consecutive cascades at one cell do in the reference.  With layers == 2
- * only loose sediment moves: a transfer is capped by the giver's sediment.
+ * at every transfer (:122-151) -- with the settling factor of the window's
+ * `events` events at the cell (or_settle).  With layers == 2 only loose
+ * sediment moves: a transfer is capped by the giver's sediment.
  * Writes the neighbour cells in offset order (in-bounds ones only) and the
- * fixed-point amount each received in total (the centre loses the sum).
+ * fixed-point amount each receives (the centre loses the sum).
  * Returns the number of neighbours.
- * Pinned by: hand-computed single-event cases (pair transfer
- * settling*(|dh|-d*maxdiff)/2, diagonal d = sqrtf(2), ties in offset order,
- * evolving centre) and multi-event cases (geometric approach to the
- * threshold of an isolated pair).
+ * Pinned by: hand-computed cases (pair transfer settling*(|dh|-d*maxdiff)/2,
+ * diagonal d = sqrtf(2), ties in offset order, evolving centre) and the
+ * isolated-pair law for several events.
  */
 int32_t or_cascade_events(const float* S, const float* SS, int32_t nx, int32_t ny,
                           const or_params* p, int32_t x, int32_t y, uint32_t events,
@@ -332,44 +351,38 @@ int32_t or_cascade_events(const float* S, const float* SS, int32_t nx, int32_t n
     nb_fx[num] = 0;
     num++;
   }
+  int ord[8];
+  for (int i = 0; i < num; i++) ord[i] = i;
+  for (int i = 1; i < num; i++) {                    /* stable insertion sort, h descending */
+    int v = ord[i];
+    int j = i - 1;
+    while (j >= 0 && sn[v].h > sn[ord[j]].h) { ord[j + 1] = ord[j]; j--; }
+    ord[j + 1] = v;
+  }
   int64_t cc = (int64_t)(p->periodic ? wrapi(x, nx) : x) * ny + (p->periodic ? wrapi(y, ny) : y);
   float hc = S[cc];
   float sc = SS ? SS[cc] : 0.0f;
-  for (uint32_t e = 0; e < events; e++) {
-    int ord[8];
-    for (int i = 0; i < num; i++) ord[i] = i;
-    for (int i = 1; i < num; i++) {                  /* stable insertion sort, h descending */
-      int v = ord[i];
-      int j = i - 1;
-      while (j >= 0 && sn[v].h > sn[ord[j]].h) { ord[j + 1] = ord[j]; j--; }
-      ord[j + 1] = v;
+  const float settling = or_settle(p->settling, events);
+  for (int r = 0; r < num; r++) {
+    const int i = ord[r];
+    float diff = hc - sn[i].h;
+    if (diff == 0.0f) continue;
+    float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;   /* * lodsize */
+    if (excess <= 0.0f) continue;
+    float t = settling * excess / 2.0f;
+    if (p->layers == 2) {
+      float avail = diff > 0.0f ? sc : sn[i].s;
+      if (t > avail) t = avail;
+      if (!(t > 0.0f)) continue;
     }
-    int moved = 0;
-    for (int r = 0; r < num; r++) {
-      const int i = ord[r];
-      float diff = hc - sn[i].h;
-      if (diff == 0.0f) continue;
-      float excess = fabsf(diff) - (sn[i].d * p->maxdiff) * 1.0f;   /* * lodsize */
-      if (excess <= 0.0f) continue;
-      float t = p->settling * excess / 2.0f;
-      if (p->layers == 2) {
-        float avail = diff > 0.0f ? sc : sn[i].s;
-        if (t > avail) t = avail;
-        if (!(t > 0.0f)) continue;
-      }
-      int64_t tf = or_fx(t);
-      if (diff > 0.0f) {
-        hc -= t; sn[i].h += t;
-        if (p->layers == 2) { sc -= t; sn[i].s += t; }
-        nb_fx[i] += tf;
-      } else {
-        hc += t; sn[i].h -= t;
-        if (p->layers == 2) { sc += t; sn[i].s -= t; }
-        nb_fx[i] -= tf;
-      }
-      moved = 1;
+    int64_t tf = or_fx(t);
+    if (diff > 0.0f) {
+      hc -= t; sc -= t;
+      nb_fx[i] += tf;
+    } else {
+      hc += t; sc += t;
+      nb_fx[i] -= tf;
     }
-    if (!moved) break;   /* nothing changed: every later event is identical */
   }
   return num;
 }

@@ -4,10 +4,11 @@
 //             (wind.hpp:28-117) against the window's float32 snapshot,
 //             int64 fixed-point pickup deltas and per-cell cascade-event
 //             counts by device-scope atomics.
-//   k_relax   LDS-tiled stencil: each cell's cascade events of the window
-//             (world.hpp:79-153) run in sequence on a private copy of its
-//             3x3 neighbourhood, net transfers summed in int64, then
-//             master += deltas and the next snapshot, in one pass.
+//   k_relax   LDS-tiled stencil: each visited cell's cascade of the window
+//             (world.hpp:79-153) against the snapshot, with the settling
+//             factor of its V events, 1 - (1 - settling)^V; transfers summed
+//             in int64, then master += deltas and the next snapshot, in one
+//             pass.
 //   k_ema     World::erode's EMA flow fields (world.hpp:69-76).
 //   small fills/conversions for upload, download and zeroing.
 //
@@ -225,7 +226,7 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
 // Tile of kTX x kTY cells (y contiguous).  LDS holds the snapshot with a
 // 2-cell halo, the event counts with a 1-cell halo, and for every cell of the
 // 1-ring region the fixed-point amount each of its 8 neighbours receives
-// from that cell's cascade events of the window.
+// from that cell's cascade of the window.
 constexpr int kTX = 8;
 constexpr int kTY = 64;
 constexpr int kHX = kTX + 4, kHY = kTY + 4;  // snapshot region
@@ -236,29 +237,40 @@ constexpr int kRelaxThreads = 256;
 constexpr int32_t c_offx[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int32_t c_offy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
 
-// The cascade events of one cell in one window (world.hpp:79-153), run one
-// after another on a private copy of the cell's 3x3 neighbourhood: hn[k] /
-// sn[k] are neighbour k's height / sediment in the fixed offset order
-// (world.hpp:83-92), hc / sc the centre's, bit k of validmask set for
-// in-map neighbours.  Each event sorts the neighbours by height descending
-// (ties in offset order, as the stable insertion sort std::sort uses on
-// eight elements) with a sorting network on (height, offset) keys, then
-// walks them with the centre evolving.  amt[k] accumulates the fixed-point
-// amount neighbour k received.
+// Settling factor of a cell's `events` cascade events in one window:
+// settling for one event, else 1 - (1 - settling)^events by binary
+// exponentiation (least significant bit first) in float.
+__device__ __forceinline__ float settle_factor(float settling, uint32_t events) {
+  if (events <= 1u) return settling;
+  float b = 1.0f - settling, r = 1.0f;
+  uint32_t e = events;
+  while (e) {
+    if (e & 1u) r = r * b;
+    b = b * b;
+    e >>= 1;
+  }
+  return 1.0f - r;
+}
+
+// One cell's cascade of the window: World::cascade (world.hpp:79-153)
+// against the snapshot with the window's settling factor.  hn[k] / sn[k]:
+// neighbour k's height / sediment in the fixed offset order
+// (world.hpp:83-92); hc / sc: the centre's; bit k of validmask set for
+// in-map neighbours.  The neighbours are sorted by height descending, ties
+// in offset order (what the stable insertion sort std::sort uses on eight
+// elements does), by a sorting network on (height, offset) keys; the walk
+// then evolves the centre.  amt[k] = fixed-point amount neighbour k receives.
 template <int LAYERS>
-__device__ __forceinline__ void cascade_events(float hn[8], float sn[8], float hc, float sc, uint32_t validmask,
-                                               uint32_t events, float maxdiff, float settling, int64_t amt[8]) {
+__device__ __forceinline__ void cascade_cell(const float hn[8], const float sn[8], float hc, float sc,
+                                             uint32_t validmask, float settling, float maxdiff, int64_t amt[8]) {
+  float key[8], sed[8];
+  int32_t idx[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) amt[k] = 0;
-  const float dd = __builtin_sqrtf(2.0f);
-  for (uint32_t e = 0; e < events; e++) {
-    float key[8];
-    int32_t idx[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
-      idx[k] = k;
-    }
+  for (int k = 0; k < 8; k++) {
+    key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
+    sed[k] = LAYERS == 2 ? sn[k] : 0.0f;
+    idx[k] = k;
+  }
 #define SWE_CAS(i, j)                                                                  \
   {                                                                                    \
     const bool sw = key[j] > key[i] || (key[j] == key[i] && idx[j] < idx[i]);          \
@@ -268,54 +280,55 @@ __device__ __forceinline__ void cascade_events(float hn[8], float sn[8], float h
     idx[j] = sw ? idx[i] : idx[j];                                                     \
     key[i] = tk;                                                                       \
     idx[i] = ti;                                                                       \
+    if (LAYERS == 2) {                                                                 \
+      const float ts = sw ? sed[j] : sed[i];                                           \
+      sed[j] = sw ? sed[i] : sed[j];                                                   \
+      sed[i] = ts;                                                                     \
+    }                                                                                  \
   }
-    SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
-    SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
-    SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
-    SWE_CAS(2, 4) SWE_CAS(3, 5)
-    SWE_CAS(1, 4) SWE_CAS(3, 6)
-    SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
+  SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
+  SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
+  SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
+  SWE_CAS(2, 4) SWE_CAS(3, 5)
+  SWE_CAS(1, 4) SWE_CAS(3, 6)
+  SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
 #undef SWE_CAS
-    bool moved = false;
+  const float dd = __builtin_sqrtf(2.0f);
+  int64_t got[8];  // in sorted slot order
 #pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const int32_t k = idx[r];
-      if (!((validmask >> k) & 1u)) continue;
-      const float hk = key[r];
-      const float diff = hc - hk;
-      if (diff == 0.0f) continue;
-      const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
-      const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
-      if (excess <= 0.0f) continue;
-      float t = settling * excess / 2.0f;
-      float sk = 0.0f;
-      if (LAYERS == 2) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) sk = (j == k) ? sn[j] : sk;
-        const float avail = diff > 0.0f ? sc : sk;
-        if (t > avail) t = avail;
-        if (!(t > 0.0f)) continue;
-      }
-      const int64_t tf = fx_from_f32(t);
-      const float nh = diff > 0.0f ? hk + t : hk - t;
-      const float ns = diff > 0.0f ? sk + t : sk - t;
-      const int64_t v = diff > 0.0f ? tf : -tf;
-      if (diff > 0.0f) {
-        hc -= t;
-        sc -= t;
-      } else {
-        hc += t;
-        sc += t;
-      }
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        amt[j] = (j == k) ? amt[j] + v : amt[j];
-        hn[j] = (j == k) ? nh : hn[j];
-        if (LAYERS == 2) sn[j] = (j == k) ? ns : sn[j];
-      }
-      moved = true;
+  for (int r = 0; r < 8; r++) {
+    got[r] = 0;
+    const int32_t k = idx[r];
+    if (!((validmask >> k) & 1u)) continue;
+    const float diff = hc - key[r];
+    if (diff == 0.0f) continue;
+    const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
+    const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
+    if (excess <= 0.0f) continue;
+    float t = settling * excess / 2.0f;
+    if (LAYERS == 2) {
+      const float avail = diff > 0.0f ? sc : sed[r];
+      if (t > avail) t = avail;
+      if (!(t > 0.0f)) continue;
     }
-    if (!moved) break;  // nothing changed: every later event is identical
+    const int64_t tf = fx_from_f32(t);
+    if (diff > 0.0f) {
+      hc -= t;
+      sc -= t;
+      got[r] = tf;
+    } else {
+      hc += t;
+      sc += t;
+      got[r] = -tf;
+    }
+  }
+  // back to offset order
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    int64_t v = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++) v = idx[r] == k ? got[r] : v;
+    amt[k] = v;
   }
 }
 
@@ -400,7 +413,7 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
     const float hc = sh[(lx + 1) * kHY + ly + 1];
     const float sc = LAYERS == 2 ? shs[(lx + 1) * kHY + ly + 1] : 0.0f;
     int64_t amt[8];
-    cascade_events<LAYERS>(hn, sn, hc, sc, valid, v, a.maxdiff, a.settling, amt);
+    cascade_cell<LAYERS>(hn, sn, hc, sc, valid, settle_factor(a.settling, v), a.maxdiff, amt);
 #pragma unroll
     for (int k = 0; k < 8; k++) out[k] = amt[k];
   }
