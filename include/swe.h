@@ -136,6 +136,15 @@ int swe_upload_fields(swe_ctx* ctx, const float* massflow, const float* momentum
 int swe_erode(swe_ctx* ctx, int64_t particles_per_cycle, int32_t n_cycles);
 int swe_synchronize(swe_ctx* ctx);
 
+/*
+ * Execution path of swe_erode (results are identical on every path):
+ * 0 = automatic (tile-owned step kernel for window == 1, else the global
+ * march), 1 = global-memory march with device-scope atomics (any window),
+ * 2 = tile-owned step kernel (window == 1 only).  No reference counterpart:
+ * the reference has a single sequential path.
+ */
+int swe_set_path(swe_ctx* ctx, int32_t path);
+
 /* Exact mass audit of the current state (synchronizes). */
 int swe_audit(swe_ctx* ctx, swe_audit_t* out);
 /* RNG cycle counter (for resume); setting it does not touch the map. */

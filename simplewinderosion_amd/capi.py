@@ -108,6 +108,7 @@ SIGNATURES = {
     "swe_upload_fields": (ctypes.c_int, [_P, _FP, _FP]),
     "swe_erode": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32]),
     "swe_synchronize": (ctypes.c_int, [_P]),
+    "swe_set_path": (ctypes.c_int, [_P, ctypes.c_int32]),
     "swe_audit": (ctypes.c_int, [_P, ctypes.POINTER(SweAudit)]),
     "swe_get_cycle": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "swe_set_cycle": (ctypes.c_int, [_P, ctypes.c_uint64]),

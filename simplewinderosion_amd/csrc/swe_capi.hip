@@ -16,15 +16,10 @@
 
 #include "../../include/swe.h"
 #include "swe_kernels.h"
+#include "swe_step.h"
 
 namespace {
 
-struct Graph {
-  hipGraphExec_t exec = nullptr;
-  int64_t n = -1;
-  int parity = -1;
-  bool valid = false;
-};
 
 }  // namespace
 
@@ -35,25 +30,40 @@ struct swe_ctx {
   hipStream_t stream = nullptr;
   swe_params p{};
   std::string err;
+  int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
 
-  // map (device)
-  int64_t* zm = nullptr;
+  // map (device); index [g & 1] is current, g counts snapshot windows
+  uint64_t g = 0;
+  int64_t* zmv[2] = {nullptr, nullptr};
   float* zs[2] = {nullptr, nullptr};
-  int64_t* dz = nullptr;
-  uint32_t* vis[2] = {nullptr, nullptr};
-  int64_t* sm = nullptr;
+  int64_t* smv[2] = {nullptr, nullptr};
   float* ss[2] = {nullptr, nullptr};
-  int64_t* ds = nullptr;
   int64_t* trk = nullptr;
   float* ema = nullptr;
-  int cur = 0;  // which snapshot / event buffer the next window reads
 
-  // particles (device)
+  // global-atomic march path
+  int64_t* dz = nullptr;
+  int64_t* ds = nullptr;
+  uint32_t* vis[2] = {nullptr, nullptr};
   int64_t cap = 0;
   float4* P0 = nullptr;
   float4* P1 = nullptr;
   int64_t* L = nullptr;
   int32_t* LC = nullptr;
+
+  // tile-owned step path
+  int32_t tiles_x = 0, tiles_y = 0;
+  int64_t* fD[2] = {nullptr, nullptr};
+  int64_t* fDS[2] = {nullptr, nullptr};
+  uint32_t* fV[2] = {nullptr, nullptr};
+  int64_t fcap = 0;
+  swe::ParticleBuf PB[2] = {}, FB[3] = {};
+  uint32_t* farcnt = nullptr;  // 3
+  int64_t* tbase[2] = {nullptr, nullptr};
+  int64_t* tcnt[2] = {nullptr, nullptr};
+  uint32_t* nin = nullptr;     // 3 x tiles
+  int32_t* sp_tile = nullptr;
+  uint32_t* sp_rank = nullptr;
 
   // counters (device)
   int64_t* lost = nullptr;     // swe::kLostSlots
@@ -70,6 +80,8 @@ struct swe_ctx {
   };
   std::vector<Pending> pending;
   swe_stats_t st{};
+
+  int cur() const { return (int)(g & 1u); }
 };
 
 namespace {
@@ -129,11 +141,11 @@ int check_params(swe_ctx* c, const swe_params* p) {
 
 int ensure_layers(swe_ctx* c) {
   int r;
-  if (c->p.layers == 2 && !c->sm) {
-    if ((r = dalloc(c, &c->sm, c->ncell))) return r;
-    if ((r = dalloc(c, &c->ss[0], c->ncell))) return r;
-    if ((r = dalloc(c, &c->ss[1], c->ncell))) return r;
-    if ((r = dalloc(c, &c->ds, c->ncell))) return r;
+  if (c->p.layers == 2 && !c->smv[0]) {
+    if ((r = dalloc(c, &c->smv[0], c->ncell)) || (r = dalloc(c, &c->smv[1], c->ncell)) ||
+        (r = dalloc(c, &c->ss[0], c->ncell)) || (r = dalloc(c, &c->ss[1], c->ncell)) ||
+        (r = dalloc(c, &c->ds, c->ncell)))
+      return r;
   }
   if (c->p.track && !c->trk) {
     if ((r = dalloc(c, &c->trk, 4 * c->ncell))) return r;
@@ -157,6 +169,53 @@ int ensure_particles(swe_ctx* c, int64_t n) {
   return 0;
 }
 
+void free_pbuf(swe::ParticleBuf& b) {
+  dfree(b.P0);
+  dfree(b.P1);
+  dfree(b.L);
+  dfree(b.LC);
+  dfree(b.PID);
+  dfree(b.CODE);
+}
+
+int alloc_pbuf(swe_ctx* c, swe::ParticleBuf& b, int64_t n) {
+  int r;
+  if ((r = dalloc(c, &b.P0, n)) || (r = dalloc(c, &b.P1, n)) || (r = dalloc(c, &b.L, n)) ||
+      (r = dalloc(c, &b.LC, n)) || (r = dalloc(c, &b.PID, n)) || (r = dalloc(c, &b.CODE, n)))
+    return r;
+  return 0;
+}
+
+// buffers of the tile-owned step path for n particles
+int ensure_step(swe_ctx* c, int64_t n) {
+  int r;
+  if (!c->fD[0]) {
+    const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
+    if ((r = dalloc(c, &c->fD[0], c->ncell)) || (r = dalloc(c, &c->fD[1], c->ncell)) ||
+        (r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
+        (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->tbase[0], nt)) || (r = dalloc(c, &c->tbase[1], nt)) ||
+        (r = dalloc(c, &c->tcnt[0], nt)) || (r = dalloc(c, &c->tcnt[1], nt)) || (r = dalloc(c, &c->nin, 3 * nt)))
+      return r;
+  }
+  if (c->p.layers == 2 && !c->fDS[0]) {
+    if ((r = dalloc(c, &c->fDS[0], c->ncell)) || (r = dalloc(c, &c->fDS[1], c->ncell))) return r;
+  }
+  if (n > c->fcap) {
+    for (int k = 0; k < 2; k++) free_pbuf(c->PB[k]);
+    for (int k = 0; k < 3; k++) free_pbuf(c->FB[k]);
+    dfree(c->sp_tile);
+    dfree(c->sp_rank);
+    c->fcap = 0;
+    for (int k = 0; k < 2; k++)
+      if ((r = alloc_pbuf(c, c->PB[k], n))) return r;
+    for (int k = 0; k < 3; k++)
+      if ((r = alloc_pbuf(c, c->FB[k], n))) return r;
+    if ((r = dalloc(c, &c->sp_tile, n)) || (r = dalloc(c, &c->sp_rank, n))) return r;
+    c->fcap = n;
+  }
+  return 0;
+}
+
 hipEvent_t take_event(swe_ctx* c) {
   if (c->ev_used == c->ev_pool.size()) {
     hipEvent_t e;
@@ -173,7 +232,7 @@ int collect_timing(swe_ctx* c) {
   for (auto& pd : c->pending) {
     float ms = 0.0f;
     HIPCHK(c, hipEventElapsedTime(&ms, pd.a, pd.b));
-    if (pd.kind == 0) {
+    if (pd.kind == 0 || pd.kind == 2) {
       c->st.march_ms += ms;
       c->st.march_launches++;
     } else {
@@ -192,6 +251,211 @@ int64_t windows_of(const swe_params& p, int32_t* R) {
   const int32_t r = p.window > 0 ? p.window : (int32_t)calls;
   *R = r;
   return (calls + r - 1) / r;
+}
+
+void fill_consts(const swe_ctx* c, float ps[3], float nps[3]) {
+  for (int k = 0; k < 3; k++) ps[k] = c->p.pspeed[k];
+  normalize3(c->p.pspeed, nps);
+}
+
+int timed_begin(swe_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  if (!c->timing) return 0;
+  *e0 = take_event(c);
+  *e1 = take_event(c);
+  if (!*e0 || !*e1) return fail(c, SWE_EHIP, "hipEventCreate failed");
+  HIPCHK(c, hipEventRecord(*e0, c->stream));
+  return 0;
+}
+
+int timed_end(swe_ctx* c, hipEvent_t e0, hipEvent_t e1, int kind) {
+  if (!c->timing) return 0;
+  HIPCHK(c, hipEventRecord(e1, c->stream));
+  c->pending.push_back({e0, e1, kind});
+  return 0;
+}
+
+// World::erode (world.hpp:36-77) with the global-memory march: k_march
+// (R fly() calls per particle, device-scope atomics) then k_relax per window
+int erode_global(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin, int32_t R) {
+  int r;
+  if ((r = ensure_particles(c, particles))) return r;
+  swe::MarchArgs ma{};
+  ma.n = particles;
+  ma.ncell = c->ncell;
+  ma.nx = c->nx;
+  ma.ny = c->ny;
+  ma.rcp_nx = 1.0f / (float)c->nx;
+  ma.rcp_ny = 1.0f / (float)c->ny;
+  ma.mapscale = c->p.mapscale;
+  fill_consts(c, ma.ps, ma.nps);
+  ma.p = c->p;
+  ma.P0 = c->P0;
+  ma.P1 = c->P1;
+  ma.L = c->L;
+  ma.LC = c->LC;
+  ma.lost = c->lost;
+  ma.cycle = c->dcycle;
+  ma.T = c->trk;
+  ma.D = c->dz;
+  ma.DS = c->ds;
+  ma.R = R;
+
+  swe::RelaxArgs ra{};
+  ra.nx = c->nx;
+  ra.ny = c->ny;
+  ra.rcp_nx = ma.rcp_nx;
+  ra.rcp_ny = ma.rcp_ny;
+  ra.maxdiff = c->p.maxdiff;
+  ra.settling = c->p.settling;
+  ra.dz = c->dz;
+  ra.ds = c->ds;
+  const bool layers2 = c->p.layers == 2;
+
+  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+    if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
+    for (int64_t w = 0; w < nwin; w++) {
+      const int cur = c->cur();
+      ma.S = c->zs[cur];
+      ma.SS = layers2 ? c->ss[cur] : nullptr;
+      ma.V = c->vis[cur];
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if ((r = timed_begin(c, &e0, &e1))) return r;
+      swe::launch_march(ma, w == 0, c->stream);
+      HIPCHK(c, hipGetLastError());
+      if ((r = timed_end(c, e0, e1, 0))) return r;
+      ra.S = c->zs[cur];
+      ra.SS = layers2 ? c->ss[cur] : nullptr;
+      ra.V = c->vis[cur];
+      ra.v_clear = c->vis[cur ^ 1];
+      ra.zm_in = c->zmv[cur];
+      ra.zm_out = c->zmv[cur ^ 1];
+      ra.zs_out = c->zs[cur ^ 1];
+      ra.sm_in = layers2 ? c->smv[cur] : nullptr;
+      ra.sm_out = layers2 ? c->smv[cur ^ 1] : nullptr;
+      ra.ss_out = layers2 ? c->ss[cur ^ 1] : nullptr;
+      if ((r = timed_begin(c, &e0, &e1))) return r;
+      swe::launch_relax(ra, c->p.periodic != 0, c->p.layers, c->stream);
+      HIPCHK(c, hipGetLastError());
+      if ((r = timed_end(c, e0, e1, 1))) return r;
+      c->g++;
+    }
+    if (c->p.track) swe::launch_ema(c->ema, c->trk, 4 * c->ncell, c->p.lrate, c->stream);
+    swe::launch_bump(c->dcycle, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->cycle++;
+    c->st.cycles++;
+    if (c->timing && c->pending.size() > 512) {
+      if ((r = collect_timing(c))) return r;
+    }
+  }
+  return 0;
+}
+
+// World::erode with one-call windows on the tile-owned step kernel: spawn and
+// bin by tile, then one k_step per window (relax of the previous window +
+// one fly() call of every particle), then a relax-only k_step
+int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
+  int r;
+  if ((r = ensure_step(c, particles > 0 ? particles : 1))) return r;
+  const bool layers2 = c->p.layers == 2;
+  swe::StepArgs sa{};
+  sa.trk = c->trk;
+  sa.lost = c->lost;
+  sa.cycle = c->dcycle;
+  sa.seed = c->p.seed;
+  sa.ncell = c->ncell;
+  sa.nx = c->nx;
+  sa.ny = c->ny;
+  sa.tiles_x = c->tiles_x;
+  sa.tiles_y = c->tiles_y;
+  sa.rcp_nx = 1.0f / (float)c->nx;
+  sa.rcp_ny = 1.0f / (float)c->ny;
+  sa.mapscale = c->p.mapscale;
+  sa.boundarylayer = c->p.boundarylayer;
+  sa.suspension = c->p.suspension;
+  sa.gravity = c->p.gravity;
+  sa.abrasion = c->p.abrasion;
+  sa.maxdiff = c->p.maxdiff;
+  sa.settling = c->p.settling;
+  sa.max_age = c->p.max_age;
+  fill_consts(c, sa.ps, sa.nps);
+
+  const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
+  swe::SpawnArgs sp{};
+  sp.out = c->PB[0];
+  sp.tile = c->sp_tile;
+  sp.rank = c->sp_rank;
+  sp.count = c->nin;  // window 0's input counts
+  sp.base1 = c->tbase[1];
+  sp.cnt = c->tcnt[1];
+  sp.cycle = c->dcycle;
+  sp.seed = c->p.seed;
+  sp.n = particles;
+  sp.nx = c->nx;
+  sp.ny = c->ny;
+  sp.tiles_x = c->tiles_x;
+  sp.tiles_y = c->tiles_y;
+  sp.periodic = c->p.periodic;
+
+  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+    if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
+    if (particles > 0) {
+      swe::launch_zero32(c->nin, 3 * nt, c->stream);
+      swe::launch_zero32(c->farcnt, 3, c->stream);
+      swe::launch_spawn(sp, c->stream);
+      HIPCHK(c, hipGetLastError());
+      // windows 0 .. nwin-1 step the particles; window nwin only relaxes
+      for (int64_t w = 0; w <= nwin; w++) {
+        const int cur = c->cur(), nxt = cur ^ 1;
+        const int wp = (int)(w & 1);
+        sa.zm_in = c->zmv[cur];
+        sa.zm_out = c->zmv[nxt];
+        sa.snap_in = c->zs[cur];
+        sa.snap_out = c->zs[nxt];
+        sa.sm_in = layers2 ? c->smv[cur] : nullptr;
+        sa.sm_out = layers2 ? c->smv[nxt] : nullptr;
+        sa.ssnap_in = layers2 ? c->ss[cur] : nullptr;
+        sa.ssnap_out = layers2 ? c->ss[nxt] : nullptr;
+        sa.d_in = c->fD[cur];
+        sa.ds_in = layers2 ? c->fDS[cur] : nullptr;
+        sa.v_in = c->fV[cur];
+        sa.d_out = c->fD[nxt];
+        sa.ds_out = layers2 ? c->fDS[nxt] : nullptr;
+        sa.v_out = c->fV[nxt];
+        const int w3 = (int)(w % 3), w3n = (int)((w + 1) % 3), w3c = (int)((w + 2) % 3);
+        sa.in = c->PB[wp];
+        sa.out = c->PB[wp ^ 1];
+        sa.far_in = c->FB[w3];
+        sa.far_out = c->FB[w3n];
+        sa.far_cnt_in = c->farcnt + w3;
+        sa.far_cnt_out = c->farcnt + w3n;
+        sa.far_cnt_clear = c->farcnt + w3c;
+        sa.base_prev = c->tbase[wp ^ 1];
+        sa.cnt_prev = c->tcnt[wp ^ 1];
+        sa.base_cur = c->tbase[wp];
+        sa.cnt_cur = c->tcnt[wp];
+        sa.nin_cur = c->nin + w3 * nt;
+        sa.nin_next = c->nin + w3n * nt;
+        sa.nin_clear = c->nin + w3c * nt;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if ((r = timed_begin(c, &e0, &e1))) return r;
+        swe::launch_step(sa, c->p.periodic != 0, c->p.track != 0, c->p.layers, c->p.deposit_on_death != 0,
+                         w == nwin, c->stream);
+        HIPCHK(c, hipGetLastError());
+        if ((r = timed_end(c, e0, e1, 2))) return r;
+        c->g++;
+      }
+    }
+    if (c->p.track) swe::launch_ema(c->ema, c->trk, 4 * c->ncell, c->p.lrate, c->stream);
+    swe::launch_bump(c->dcycle, c->stream);
+    HIPCHK(c, hipGetLastError());
+    c->cycle++;
+    c->st.cycles++;
+    if (c->timing && c->pending.size() > 512) {
+      if ((r = collect_timing(c))) return r;
+    }
+  }
+  return 0;
 }
 
 }  // namespace
@@ -229,22 +493,32 @@ int swe_destroy(swe_ctx* c) {
   if (!c) return SWE_EINVAL;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  dfree(c->zm);
-  dfree(c->zs[0]);
-  dfree(c->zs[1]);
-  dfree(c->dz);
-  dfree(c->vis[0]);
-  dfree(c->vis[1]);
-  dfree(c->sm);
-  dfree(c->ss[0]);
-  dfree(c->ss[1]);
-  dfree(c->ds);
+  for (int k = 0; k < 2; k++) {
+    dfree(c->zmv[k]);
+    dfree(c->zs[k]);
+    dfree(c->smv[k]);
+    dfree(c->ss[k]);
+    dfree(c->vis[k]);
+    dfree(c->fD[k]);
+    dfree(c->fDS[k]);
+    dfree(c->fV[k]);
+    dfree(c->tbase[k]);
+    dfree(c->tcnt[k]);
+    free_pbuf(c->PB[k]);
+  }
+  for (int k = 0; k < 3; k++) free_pbuf(c->FB[k]);
   dfree(c->trk);
   dfree(c->ema);
+  dfree(c->dz);
+  dfree(c->ds);
   dfree(c->P0);
   dfree(c->P1);
   dfree(c->L);
   dfree(c->LC);
+  dfree(c->farcnt);
+  dfree(c->nin);
+  dfree(c->sp_tile);
+  dfree(c->sp_rank);
   dfree(c->lost);
   dfree(c->dcycle);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -266,14 +540,17 @@ int swe_create(swe_ctx** out, int32_t nx, int32_t ny, int32_t device) {
   c->nx = nx;
   c->ny = ny;
   c->ncell = (int64_t)nx * ny;
+  c->tiles_x = (nx + swe::kStepTileX - 1) / swe::kStepTileX;
+  c->tiles_y = (ny + swe::kStepTileY - 1) / swe::kStepTileY;
   swe_params_default(&c->p);
   int r = 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     swe_destroy(c);
     return SWE_EHIP;
   }
-  if ((r = dalloc(c, &c->zm, c->ncell)) || (r = dalloc(c, &c->zs[0], c->ncell)) ||
-      (r = dalloc(c, &c->zs[1], c->ncell)) || (r = dalloc(c, &c->dz, c->ncell)) ||
+  if ((r = dalloc(c, &c->zmv[0], c->ncell)) || (r = dalloc(c, &c->zmv[1], c->ncell)) ||
+      (r = dalloc(c, &c->zs[0], c->ncell)) || (r = dalloc(c, &c->zs[1], c->ncell)) ||
+      (r = dalloc(c, &c->dz, c->ncell)) ||
       (r = dalloc(c, &c->vis[0], c->ncell)) || (r = dalloc(c, &c->vis[1], c->ncell)) ||
       (r = dalloc(c, &c->ema, 4 * c->ncell)) || (r = dalloc(c, &c->lost, swe::kLostSlots)) ||
       (r = dalloc(c, &c->dcycle, 1)) || (r = ensure_layers(c))) {
@@ -293,7 +570,7 @@ int swe_set_params(swe_ctx* c, const swe_params* p) {
   int r = check_params(c, p);
   if (r) return r;
   HIPCHK(c, hipSetDevice(c->device));
-  const bool add_layer = p->layers == 2 && c->p.layers != 2 && !c->sm;
+  const bool add_layer = p->layers == 2 && c->p.layers != 2 && !c->smv[0];
   c->p = *p;
   if ((r = ensure_layers(c))) return r;
   if (add_layer) {
@@ -316,22 +593,23 @@ int swe_upload_layers(swe_ctx* c, const float* height, const float* sediment) {
   HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * c->ncell));
   hipError_t e = hipMemcpyAsync(tmp, height, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    swe::launch_from_f32(tmp, c->zm, c->zs[c->cur], c->ncell, c->stream);
+    swe::launch_from_f32(tmp, c->zmv[c->cur()], c->zs[c->cur()], c->ncell, c->stream);
     e = hipGetLastError();
   }
   if (e == hipSuccess && sediment) {
     int r = 0;
-    if (!c->sm) {
+    if (!c->smv[0]) {
       // a sediment layer can be uploaded before layers == 2 is selected
-      if ((r = dalloc(c, &c->sm, c->ncell)) || (r = dalloc(c, &c->ss[0], c->ncell)) ||
-          (r = dalloc(c, &c->ss[1], c->ncell)) || (r = dalloc(c, &c->ds, c->ncell))) {
+      if ((r = dalloc(c, &c->smv[0], c->ncell)) || (r = dalloc(c, &c->smv[1], c->ncell)) ||
+          (r = dalloc(c, &c->ss[0], c->ncell)) || (r = dalloc(c, &c->ss[1], c->ncell)) ||
+          (r = dalloc(c, &c->ds, c->ncell))) {
         (void)hipFree(tmp);
         return r;
       }
     }
     e = hipMemcpyAsync(tmp, sediment, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-      swe::launch_from_f32(tmp, c->sm, c->ss[c->cur], c->ncell, c->stream);
+      swe::launch_from_f32(tmp, c->smv[c->cur()], c->ss[c->cur()], c->ncell, c->stream);
       e = hipGetLastError();
     }
   }
@@ -346,14 +624,14 @@ int swe_upload_master(swe_ctx* c, const int64_t* height_fx, const int64_t* sedim
   if (!c) return SWE_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
   if (height_fx) {
-    HIPCHK(c, hipMemcpyAsync(c->zm, height_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
-    swe::launch_snap(c->zm, c->zs[c->cur], c->ncell, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->zmv[c->cur()], height_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    swe::launch_snap(c->zmv[c->cur()], c->zs[c->cur()], c->ncell, c->stream);
     HIPCHK(c, hipGetLastError());
   }
   if (sediment_fx) {
-    if (!c->sm) return fail(c, SWE_ESTATE, "no sediment layer: set layers = 2 first");
-    HIPCHK(c, hipMemcpyAsync(c->sm, sediment_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
-    swe::launch_snap(c->sm, c->ss[c->cur], c->ncell, c->stream);
+    if (!c->smv[0]) return fail(c, SWE_ESTATE, "no sediment layer: set layers = 2 first");
+    HIPCHK(c, hipMemcpyAsync(c->smv[c->cur()], sediment_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
+    swe::launch_snap(c->smv[c->cur()], c->ss[c->cur()], c->ncell, c->stream);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -373,10 +651,10 @@ int swe_upload_fields(swe_ctx* c, const float* massflow, const float* momentum) 
 int swe_download_layers(swe_ctx* c, float* height, float* sediment, float* massflow, float* momentum) {
   if (!c) return SWE_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
-  if (height) HIPCHK(c, hipMemcpyAsync(height, c->zs[c->cur], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  if (height) HIPCHK(c, hipMemcpyAsync(height, c->zs[c->cur()], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
   if (sediment) {
-    if (c->ss[c->cur])
-      HIPCHK(c, hipMemcpyAsync(sediment, c->ss[c->cur], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    if (c->ss[c->cur()])
+      HIPCHK(c, hipMemcpyAsync(sediment, c->ss[c->cur()], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
     else
       std::memset(sediment, 0, sizeof(float) * c->ncell);
   }
@@ -390,10 +668,10 @@ int swe_download_layers(swe_ctx* c, float* height, float* sediment, float* massf
 int swe_download_master(swe_ctx* c, int64_t* height_fx, int64_t* sediment_fx) {
   if (!c) return SWE_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));
-  if (height_fx) HIPCHK(c, hipMemcpyAsync(height_fx, c->zm, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  if (height_fx) HIPCHK(c, hipMemcpyAsync(height_fx, c->zmv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
   if (sediment_fx) {
-    if (c->sm)
-      HIPCHK(c, hipMemcpyAsync(sediment_fx, c->sm, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    if (c->smv[0])
+      HIPCHK(c, hipMemcpyAsync(sediment_fx, c->smv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
     else
       std::memset(sediment_fx, 0, sizeof(int64_t) * c->ncell);
   }
@@ -408,95 +686,18 @@ int swe_erode(swe_ctx* c, int64_t particles, int32_t n_cycles) {
   HIPCHK(c, hipSetDevice(c->device));
   int r;
   if ((r = ensure_layers(c))) return r;
-  if ((r = ensure_particles(c, particles))) return r;
-
-  swe::MarchArgs ma{};
-  ma.n = particles;
-  ma.ncell = c->ncell;
-  ma.nx = c->nx;
-  ma.ny = c->ny;
-  ma.rcp_nx = 1.0f / (float)c->nx;
-  ma.rcp_ny = 1.0f / (float)c->ny;
-  ma.mapscale = c->p.mapscale;
-  for (int k = 0; k < 3; k++) ma.ps[k] = c->p.pspeed[k];
-  normalize3(c->p.pspeed, ma.nps);
-  ma.p = c->p;
-  ma.P0 = c->P0;
-  ma.P1 = c->P1;
-  ma.L = c->L;
-  ma.LC = c->LC;
-  ma.lost = c->lost;
-  ma.cycle = c->dcycle;
-  ma.T = c->trk;
-  ma.D = c->dz;
-  ma.DS = c->ds;
-
-  swe::RelaxArgs ra{};
-  ra.nx = c->nx;
-  ra.ny = c->ny;
-  ra.rcp_nx = ma.rcp_nx;
-  ra.rcp_ny = ma.rcp_ny;
-  ra.maxdiff = c->p.maxdiff;
-  ra.settling = c->p.settling;
-  ra.zm = c->zm;
-  ra.dz = c->dz;
-  ra.sm = c->sm;
-  ra.ds = c->ds;
-
   int32_t R;
   const int64_t nwin = windows_of(c->p, &R);
-  ma.R = R;
-  const bool layers2 = c->p.layers == 2;
+  const bool tiled = c->path == 2 || (c->path == 0 && R == 1);
+  if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path needs window == 1");
+  if (tiled) return erode_tiled(c, particles, n_cycles, nwin);
+  return erode_global(c, particles, n_cycles, nwin, R);
+}
 
-  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
-    if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
-    for (int64_t w = 0; w < nwin; w++) {
-      const int cur = c->cur;
-      ma.S = c->zs[cur];
-      ma.SS = layers2 ? c->ss[cur] : nullptr;
-      ma.V = c->vis[cur];
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      if (c->timing) {
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (!e0 || !e1) return fail(c, SWE_EHIP, "hipEventCreate failed");
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-      }
-      swe::launch_march(ma, w == 0, c->stream);
-      HIPCHK(c, hipGetLastError());
-      if (c->timing) {
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        c->pending.push_back({e0, e1, 0});
-      }
-      ra.S = c->zs[cur];
-      ra.SS = layers2 ? c->ss[cur] : nullptr;
-      ra.V = c->vis[cur];
-      ra.v_clear = c->vis[cur ^ 1];
-      ra.zs_out = c->zs[cur ^ 1];
-      ra.ss_out = layers2 ? c->ss[cur ^ 1] : nullptr;
-      if (c->timing) {
-        e0 = take_event(c);
-        e1 = take_event(c);
-        if (!e0 || !e1) return fail(c, SWE_EHIP, "hipEventCreate failed");
-        HIPCHK(c, hipEventRecord(e0, c->stream));
-      }
-      swe::launch_relax(ra, c->p.periodic != 0, c->p.layers, c->stream);
-      HIPCHK(c, hipGetLastError());
-      if (c->timing) {
-        HIPCHK(c, hipEventRecord(e1, c->stream));
-        c->pending.push_back({e0, e1, 1});
-      }
-      c->cur ^= 1;
-    }
-    if (c->p.track) swe::launch_ema(c->ema, c->trk, 4 * c->ncell, c->p.lrate, c->stream);
-    swe::launch_bump(c->dcycle, c->stream);
-    HIPCHK(c, hipGetLastError());
-    c->cycle++;
-    c->st.cycles++;
-    if (c->timing && c->pending.size() > 512) {
-      if ((r = collect_timing(c))) return r;
-    }
-  }
+int swe_set_path(swe_ctx* c, int32_t path) {
+  if (!c) return SWE_EINVAL;
+  if (path < 0 || path > 2) return fail(c, SWE_EINVAL, "path must be 0 (auto), 1 (global march) or 2 (tile step)");
+  c->path = path;
   return 0;
 }
 
@@ -512,10 +713,10 @@ int swe_audit(swe_ctx* c, swe_audit_t* out) {
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<int64_t> z((size_t)c->ncell), s;
   int64_t lost[swe::kLostSlots];
-  HIPCHK(c, hipMemcpyAsync(z.data(), c->zm, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
-  if (c->sm) {
+  HIPCHK(c, hipMemcpyAsync(z.data(), c->zmv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  if (c->smv[0]) {
     s.resize((size_t)c->ncell);
-    HIPCHK(c, hipMemcpyAsync(s.data(), c->sm, sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s.data(), c->smv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHK(c, hipMemcpyAsync(lost, c->lost, sizeof(lost), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -587,7 +788,7 @@ int swe_normals(swe_ctx* c, float* out) {
   HIPCHK(c, hipSetDevice(c->device));
   float* tmp = nullptr;
   HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 3 * c->ncell));
-  swe::launch_normals(c->zs[c->cur], tmp, c->nx, c->ny, c->p.mapscale, c->p.periodic != 0, c->stream);
+  swe::launch_normals(c->zs[c->cur()], tmp, c->nx, c->ny, c->p.mapscale, c->p.periodic != 0, c->stream);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 3 * c->ncell, hipMemcpyDeviceToHost, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);

@@ -40,10 +40,12 @@ struct RelaxArgs {
   const float* SS;
   const uint32_t* V;   // window event counts (read, with halo)
   uint32_t* v_clear;   // the other event buffer, zeroed for the next window
-  int64_t* zm;         // master height (read-modify-write)
-  int64_t* dz;         // pickup deltas (read, then zeroed)
-  float* zs_out;       // next snapshot
-  int64_t* sm;
+  const int64_t* zm_in;  // master height of the window
+  int64_t* zm_out;       // master height after the window
+  int64_t* dz;           // pickup deltas (read, then zeroed)
+  float* zs_out;         // next snapshot
+  const int64_t* sm_in;
+  int64_t* sm_out;
   int64_t* ds;
   float* ss_out;
   int32_t nx, ny;

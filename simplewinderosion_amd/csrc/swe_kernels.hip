@@ -22,193 +22,111 @@
 //              ledger int64 (fixed-point load), lastc int32.
 #include "swe_kernels.h"
 
-#include "swe_device.hpp"
+#include "swe_cascade.hpp"
+#include "swe_fly.hpp"
 
 namespace swe {
 
 // ---------------------------------------------------------------------------
 // Particle march
 // ---------------------------------------------------------------------------
+// Global-memory stencil: five-point heights of the window's snapshot
+template <bool PERIODIC>
+struct GlobalMap {
+  const float* S;
+  const float* SS;
+  int32_t cx, cy, nx, ny;
+  __device__ float h(int dx, int dy) const {
+    int32_t x = cx + dx, y = cy + dy;
+    if (PERIODIC) {
+      x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+      y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+    }
+    return S[(int64_t)x * ny + y];
+  }
+  __device__ float s0() const { return SS[(int64_t)cx * ny + cy]; }
+};
+
+// Device-scope atomics into the window's delta, event and track maps
+struct GlobalAcc {
+  int64_t* D;
+  int64_t* DS;
+  uint32_t* V;
+  int64_t* T;
+  int64_t ncell;
+  __device__ void add_pickup(int64_t cell, int64_t q, bool both) {
+    atomicAdd((unsigned long long*)&D[cell], (unsigned long long)q);
+    if (both) atomicAdd((unsigned long long*)&DS[cell], (unsigned long long)q);
+  }
+  __device__ void add_event(int64_t cell) { atomicAdd(&V[cell], 1u); }
+  __device__ void add_track(int64_t cell, float sed, f3 sp) {
+    atomicAdd((unsigned long long*)&T[cell], (unsigned long long)fx_from_f32(sed));
+    atomicAdd((unsigned long long*)&T[ncell + cell], (unsigned long long)fx_from_f32(sp.x));
+    atomicAdd((unsigned long long*)&T[2 * ncell + cell], (unsigned long long)fx_from_f32(sp.y));
+    atomicAdd((unsigned long long*)&T[3 * ncell + cell], (unsigned long long)fx_from_f32(sp.z));
+  }
+};
+
 template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool FIRST>
 __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t lost_here = 0;
   if (i < a.n) {
-    const int32_t nx = a.nx, ny = a.ny;
+    FlyConsts k;
+    k.nx = a.nx;
+    k.ny = a.ny;
+    k.rcp_nx = a.rcp_nx;
+    k.rcp_ny = a.rcp_ny;
+    k.mapscale = a.mapscale;
+    k.max_age = a.p.max_age;
+    k.boundarylayer = a.p.boundarylayer;
+    k.suspension = a.p.suspension;
+    k.gravity = a.p.gravity;
+    k.abrasion = a.p.abrasion;
+    k.ps = mk(a.ps[0], a.ps[1], a.ps[2]);
+    k.nps = mk(a.nps[0], a.nps[1], a.nps[2]);
     const uint64_t pk = particle_key(cycle_key(a.p.seed, *a.cycle), (uint64_t)i);
-    float px, py, pz, sed, sx, sy, sz;
-    int32_t age;
-    int64_t ledger;
-    int32_t lastc;
+    Particle w;
     if (FIRST) {
-      // world.hpp:52-54,62 / wind.hpp:6-13 on the counter RNG
-      const uint64_t ra = mix64(pk + 1 * kGolden);
-      const uint64_t rb = mix64(pk + 2 * kGolden);
-      const int32_t ix = (int32_t)((uint32_t)ra % (uint32_t)nx);
-      const int32_t iy = (int32_t)((uint32_t)(ra >> 32) % (uint32_t)ny);
-      const int32_t fxp = (int32_t)((uint32_t)rb % 1000u);
-      const int32_t fyp = (int32_t)((uint32_t)(rb >> 32) % 1000u);
-      px = (float)ix + (float)fxp / 1000.0f;
-      pz = (float)iy + (float)fyp / 1000.0f;
-      py = 0.0f;
-      sed = 0.0f;
-      sx = sy = sz = 0.0f;
-      age = 0;
-      ledger = 0;
-      lastc = 0;
+      spawn(pk, a.nx, a.ny, w);
     } else {
       const float4 q0 = a.P0[i];
       const float4 q1 = a.P1[i];
-      px = q0.x; py = q0.y; pz = q0.z; sed = q0.w;
-      sx = q1.x; sy = q1.y; sz = q1.z;
-      age = (int32_t)f2u(q1.w);
-      ledger = a.L[i];
-      lastc = a.LC[i];
+      w.px = q0.x; w.py = q0.y; w.pz = q0.z; w.sed = q0.w;
+      w.sx = q1.x; w.sy = q1.y; w.sz = q1.z;
+      w.age = (int32_t)f2u(q1.w);
+      w.ledger = a.L[i];
+      w.lastc = a.LC[i];
     }
-    const f3 ps = mk(a.ps[0], a.ps[1], a.ps[2]);
-    const f3 nps = mk(a.nps[0], a.nps[1], a.nps[2]);
-    for (int32_t s = 0; s < a.R && age >= 0; s++) {
-      // wind.hpp:30 ipos = round(vec2(pos.x, pos.z)), half away from zero
-      const float rx = __builtin_roundf(px), rz = __builtin_roundf(pz);
+    GlobalAcc acc{a.D, a.DS, a.V, a.T, a.ncell};
+    for (int32_t s = 0; s < a.R && w.age >= 0; s++) {
+      int32_t cx, cy;
       int64_t dead_cell = -1;
-      bool die = false;
-      if (!(__builtin_fabsf(rx) < 1073741824.0f) || !(__builtin_fabsf(rz) < 1073741824.0f)) {
-        die = true;
-      }
-      int32_t ix = die ? 0 : (int32_t)rx, iy = die ? 0 : (int32_t)rz;
-      if (!PERIODIC && !die && (ix < 0 || iy < 0 || ix >= nx || iy >= ny)) die = true;  // :32-38
-      int32_t cx = 0, cy = 0;
-      f3 n = mk(0.0f, 0.0f, 0.0f);
-      float h = 0.0f;
+      bool die = !cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);  // wind.hpp:30-38
       if (!die) {
-        if (PERIODIC) {
-          cx = wrap(ix, nx, a.rcp_nx);
-          cy = wrap(iy, ny, a.rcp_ny);
-        } else {
-          cx = ix;
-          cy = iy;
-        }
-        // five-point stencil of quad::_normal (cellpool.h:200-223)
-        const int64_t row = (int64_t)cx * ny;
-        h = a.S[row + cy];
-        float hyp = 0.0f, hym = 0.0f, hxp = 0.0f, hxm = 0.0f;
-        bool g1, g2, g3, g4;
-        if (PERIODIC) {
-          const int32_t xp = cx + 1 == nx ? 0 : cx + 1;
-          const int32_t xm = cx == 0 ? nx - 1 : cx - 1;
-          const int32_t yp = cy + 1 == ny ? 0 : cy + 1;
-          const int32_t ym = cy == 0 ? ny - 1 : cy - 1;
-          hyp = a.S[row + yp];
-          hym = a.S[row + ym];
-          hxp = a.S[(int64_t)xp * ny + cy];
-          hxm = a.S[(int64_t)xm * ny + cy];
-          g1 = g2 = g3 = g4 = true;
-        } else {
-          const bool bxp = cx + 1 < nx, bxm = cx > 0, byp = cy + 1 < ny, bym = cy > 0;
-          g1 = bxp && byp;
-          g2 = bxm && bym;
-          g3 = bxp && bym;
-          g4 = bxm && byp;
-          if (byp) hyp = a.S[row + cy + 1];
-          if (bym) hym = a.S[row + cy - 1];
-          if (bxp) hxp = a.S[row + ny + cy];
-          if (bxm) hxm = a.S[row - ny + cy];
-        }
-        n = surface_normal(h, hyp, hxp, hym, hxm, g1, g2, g3, g4, a.mapscale);
-        if (age == 0 || py < h) py = h;  // :42-43
-        if (age > a.p.max_age) {         // :45-46
+        GlobalMap<PERIODIC> m{a.S, a.SS, cx, cy, a.nx, a.ny};
+        if (fly<PERIODIC, TRACK, LAYERS>(w, pk, cx, cy, k, m, acc) == kDied) {
           die = true;
-          dead_cell = row + cy;
+          dead_cell = (int64_t)cx * a.ny + cy;
         }
       }
       if (die) {
         if (DEPOSIT) {
-          if (ledger != 0) {
-            const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)lastc;
-            atomicAdd((unsigned long long*)&a.D[c], (unsigned long long)ledger);
-            if (LAYERS == 2) atomicAdd((unsigned long long*)&a.DS[c], (unsigned long long)ledger);
+          if (w.ledger != 0) {
+            const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)w.lastc;
+            atomicAdd((unsigned long long*)&a.D[c], (unsigned long long)w.ledger);
+            if (LAYERS == 2) atomicAdd((unsigned long long*)&a.DS[c], (unsigned long long)w.ledger);
           }
         } else {
-          lost_here += ledger;
+          lost_here += w.ledger;
         }
-        age = -1;
-        break;
+        w.age = -1;
       }
-      const int32_t call = age;
-      age = age + 1;
-      const int64_t cell = (int64_t)cx * ny + cy;
-
-      float hfac = exp_f32(-(py - h) / a.p.boundarylayer);  // :50-52
-      if (hfac < 0.0f) hfac = 0.0f;
-
-      float shadow = dot(nps, n);  // :56-59
-      if (shadow < 0.0f) shadow = 0.0f;
-      shadow = 1.0f - shadow;
-
-      f3 sp = mk(sx, sy, sz);
-      sp = add(sp, smul(0.05f, sub(muls(ps, 0.1f + 0.9f * shadow), sp)));  // :61
-
-      if (py > h) sp.y -= a.p.gravity * sed;  // :65-66
-
-      float collision = -dot(normalize(sp), n);  // :70-71
-      if (collision < 0.0f) collision = 0.0f;
-
-      const f3 rspeed = cross(n, cross(smul(1.0f - collision, sp), n));  // :75
-
-      sp = add(sp, smul(0.9f, sub(smul(shadow, mix(ps, rspeed, shadow * hfac)), sp)));  // :79
-
-      {  // :83 turbulence, three draws in [0,1000] for this call
-        const uint64_t t = mix64(pk + (uint64_t)(call + 3) * kGolden);
-        const float r0 = (float)(int32_t)((t & 0x1fffffull) % 1001ull);
-        const float r1 = (float)(int32_t)(((t >> 21) & 0x1fffffull) % 1001ull);
-        const float r2 = (float)(int32_t)(((t >> 42) & 0x1fffffull) % 1001ull);
-        const float k = 0.1f * hfac * collision;
-        sp = add(sp, divs(smul(k, subs(mk(r0, r1, r2), 500.0f)), 500.0f));
-      }
-
-      sp = muls(sp, (float)(1.0 - 0.3 * (double)sed));  // :87 (double, then float)
-
-      px += sp.x;  // :91
-      py += sp.y;
-      pz += sp.z;
-
-      if (TRACK) {  // :93-96 on the pre-move cell
-        atomicAdd((unsigned long long*)&a.T[cell], (unsigned long long)fx_from_f32(sed));
-        atomicAdd((unsigned long long*)&a.T[a.ncell + cell], (unsigned long long)fx_from_f32(sp.x));
-        atomicAdd((unsigned long long*)&a.T[2 * a.ncell + cell], (unsigned long long)fx_from_f32(sp.y));
-        atomicAdd((unsigned long long*)&a.T[3 * a.ncell + cell], (unsigned long long)fx_from_f32(sp.z));
-      }
-
-      float force = -dot(normalize(sp), n) * length(sp);  // :100-102
-      if (force < 0.0f) force = 0.0f;
-      const float lift = (1.0f - collision) * length(sp);  // :104
-      const float capacity = force * hfac + 0.02f * lift * hfac;  // :106
-
-      const float diff = capacity - sed;  // :110-112
-      float q;
-      if (LAYERS == 2 && diff > 0.0f && a.SS[cell] <= 0.0f) {
-        q = a.p.abrasion * diff;  // bare bedrock: abrasion rate, sediment layer untouched
-        const int64_t qf = fx_from_f32(q);
-        atomicAdd((unsigned long long*)&a.D[cell], (unsigned long long)(-qf));
-        ledger += qf;
-      } else {
-        q = a.p.suspension * diff;
-        const int64_t qf = fx_from_f32(q);
-        atomicAdd((unsigned long long*)&a.D[cell], (unsigned long long)(-qf));
-        if (LAYERS == 2) atomicAdd((unsigned long long*)&a.DS[cell], (unsigned long long)(-qf));
-        ledger += qf;
-      }
-      sed += q;
-      sx = sp.x;
-      sy = sp.y;
-      sz = sp.z;
-      lastc = (int32_t)cell;
-      atomicAdd(&a.V[cell], 1u);  // :114 World::cascade(ipos), applied by k_relax
     }
-    a.P0[i] = make_float4(px, py, pz, sed);
-    a.P1[i] = make_float4(sx, sy, sz, u2f((uint32_t)age));
-    a.L[i] = ledger;
-    a.LC[i] = lastc;
+    a.P0[i] = make_float4(w.px, w.py, w.pz, w.sed);
+    a.P1[i] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
+    a.L[i] = w.ledger;
+    a.LC[i] = w.lastc;
   }
   if (!DEPOSIT) {
     // carried-off mass: one atomic per wave into one of kLostSlots counters
@@ -236,101 +154,6 @@ constexpr int kRelaxThreads = 256;
 // world.hpp:83-92 neighbour order
 constexpr int32_t c_offx[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int32_t c_offy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
-
-// Settling factor of a cell's `events` cascade events in one window:
-// settling for one event, else 1 - (1 - settling)^events by binary
-// exponentiation (least significant bit first) in float.
-__device__ __forceinline__ float settle_factor(float settling, uint32_t events) {
-  if (events <= 1u) return settling;
-  float b = 1.0f - settling, r = 1.0f;
-  uint32_t e = events;
-  while (e) {
-    if (e & 1u) r = r * b;
-    b = b * b;
-    e >>= 1;
-  }
-  return 1.0f - r;
-}
-
-// One cell's cascade of the window: World::cascade (world.hpp:79-153)
-// against the snapshot with the window's settling factor.  hn[k] / sn[k]:
-// neighbour k's height / sediment in the fixed offset order
-// (world.hpp:83-92); hc / sc: the centre's; bit k of validmask set for
-// in-map neighbours.  The neighbours are sorted by height descending, ties
-// in offset order (what the stable insertion sort std::sort uses on eight
-// elements does), by a sorting network on (height, offset) keys; the walk
-// then evolves the centre.  amt[k] = fixed-point amount neighbour k receives.
-template <int LAYERS>
-__device__ __forceinline__ void cascade_cell(const float hn[8], const float sn[8], float hc, float sc,
-                                             uint32_t validmask, float settling, float maxdiff, int64_t amt[8]) {
-  float key[8], sed[8];
-  int32_t idx[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
-    sed[k] = LAYERS == 2 ? sn[k] : 0.0f;
-    idx[k] = k;
-  }
-#define SWE_CAS(i, j)                                                                  \
-  {                                                                                    \
-    const bool sw = key[j] > key[i] || (key[j] == key[i] && idx[j] < idx[i]);          \
-    const float tk = sw ? key[j] : key[i];                                             \
-    const int32_t ti = sw ? idx[j] : idx[i];                                           \
-    key[j] = sw ? key[i] : key[j];                                                     \
-    idx[j] = sw ? idx[i] : idx[j];                                                     \
-    key[i] = tk;                                                                       \
-    idx[i] = ti;                                                                       \
-    if (LAYERS == 2) {                                                                 \
-      const float ts = sw ? sed[j] : sed[i];                                           \
-      sed[j] = sw ? sed[i] : sed[j];                                                   \
-      sed[i] = ts;                                                                     \
-    }                                                                                  \
-  }
-  SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
-  SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
-  SWE_CAS(0, 1) SWE_CAS(2, 3) SWE_CAS(4, 5) SWE_CAS(6, 7)
-  SWE_CAS(2, 4) SWE_CAS(3, 5)
-  SWE_CAS(1, 4) SWE_CAS(3, 6)
-  SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
-#undef SWE_CAS
-  const float dd = __builtin_sqrtf(2.0f);
-  int64_t got[8];  // in sorted slot order
-#pragma unroll
-  for (int r = 0; r < 8; r++) {
-    got[r] = 0;
-    const int32_t k = idx[r];
-    if (!((validmask >> k) & 1u)) continue;
-    const float diff = hc - key[r];
-    if (diff == 0.0f) continue;
-    const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
-    const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
-    if (excess <= 0.0f) continue;
-    float t = settling * excess / 2.0f;
-    if (LAYERS == 2) {
-      const float avail = diff > 0.0f ? sc : sed[r];
-      if (t > avail) t = avail;
-      if (!(t > 0.0f)) continue;
-    }
-    const int64_t tf = fx_from_f32(t);
-    if (diff > 0.0f) {
-      hc -= t;
-      sc -= t;
-      got[r] = tf;
-    } else {
-      hc += t;
-      sc += t;
-      got[r] = -tf;
-    }
-  }
-  // back to offset order
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    int64_t v = 0;
-#pragma unroll
-    for (int r = 0; r < 8; r++) v = idx[r] == k ? got[r] : v;
-    amt[k] = v;
-  }
-}
 
 template <bool PERIODIC, int LAYERS>
 __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
@@ -413,7 +236,7 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
     const float hc = sh[(lx + 1) * kHY + ly + 1];
     const float sc = LAYERS == 2 ? shs[(lx + 1) * kHY + ly + 1] : 0.0f;
     int64_t amt[8];
-    cascade_cell<LAYERS>(hn, sn, hc, sc, valid, settle_factor(a.settling, v), a.maxdiff, amt);
+    cascade_cell_dev<LAYERS>(hn, sn, hc, sc, valid, settle_factor_dev(a.settling, v), a.maxdiff, amt);
 #pragma unroll
     for (int k = 0; k < 8; k++) out[k] = amt[k];
   }
@@ -435,15 +258,15 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
       cas += sx[ry * 8 + k];
     }
     const int64_t c = (int64_t)gx * ny + gy;
-    const int64_t z = a.zm[c] + a.dz[c] + cas;
-    a.zm[c] = z;
+    const int64_t z = a.zm_in[c] + a.dz[c] + cas;
+    a.zm_out[c] = z;
     a.zs_out[c] = f32_from_fx(z);
     a.dz[c] = 0;
     a.v_clear[c] = 0;
     if (LAYERS == 2) {
-      int64_t s = a.sm[c] + a.ds[c] + cas;
+      int64_t s = a.sm_in[c] + a.ds[c] + cas;
       if (s < 0) s = 0;
-      a.sm[c] = s;
+      a.sm_out[c] = s;
       a.ss_out[c] = f32_from_fx(s);
       a.ds[c] = 0;
     }

@@ -1,0 +1,79 @@
+// swe_step.h -- argument blocks of the tile-owned step kernel (swe_step.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace swe {
+
+constexpr int kStepTileX = 16;  // tile extent along x (slow axis)
+constexpr int kStepTileY = 32;  // tile extent along y (fast axis)
+
+// One particle buffer (structure of arrays)
+struct ParticleBuf {
+  float4* P0;     // pos.xyz, sediment
+  float4* P1;     // speed.xyz, age bits
+  int64_t* L;     // fixed-point load
+  int32_t* LC;    // last pre-move cell
+  uint32_t* PID;  // particle index (RNG stream)
+  uint8_t* CODE;  // direction (0..8) of the tile it occupies from the writing tile
+};
+
+struct StepArgs {
+  // map, window parity in -> out
+  const int64_t* zm_in;
+  int64_t* zm_out;
+  const float* snap_in;
+  float* snap_out;
+  const int64_t* sm_in;
+  int64_t* sm_out;
+  const float* ssnap_in;
+  float* ssnap_out;
+  const int64_t* d_in;   // previous window's pickups
+  const int64_t* ds_in;
+  const uint32_t* v_in;  // previous window's cascade events
+  int64_t* d_out;        // this window's pickups
+  int64_t* ds_out;
+  uint32_t* v_out;
+  int64_t* trk;          // 4 x ncell tracks of the cycle
+  int64_t* lost;         // kLostSlots counters
+  // particles
+  ParticleBuf in, out, far_in, far_out;
+  uint32_t* far_cnt_in;
+  uint32_t* far_cnt_out;
+  uint32_t* far_cnt_clear;   // the far counter of the window after next
+  const int64_t* base_prev;  // segment base per tile of `in`
+  const int64_t* cnt_prev;   // segment length per tile of `in`
+  int64_t* base_cur;         // segment base per tile of `out` (written here)
+  int64_t* cnt_cur;          // segment length per tile of `out`
+  const uint32_t* nin_cur;   // particles each tile receives this window
+  uint32_t* nin_next;        // ... next window (accumulated here)
+  uint32_t* nin_clear;       // ... the window after next (zeroed here)
+  const uint64_t* cycle;
+  uint64_t seed;
+  int64_t ncell;
+  int32_t nx, ny, tiles_x, tiles_y;
+  float rcp_nx, rcp_ny;
+  float mapscale, boundarylayer, suspension, gravity, abrasion, maxdiff, settling;
+  int32_t max_age;
+  float ps[3], nps[3];
+};
+
+struct SpawnArgs {
+  ParticleBuf out;
+  int32_t* tile;     // per particle
+  uint32_t* rank;    // per particle
+  uint32_t* count;   // per tile, zero on entry: window 0's input counts
+  int64_t* base1;    // per tile: segment base of the spawned particles
+  int64_t* cnt;      // per tile: segment length
+  const uint64_t* cycle;
+  uint64_t seed;
+  int64_t n;
+  int32_t nx, ny, tiles_x, tiles_y, periodic;
+};
+
+void launch_step(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,
+                 hipStream_t s);
+void launch_spawn(const SpawnArgs& a, hipStream_t s);
+
+}  // namespace swe

@@ -1,0 +1,581 @@
+// swe_step.hip -- the tile-owned step kernel for one-call snapshot windows.
+//
+// With window == 1 every fly() call writes only to the particle's pre-move
+// cell, i.e. the cell it occupies when the window starts.  k_step gives each
+// workgroup one tile of kSX x kSY cells and the particles currently in it,
+// so all of a window's writes land in LDS of the owning workgroup and reach
+// HBM as plain coalesced stores -- no global atomics.  One launch per window:
+//
+//   1. relax of the previous window (the k_relax stencil) for the tile plus
+//      a one-cell halo, recomputed redundantly from a three-cell snapshot
+//      halo in LDS: master/snapshot of the tile written out, the halo kept in
+//      LDS for the particles' five-point stencils;
+//   2. one fly() call (wind.hpp:28-117) for every particle in the tile
+//      against the LDS snapshot; pickups, cascade-event counts and tracks
+//      summed in LDS (int64 / uint32) and stored once per cell;
+//   3. routing: each particle is appended to its tile's segment of the
+//      output particle buffer tagged with the direction of the tile it now
+//      occupies (one of nine); per-direction counts are added to the
+//      destination tiles' input counts for the next window, where every
+//      workgroup turns them into its own segment base with a block-wide sum.
+//
+// A particle that jumps more than one tile (never seen in practice) goes to a
+// "far" list that every workgroup scans for particles in its tile.
+#include "swe_cascade.hpp"
+#include "swe_fly.hpp"
+#include "swe_kernels.h"
+#include "swe_step.h"
+
+namespace swe {
+
+constexpr int kSX = kStepTileX, kSY = kStepTileY;
+constexpr int kH3X = kSX + 6, kH3Y = kSY + 6;  // snapshot halo 3
+constexpr int kH2X = kSX + 4, kH2Y = kSY + 4;  // events halo 2
+constexpr int kH1X = kSX + 2, kH1Y = kSY + 2;  // new values halo 1
+constexpr int kStepThreads = 256;
+constexpr int kCell = kSX * kSY;
+
+constexpr int32_t c_ox[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+constexpr int32_t c_oy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+
+// LDS-backed stencil and accumulator for one tile
+struct TileMap {
+  const float* s_new;   // H1 heights
+  const float* s_snew;  // H1 sediment
+  int32_t c;            // H1 index of the particle's cell
+  __device__ float h(int dx, int dy) const { return s_new[c + dx * kH1Y + dy]; }
+  __device__ float s0() const { return s_snew[c]; }
+};
+
+struct TileAcc {
+  int64_t* d;
+  int64_t* ds;
+  uint32_t* v;
+  int64_t* t;
+  int32_t li;  // tile-local index of the particle's cell
+  __device__ void add_pickup(int64_t, int64_t q, bool both) {
+    atomicAdd((unsigned long long*)&d[li], (unsigned long long)q);
+    if (both) atomicAdd((unsigned long long*)&ds[li], (unsigned long long)q);
+  }
+  __device__ void add_event(int64_t) { atomicAdd(&v[li], 1u); }
+  __device__ void add_track(int64_t, float sed, f3 sp) {
+    atomicAdd((unsigned long long*)&t[li], (unsigned long long)fx_from_f32(sed));
+    atomicAdd((unsigned long long*)&t[kCell + li], (unsigned long long)fx_from_f32(sp.x));
+    atomicAdd((unsigned long long*)&t[2 * kCell + li], (unsigned long long)fx_from_f32(sp.y));
+    atomicAdd((unsigned long long*)&t[3 * kCell + li], (unsigned long long)fx_from_f32(sp.z));
+  }
+};
+
+__device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
+  // small-range wrap for tile coordinates (|v| < 2n)
+  if (v < 0) v += n;
+  if (v >= n) v -= n;
+  return v;
+}
+
+template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool RELAX_ONLY>
+__global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
+  __shared__ float s_new[kH1X * kH1Y];
+  __shared__ float s_snew[LAYERS == 2 ? kH1X * kH1Y : 1];
+  // relax-phase and march-phase working sets share one buffer
+  constexpr int kRelaxBytes = 4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y + 8 * kH1X * kH1Y;
+  constexpr int kMarchBytes = 8 * kCell * (LAYERS == 2 ? 2 : 1) + 4 * kCell + (TRACK ? 32 * kCell : 0);
+  constexpr int kWorkBytes = kRelaxBytes > kMarchBytes ? kRelaxBytes : kMarchBytes;
+  __shared__ __attribute__((aligned(16))) unsigned char s_work[kWorkBytes];
+  __shared__ uint32_t s_out;
+  __shared__ uint32_t s_slot[9];
+  __shared__ int32_t s_src[9];
+  __shared__ uint32_t s_srcmask[9];
+  __shared__ int32_t s_nsrc;
+  __shared__ int64_t s_lost[kStepThreads / 64];
+  __shared__ int64_t s_base;
+
+  const int32_t nx = a.nx, ny = a.ny;
+  const int32_t tiles_x = a.tiles_x, tiles_y = a.tiles_y;
+  // XCD-aware order: XCD k (workgroups b with b % 8 == k) takes a contiguous
+  // range of tiles, so neighbouring tiles share halo lines in one L2.
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q8 = nb / 8, r8 = nb % 8, xcd = b % 8;
+  const int32_t u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);
+  const int32_t tx = u / tiles_y, ty = u % tiles_y;
+  const int32_t x0 = tx * kSX, y0 = ty * kSY;
+
+  // this window's output segment starts after those of tiles 0 .. u-1, whose
+  // lengths are bounded by their input counts nin[t]
+  {
+    int64_t part = 0;
+    for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t b0 = 0;
+      for (int i = 0; i < kStepThreads / 64; i++) b0 += s_lost[i];
+      s_base = b0;
+      a.base_cur[u] = b0;
+    }
+    __syncthreads();
+  }
+
+  // ---------------- 1. relax of the previous window ----------------
+  {
+    float* s_old = (float*)s_work;
+    float* s_sold = s_old + kH3X * kH3Y;  // valid when LAYERS == 2
+    uint32_t* s_vold = (uint32_t*)(s_old + kH3X * kH3Y * (LAYERS == 2 ? 2 : 1));
+    int64_t* s_casc = (int64_t*)(s_vold + kH2X * kH2Y);
+    for (int e = threadIdx.x; e < kH3X * kH3Y; e += kStepThreads) {
+      int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
+      float v = 0.0f, vs = 0.0f;
+      bool ok = true;
+      if (PERIODIC) {
+        gx = wrap(gx, nx, a.rcp_nx);
+        gy = wrap(gy, ny, a.rcp_ny);
+      } else {
+        ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+      }
+      if (ok) {
+        const int64_t g = (int64_t)gx * ny + gy;
+        v = a.snap_in[g];
+        if (LAYERS == 2) vs = a.ssnap_in[g];
+      }
+      s_old[e] = v;
+      if (LAYERS == 2) s_sold[e] = vs;
+    }
+    for (int e = threadIdx.x; e < kH2X * kH2Y; e += kStepThreads) {
+      int32_t gx = x0 - 2 + e / kH2Y, gy = y0 - 2 + e % kH2Y;
+      uint32_t v = 0;
+      bool ok = true;
+      if (PERIODIC) {
+        gx = wrap(gx, nx, a.rcp_nx);
+        gy = wrap(gy, ny, a.rcp_ny);
+      } else {
+        ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+      }
+      if (ok) v = a.v_in[(int64_t)gx * ny + gy];
+      s_vold[e] = v;
+    }
+    for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
+    __syncthreads();
+
+    // cascades of every visited cell within two cells of the tile
+    for (int e = threadIdx.x; e < kH2X * kH2Y; e += kStepThreads) {
+      const uint32_t v = s_vold[e];
+      if (v == 0) continue;
+      const int32_t lx = e / kH2Y, ly = e % kH2Y;  // H2 coords; H3 = +1, H1 = -1
+      const int32_t gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+      float hn[8], sn[8];
+      uint32_t valid = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const int32_t i3 = (lx + 1 + c_ox[k]) * kH3Y + (ly + 1 + c_oy[k]);
+        hn[k] = s_old[i3];
+        sn[k] = LAYERS == 2 ? s_sold[i3] : 0.0f;
+        bool ok = true;
+        if (!PERIODIC) {
+          const int32_t qx = gx + c_ox[k], qy = gy + c_oy[k];
+          ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
+        }
+        valid |= (ok ? 1u : 0u) << k;
+      }
+      const int32_t ic = (lx + 1) * kH3Y + (ly + 1);
+      int64_t amt[8];
+      cascade_cell_dev<LAYERS>(hn, sn, s_old[ic], LAYERS == 2 ? s_sold[ic] : 0.0f, valid,
+                               settle_factor_dev(a.settling, v), a.maxdiff, amt);
+      int64_t centre = 0;
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        if (amt[k] == 0) continue;
+        centre -= amt[k];
+        const int32_t hx = lx - 1 + c_ox[k], hy = ly - 1 + c_oy[k];
+        if (hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
+          atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)amt[k]);
+      }
+      const int32_t hx = lx - 1, hy = ly - 1;
+      if (centre != 0 && hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
+        atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)centre);
+    }
+    __syncthreads();
+
+    // master += pickups + cascade; next snapshot for the tile and its halo
+    for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) {
+      const int32_t lx = e / kH1Y, ly = e % kH1Y;
+      int32_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+      bool ok = true;
+      if (PERIODIC) {
+        gx = wrap(gx, nx, a.rcp_nx);
+        gy = wrap(gy, ny, a.rcp_ny);
+      } else {
+        ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+      }
+      if (!ok) {
+        s_new[e] = 0.0f;
+        if (LAYERS == 2) s_snew[e] = 0.0f;
+        continue;
+      }
+      const int64_t g = (int64_t)gx * ny + gy;
+      const int64_t cas = s_casc[e];
+      const int64_t z = a.zm_in[g] + a.d_in[g] + cas;
+      const float zf = f32_from_fx(z);
+      s_new[e] = zf;
+      const bool mine = lx >= 1 && ly >= 1 && lx <= kSX && ly <= kSY && x0 - 1 + lx < nx && y0 - 1 + ly < ny;
+      if (mine) {
+        a.zm_out[g] = z;
+        a.snap_out[g] = zf;
+      }
+      if (LAYERS == 2) {
+        int64_t s = a.sm_in[g] + a.ds_in[g] + cas;
+        if (s < 0) s = 0;
+        const float sf = f32_from_fx(s);
+        s_snew[e] = sf;
+        if (mine) {
+          a.sm_out[g] = s;
+          a.ssnap_out[g] = sf;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------- 2. particles of the tile ----------------
+  int64_t* acc_d = (int64_t*)s_work;
+  int64_t* acc_ds = acc_d + kCell;  // LAYERS == 2
+  uint32_t* acc_v = (uint32_t*)(acc_d + kCell * (LAYERS == 2 ? 2 : 1));
+  int64_t* acc_t = (int64_t*)(acc_v + kCell);  // TRACK (kCell*4), 8-byte aligned: kCell*4 bytes of acc_v
+  for (int e = threadIdx.x; e < kCell; e += kStepThreads) {
+    acc_d[e] = 0;
+    if (LAYERS == 2) acc_ds[e] = 0;
+    acc_v[e] = 0;
+    if (TRACK) {
+      acc_t[e] = 0;
+      acc_t[kCell + e] = 0;
+      acc_t[2 * kCell + e] = 0;
+      acc_t[3 * kCell + e] = 0;
+    }
+  }
+  if (threadIdx.x < 9) s_slot[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    s_out = 0;
+    // distinct source tiles and, for each, the direction codes that lead here
+    int32_t n = 0;
+    for (int d = 0; d < 9; d++) {
+      const int32_t dx = d / 3 - 1, dy = d % 3 - 1;
+      int32_t sx = tx - dx, sy = ty - dy;
+      if (PERIODIC) {
+        sx = wrapt(sx, tiles_x);
+        sy = wrapt(sy, tiles_y);
+      } else if (sx < 0 || sy < 0 || sx >= tiles_x || sy >= tiles_y) {
+        continue;
+      }
+      const int32_t s = sx * tiles_y + sy;
+      int j = 0;
+      while (j < n && s_src[j] != s) j++;
+      if (j == n) {
+        s_src[n] = s;
+        s_srcmask[n] = 0;
+        n++;
+      }
+      s_srcmask[j] |= 1u << d;
+    }
+    s_nsrc = n;
+  }
+  int64_t lost_here = 0;
+  __syncthreads();
+
+  if (!RELAX_ONLY) {
+    FlyConsts k;
+    k.nx = nx;
+    k.ny = ny;
+    k.rcp_nx = a.rcp_nx;
+    k.rcp_ny = a.rcp_ny;
+    k.mapscale = a.mapscale;
+    k.max_age = a.max_age;
+    k.boundarylayer = a.boundarylayer;
+    k.suspension = a.suspension;
+    k.gravity = a.gravity;
+    k.abrasion = a.abrasion;
+    k.ps = mk(a.ps[0], a.ps[1], a.ps[2]);
+    k.nps = mk(a.nps[0], a.nps[1], a.nps[2]);
+    const uint64_t ck = cycle_key(a.seed, *a.cycle);
+    const int64_t out_base = s_base;
+    const int32_t nsrc = s_nsrc;
+    const uint32_t far_n = *a.far_cnt_in;
+
+    // process one input record; returns nothing, writes outputs
+    auto process = [&](const ParticleBuf& in, int64_t j) {
+      Particle w;
+      const float4 q0 = in.P0[j];
+      const float4 q1 = in.P1[j];
+      w.px = q0.x; w.py = q0.y; w.pz = q0.z; w.sed = q0.w;
+      w.sx = q1.x; w.sy = q1.y; w.sz = q1.z;
+      w.age = (int32_t)f2u(q1.w);
+      w.ledger = in.L[j];
+      w.lastc = in.LC[j];
+      const uint32_t pid = in.PID[j];
+      const uint64_t pk = particle_key(ck, pid);
+      int32_t cx, cy;
+      const bool ok = cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);
+      int64_t dead_cell = -1;
+      bool dead = !ok;
+      if (ok && (cx < x0 || cy < y0 || cx >= x0 + kSX || cy >= y0 + kSY)) {
+        // routing always places a particle in the tile of its cell; keep
+        // LDS indices in range whatever happens by forwarding a stray one
+        const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
+        a.far_out.P0[fslot] = q0;
+        a.far_out.P1[fslot] = q1;
+        a.far_out.L[fslot] = w.ledger;
+        a.far_out.LC[fslot] = w.lastc;
+        a.far_out.PID[fslot] = pid;
+        a.far_out.CODE[fslot] = 255;
+        return;
+      }
+      if (ok) {
+        const int32_t lx = cx - x0, ly = cy - y0;
+        TileMap m{s_new, s_snew, (lx + 1) * kH1Y + (ly + 1)};
+        TileAcc acc{acc_d, acc_ds, acc_v, acc_t, lx * kSY + ly};
+        if (fly<PERIODIC, TRACK, LAYERS>(w, pk, cx, cy, k, m, acc) == kDied) {
+          dead = true;
+          dead_cell = (int64_t)cx * ny + cy;
+        }
+      }
+      if (dead) {
+        const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)w.lastc;
+        if (DEPOSIT) {
+          const int32_t lx = (int32_t)(c / ny) - x0, ly = (int32_t)(c % ny) - y0;
+          if (w.ledger != 0 && lx >= 0 && ly >= 0 && lx < kSX && ly < kSY) {
+            const int32_t li = lx * kSY + ly;
+            atomicAdd((unsigned long long*)&acc_d[li], (unsigned long long)w.ledger);
+            if (LAYERS == 2) atomicAdd((unsigned long long*)&acc_ds[li], (unsigned long long)w.ledger);
+          }
+        } else {
+          lost_here += w.ledger;
+        }
+        return;
+      }
+      // route by the cell the particle now occupies; one that leaves a
+      // bounded map (or whose position stops being finite) stays with the
+      // tile of its last cell and dies there at its next call
+      int32_t ncx, ncy;
+      int32_t code = 4;
+      if (cell_of<PERIODIC>(w.px, w.pz, k, ncx, ncy)) {
+        int32_t dtx = ncx / kSX - tx, dty = ncy / kSY - ty;
+        if (PERIODIC) {
+          if (tiles_x > 2) {
+            if (dtx == tiles_x - 1) dtx = -1;
+            if (dtx == 1 - tiles_x) dtx = 1;
+          }
+          if (tiles_y > 2) {
+            if (dty == tiles_y - 1) dty = -1;
+            if (dty == 1 - tiles_y) dty = 1;
+          }
+        }
+        code = (dtx < -1 || dtx > 1 || dty < -1 || dty > 1) ? 255 : (dtx + 1) * 3 + (dty + 1);
+      }
+      const float4 o0 = make_float4(w.px, w.py, w.pz, w.sed);
+      const float4 o1 = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
+      if (code == 255) {
+        const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
+        a.far_out.P0[fslot] = o0;
+        a.far_out.P1[fslot] = o1;
+        a.far_out.L[fslot] = w.ledger;
+        a.far_out.LC[fslot] = w.lastc;
+        a.far_out.PID[fslot] = pid;
+        a.far_out.CODE[fslot] = 255;
+        return;
+      }
+      const int64_t o = out_base + atomicAdd(&s_out, 1u);
+      atomicAdd(&s_slot[code], 1u);
+      a.out.P0[o] = o0;
+      a.out.P1[o] = o1;
+      a.out.L[o] = w.ledger;
+      a.out.LC[o] = w.lastc;
+      a.out.PID[o] = pid;
+      a.out.CODE[o] = (uint8_t)code;
+    };
+
+    for (int si = 0; si < nsrc; si++) {
+      const int32_t s = s_src[si];
+      const uint32_t mask = s_srcmask[si];
+      const int64_t b0 = a.base_prev[s];
+      const int64_t n = a.cnt_prev[s];
+      for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
+        const uint32_t code = a.in.CODE[j];
+        if (code < 9 && ((mask >> code) & 1u)) process(a.in, j);
+      }
+    }
+    for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
+      int32_t cx, cy;
+      const float4 q0 = a.far_in.P0[j];
+      int32_t ctx, cty;
+      if (cell_of<PERIODIC>(q0.x, q0.z, k, cx, cy)) {
+        ctx = cx / kSX;
+        cty = cy / kSY;
+      } else {
+        const int32_t lc = a.far_in.LC[j];
+        ctx = (lc / ny) / kSX;
+        cty = (lc % ny) / kSY;
+      }
+      if (ctx == tx && cty == ty) process(a.far_in, j);
+    }
+  }
+  __syncthreads();
+
+  // ---------------- 3. write the tile's window results ----------------
+  for (int e = threadIdx.x; e < kCell; e += kStepThreads) {
+    const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
+    if (gx >= nx || gy >= ny) continue;
+    const int64_t g = (int64_t)gx * ny + gy;
+    a.d_out[g] = acc_d[e];
+    a.v_out[g] = acc_v[e];
+    if (LAYERS == 2) a.ds_out[g] = acc_ds[e];
+    if (TRACK) {
+#pragma unroll
+      for (int f = 0; f < 4; f++) {
+        const int64_t t = acc_t[f * kCell + e];
+        if (t != 0) a.trk[f * a.ncell + g] += t;
+      }
+    }
+  }
+  if (threadIdx.x == 0) a.cnt_cur[u] = (int64_t)s_out;
+  if (threadIdx.x < 9) {
+    // particles sent towards direction d: the next window's input of that tile
+    const uint32_t n = s_slot[threadIdx.x];
+    if (n) {
+      const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
+      int32_t dtx = tx + dx, dty = ty + dy;
+      if (PERIODIC) {
+        dtx = wrapt(dtx, tiles_x);
+        dty = wrapt(dty, tiles_y);
+      }
+      atomicAdd(&a.nin_next[dtx * tiles_y + dty], n);
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.nin_clear[u] = 0;  // the window after next accumulates here
+    if (u == 0) *a.far_cnt_clear = 0;
+  }
+  if (!DEPOSIT) {
+    for (int off = 32; off > 0; off >>= 1) lost_here += __shfl_xor(lost_here, off, 64);
+    if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = lost_here;
+  }
+  __syncthreads();
+  if (!DEPOSIT && threadIdx.x == 0) {
+    int64_t l = 0;
+    for (int i = 0; i < kStepThreads / 64; i++) l += s_lost[i];
+    if (l != 0) atomicAdd((unsigned long long*)&a.lost[u & (kLostSlots - 1)], (unsigned long long)l);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spawn and bin the particles of a cycle by tile (world.hpp:52-62)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_spawn_count(SpawnArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  Particle w;
+  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, w);
+  // the spawn cell is always on the map: round() of a position in
+  // [ix, ix + 0.999] is ix or ix + 1, wrapped (periodic) or, on a bounded map,
+  // possibly nx / ny, in which case the particle dies at its first call;
+  // bin it by the spawn cell clamped to the map
+  int32_t ix = (int32_t)__builtin_roundf(w.px), iy = (int32_t)__builtin_roundf(w.pz);
+  if (a.periodic) {
+    if (ix >= a.nx) ix -= a.nx;
+    if (iy >= a.ny) iy -= a.ny;
+  } else {
+    if (ix >= a.nx) ix = a.nx - 1;
+    if (iy >= a.ny) iy = a.ny - 1;
+  }
+  const int32_t t = (ix / kSX) * a.tiles_y + (iy / kSY);
+  a.tile[i] = t;
+  a.rank[i] = atomicAdd(&a.count[t], 1u);  // count is window 0's nin
+}
+
+__global__ __launch_bounds__(256) void k_spawn_scan(SpawnArgs a) {
+  // one workgroup: exclusive prefix of the per-tile counts
+  __shared__ int64_t s_part[256];
+  const int32_t ntiles = a.tiles_x * a.tiles_y;
+  const int32_t per = (ntiles + 255) / 256;
+  const int32_t t0 = threadIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  int64_t run = 0;
+  for (int32_t t = t0; t < t1; t++) {
+    a.base1[t] = run;
+    run += a.count[t];
+  }
+  s_part[threadIdx.x] = run;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t acc = 0;
+    for (int i = 0; i < 256; i++) {
+      const int64_t v = s_part[i];
+      s_part[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  const int64_t off = s_part[threadIdx.x];
+  for (int32_t t = t0; t < t1; t++) {
+    a.base1[t] += off;
+    a.cnt[t] = a.count[t];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= a.n) return;
+  Particle w;
+  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, w);
+  const int64_t o = a.base1[a.tile[i]] + a.rank[i];
+  a.out.P0[o] = make_float4(w.px, w.py, w.pz, w.sed);
+  a.out.P1[o] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
+  a.out.L[o] = 0;
+  a.out.LC[o] = 0;
+  a.out.PID[o] = (uint32_t)i;
+  a.out.CODE[o] = 4;  // in its own tile
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+template <bool P, bool T, int L, bool D>
+static void step4(const StepArgs& a, bool relax_only, hipStream_t s) {
+  const unsigned g = (unsigned)(a.tiles_x * a.tiles_y);
+  if (relax_only)
+    hipLaunchKernelGGL((k_step<P, T, L, D, true>), dim3(g), dim3(kStepThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_step<P, T, L, D, false>), dim3(g), dim3(kStepThreads), 0, s, a);
+}
+template <bool P, bool T, int L>
+static void step3(const StepArgs& a, bool ro, bool dep, hipStream_t s) {
+  if (dep) step4<P, T, L, true>(a, ro, s);
+  else step4<P, T, L, false>(a, ro, s);
+}
+template <bool P, bool T>
+static void step2(const StepArgs& a, bool ro, bool dep, int layers, hipStream_t s) {
+  if (layers == 2) step3<P, T, 2>(a, ro, dep, s);
+  else step3<P, T, 1>(a, ro, dep, s);
+}
+template <bool P>
+static void step1(const StepArgs& a, bool ro, bool dep, bool trk, int layers, hipStream_t s) {
+  if (trk) step2<P, true>(a, ro, dep, layers, s);
+  else step2<P, false>(a, ro, dep, layers, s);
+}
+
+void launch_step(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,
+                 hipStream_t s) {
+  if (periodic) step1<true>(a, relax_only, deposit, track, layers, s);
+  else step1<false>(a, relax_only, deposit, track, layers, s);
+}
+
+void launch_spawn(const SpawnArgs& a, hipStream_t s) {
+  if (a.n > 0) {
+    const unsigned g = (unsigned)((a.n + 255) / 256);
+    hipLaunchKernelGGL(k_spawn_count, dim3(g), dim3(256), 0, s, a);
+  }
+  hipLaunchKernelGGL(k_spawn_scan, dim3(1), dim3(256), 0, s, a);
+  if (a.n > 0) {
+    const unsigned g = (unsigned)((a.n + 255) / 256);
+    hipLaunchKernelGGL(k_spawn_place, dim3(g), dim3(256), 0, s, a);
+  }
+}
+
+}  // namespace swe

@@ -109,6 +109,10 @@ class World:
         self._push()
         self._check(self.lib.swe_erode(self.ctx, int(cycles), int(n_cycles)), "swe_erode")
 
+    def set_path(self, path):
+        """0 automatic, 1 global-memory march, 2 tile-owned step kernel (window 1)."""
+        self._check(self.lib.swe_set_path(self.ctx, int(path)), "swe_set_path")
+
     def synchronize(self):
         self._check(self.lib.swe_synchronize(self.ctx), "swe_synchronize")
 

@@ -21,7 +21,8 @@ def exact_sum(a):
     return int((a >> 32).sum()) * 2 ** 32 + int((a & 0xFFFFFFFF).sum())
 
 
-def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=None, settling=None, **opts):
+def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=None, settling=None, path=0,
+             **opts):
     """Run the same workload on the GPU and in the oracle; return both states."""
     if settling is not None:
         World.settling = settling
@@ -30,6 +31,7 @@ def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=Non
     w = World(nx, ny, **{k: v for k, v in opts.items() if k in
                          ("periodic", "track", "layers", "window", "abrasion", "deposit_on_death")})
     w.upload(h, sed)
+    w.set_path(path)
     World.SEED = seed
     op = O.params_from(w.params())
     st = O.OracleState(nx, ny, h, sediment=sed, params=op)
@@ -93,12 +95,34 @@ def test_parity_two_layer_abrasion(statics):
     w.close()
 
 
-@pytest.mark.parametrize("nx,ny,periodic", [(1, 1, True), (1, 1, False), (2, 3, True), (3, 2, False), (1, 70, True)])
-def test_parity_tiny_maps(statics, nx, ny, periodic):
-    say("case tiny", nx, ny, periodic)
-    w, st = run_pair(nx, ny, 5, 2, seed=3, periodic=periodic, window=4, settling=0.01)
+@pytest.mark.parametrize("window", [1, 4])
+@pytest.mark.parametrize("nx,ny,periodic", [(1, 1, True), (1, 1, False), (2, 3, True), (3, 2, False), (1, 70, True),
+                                            (20, 40, True), (33, 65, True), (17, 33, False)])
+def test_parity_small_maps(statics, nx, ny, periodic, window):
+    """Tiny maps and maps of one to three 16x32 tiles per axis (tile-direction
+    wrap-around on the tile-owned path)."""
+    say("case small", nx, ny, periodic, window)
+    w, st = run_pair(nx, ny, 3 * nx * ny // 4 + 5, 2, seed=3, periodic=periodic, window=window, settling=0.01)
     assert_same(w, st)
     w.close()
+
+
+@pytest.mark.parametrize("periodic,layers,track,deposit", [(True, 1, True, False), (False, 1, True, True),
+                                                           (True, 2, False, False), (False, 2, True, True)])
+def test_both_paths_agree_at_window_one(statics, periodic, layers, track, deposit):
+    """The tile-owned step kernel and the global-memory march give the same
+    bits, and both equal the oracle."""
+    say("case paths", periodic, layers, track, deposit)
+    Wind.suspension = 1e-2
+    sed = np.full((70, 100), 0.002, np.float32) if layers == 2 else None
+    outs = []
+    for path in (1, 2):
+        w, st = run_pair(70, 100, 4000, 2, seed=9, sed=sed, periodic=periodic, layers=layers, track=track,
+                         deposit_on_death=deposit, abrasion=1e-3, window=1, settling=0.05, path=path)
+        assert_same(w, st)
+        outs.append(w.master())
+        w.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
 def test_parity_cpu256_workload(statics):
