@@ -67,6 +67,27 @@ def cpu_baseline(workload, seed, target_s):
                       % (n, steps, w["nx"], w["ny"], dt)}
 
 
+def calibrate_copy(mib):
+    """A plain device-to-device copy of known size (hipMemcpy, one copy kernel
+    per call): the reference point that scales rocprof's FETCH_SIZE /
+    WRITE_SIZE counters to bytes."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+        raise SystemExit("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3):  # hipMemcpyDeviceToDevice
+            raise SystemExit("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +99,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--window", type=int, default=None)
     ap.add_argument("--track", type=int, default=None)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the map would shard by "

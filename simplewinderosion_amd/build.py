@@ -49,7 +49,25 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libswe.so")
     os.replace(tmp, LIB)
+    build_demo(verbose)
     return LIB
+
+
+DEMO_SRC = os.path.join(HERE, "examples", "erode_demo.cpp")
+DEMO = os.path.join(HERE, "examples", "erode_demo")
+
+
+def build_demo(verbose=False):
+    """The C++ example caller of include/swe_world.hpp (host code only)."""
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), DEMO_SRC, "-o", DEMO,
+           "-L", HERE, "-lswe", "-Wl,-rpath," + HERE]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the C++ example")
+    return DEMO
 
 
 if __name__ == "__main__":

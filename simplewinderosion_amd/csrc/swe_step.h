@@ -6,8 +6,23 @@
 
 namespace swe {
 
-constexpr int kStepTileX = 16;  // tile extent along x (slow axis)
-constexpr int kStepTileY = 32;  // tile extent along y (fast axis)
+// Tile shape and workgroup size of k_step (compile-time; overridable for tuning)
+#ifndef SWE_STEP_TILE_X
+#define SWE_STEP_TILE_X 16
+#endif
+#ifndef SWE_STEP_TILE_Y
+#define SWE_STEP_TILE_Y 32
+#endif
+#ifndef SWE_STEP_THREADS
+#define SWE_STEP_THREADS 256
+#endif
+#ifndef SWE_STEP_LIST
+#define SWE_STEP_LIST 2048
+#endif
+constexpr int kStepTileX = SWE_STEP_TILE_X;    // tile extent along x (slow axis)
+constexpr int kStepTileY = SWE_STEP_TILE_Y;    // tile extent along y (fast axis)
+constexpr int kStepTileThreads = SWE_STEP_THREADS;
+constexpr int kStepListCap = SWE_STEP_LIST;    // particles gathered in LDS per tile
 
 // One particle buffer (structure of arrays)
 struct ParticleBuf {

@@ -32,8 +32,9 @@ constexpr int kSX = kStepTileX, kSY = kStepTileY;
 constexpr int kH3X = kSX + 6, kH3Y = kSY + 6;  // snapshot halo 3
 constexpr int kH2X = kSX + 4, kH2Y = kSY + 4;  // events halo 2
 constexpr int kH1X = kSX + 2, kH1Y = kSY + 2;  // new values halo 1
-constexpr int kStepThreads = 256;
+constexpr int kStepThreads = kStepTileThreads;
 constexpr int kCell = kSX * kSY;
+constexpr int kList = kStepListCap;  // particles of a tile gathered into LDS before stepping
 
 constexpr int32_t c_ox[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int32_t c_oy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
@@ -89,6 +90,8 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   __shared__ int32_t s_nsrc;
   __shared__ int64_t s_lost[kStepThreads / 64];
   __shared__ int64_t s_base;
+  __shared__ uint32_t s_list[kList];
+  __shared__ uint32_t s_nlist;
 
   const int32_t nx = a.nx, ny = a.ny;
   const int32_t tiles_x = a.tiles_x, tiles_y = a.tiles_y;
@@ -100,21 +103,55 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   const int32_t tx = u / tiles_y, ty = u % tiles_y;
   const int32_t x0 = tx * kSX, y0 = ty * kSY;
 
-  // this window's output segment starts after those of tiles 0 .. u-1, whose
-  // lengths are bounded by their input counts nin[t]
-  {
-    int64_t part = 0;
-    for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
-    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-    if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = part;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t b0 = 0;
-      for (int i = 0; i < kStepThreads / 64; i++) b0 += s_lost[i];
-      s_base = b0;
-      a.base_cur[u] = b0;
+  // distinct source tiles and, for each, the direction codes that lead here
+  if (threadIdx.x == 0) {
+    int32_t n = 0;
+    for (int d = 0; d < 9; d++) {
+      const int32_t dx = d / 3 - 1, dy = d % 3 - 1;
+      int32_t sx = tx - dx, sy = ty - dy;
+      if (PERIODIC) {
+        sx = wrapt(sx, tiles_x);
+        sy = wrapt(sy, tiles_y);
+      } else if (sx < 0 || sy < 0 || sx >= tiles_x || sy >= tiles_y) {
+        continue;
+      }
+      const int32_t src = sx * tiles_y + sy;
+      int j = 0;
+      while (j < n && s_src[j] != src) j++;
+      if (j == n) {
+        s_src[n] = src;
+        s_srcmask[n] = 0;
+        n++;
+      }
+      s_srcmask[j] |= 1u << d;
     }
-    __syncthreads();
+    s_nsrc = n;
+    s_nlist = 0;
+  }
+  __syncthreads();
+  // this window's output segment starts after those of tiles 0 .. u-1, whose
+  // lengths are bounded by their input counts nin[t]; the loads are issued
+  // here and summed after the relax phase
+  int64_t part = 0;
+  for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
+  // gather the indices of this tile's particles into LDS (when they fit)
+  const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
+  const bool listed = nin_u <= (uint32_t)kList;
+  if (!RELAX_ONLY && listed) {
+    const int32_t nsrc = s_nsrc;
+    for (int si = 0; si < nsrc; si++) {
+      const int32_t src = s_src[si];
+      const uint32_t mask = s_srcmask[si];
+      const int64_t b0 = a.base_prev[src];
+      const int64_t n = a.cnt_prev[src];
+      for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
+        const uint32_t code = a.in.CODE[j];
+        if (code < 9 && ((mask >> code) & 1u)) {
+          const uint32_t pos = atomicAdd(&s_nlist, 1u);
+          if (pos < (uint32_t)kList) s_list[pos] = (uint32_t)j;  // pos < nin_u <= kList
+        }
+      }
+    }
   }
 
   // ---------------- 1. relax of the previous window ----------------
@@ -235,6 +272,17 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     }
     __syncthreads();
   }
+  {
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t b0 = 0;
+      for (int i = 0; i < kStepThreads / 64; i++) b0 += s_lost[i];
+      s_base = b0;
+      a.base_cur[u] = b0;
+    }
+  }
 
   // ---------------- 2. particles of the tile ----------------
   int64_t* acc_d = (int64_t*)s_work;
@@ -253,31 +301,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     }
   }
   if (threadIdx.x < 9) s_slot[threadIdx.x] = 0;
-  if (threadIdx.x == 0) {
-    s_out = 0;
-    // distinct source tiles and, for each, the direction codes that lead here
-    int32_t n = 0;
-    for (int d = 0; d < 9; d++) {
-      const int32_t dx = d / 3 - 1, dy = d % 3 - 1;
-      int32_t sx = tx - dx, sy = ty - dy;
-      if (PERIODIC) {
-        sx = wrapt(sx, tiles_x);
-        sy = wrapt(sy, tiles_y);
-      } else if (sx < 0 || sy < 0 || sx >= tiles_x || sy >= tiles_y) {
-        continue;
-      }
-      const int32_t s = sx * tiles_y + sy;
-      int j = 0;
-      while (j < n && s_src[j] != s) j++;
-      if (j == n) {
-        s_src[n] = s;
-        s_srcmask[n] = 0;
-        n++;
-      }
-      s_srcmask[j] |= 1u << d;
-    }
-    s_nsrc = n;
-  }
+  if (threadIdx.x == 0) s_out = 0;
   int64_t lost_here = 0;
   __syncthreads();
 
@@ -392,14 +416,20 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       a.out.CODE[o] = (uint8_t)code;
     };
 
-    for (int si = 0; si < nsrc; si++) {
-      const int32_t s = s_src[si];
-      const uint32_t mask = s_srcmask[si];
-      const int64_t b0 = a.base_prev[s];
-      const int64_t n = a.cnt_prev[s];
-      for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
-        const uint32_t code = a.in.CODE[j];
-        if (code < 9 && ((mask >> code) & 1u)) process(a.in, j);
+    if (listed) {
+      const uint32_t nl = s_nlist < (uint32_t)kList ? s_nlist : (uint32_t)kList;
+      for (uint32_t e = threadIdx.x; e < nl; e += kStepThreads) process(a.in, (int64_t)s_list[e]);
+    } else {
+      // a crowded tile: scan the sources again and step in place
+      for (int si = 0; si < nsrc; si++) {
+        const int32_t src = s_src[si];
+        const uint32_t mask = s_srcmask[si];
+        const int64_t b0 = a.base_prev[src];
+        const int64_t n = a.cnt_prev[src];
+        for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
+          const uint32_t code = a.in.CODE[j];
+          if (code < 9 && ((mask >> code) & 1u)) process(a.in, j);
+        }
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {

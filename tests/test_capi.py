@@ -88,3 +88,15 @@ def test_gpu_entry_points_fail_cleanly_without_device():
     rc = lib.swe_create(ctypes.byref(ctx), 0, 4, 0)
     assert rc == capi.SWE_EINVAL
     assert lib.swe_erode(None, 1, 1) == capi.SWE_EINVAL
+
+
+def test_cpp_header_mirror_compiles_and_fails_cleanly_without_gpu(tmp_path):
+    """include/swe_world.hpp builds with g++ -std=c++17 and reports errors as exceptions."""
+    from simplewinderosion_amd.build import DEMO, build_demo
+
+    if not os.path.exists(DEMO):
+        build_demo()
+    out = subprocess.run([DEMO, "8", "8", "1", "4", "1"], capture_output=True, text=True, timeout=60)
+    if out.returncode == 0:
+        pytest.skip("a GPU is present: the GPU test covers the run")
+    assert "swe_create failed" in out.stderr
