@@ -153,6 +153,18 @@ int swe_set_cycle(swe_ctx* ctx, uint64_t cycle);
 /* Carried-off mass counter (fixed point) for resume. */
 int swe_set_lost(swe_ctx* ctx, int64_t lost);
 
+/* Map size of a context. */
+int swe_get_size(const swe_ctx* ctx, int32_t* nx, int32_t* ny);
+
+/*
+ * Checkpoint / resume: the exact int64 master layers, EMA fields, params,
+ * cycle counter and lost-mass counter in one little-endian file, so that
+ * load + erode continues bit for bit.  No counterpart in the reference,
+ * whose only state is its seed (SimpleWindErosion.cpp:27-34).
+ */
+int swe_save_state(swe_ctx* ctx, const char* path);
+int swe_load_state(swe_ctx* ctx, const char* path);
+
 /* Kernel timing with HIP events on the context stream (off by default). */
 int swe_set_timing(swe_ctx* ctx, int32_t on);
 int swe_stats(swe_ctx* ctx, swe_stats_t* out);

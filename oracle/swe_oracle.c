@@ -814,7 +814,7 @@ static void seq_cascade(or_seqmap* m, int32_t x, int32_t y)
   }
 }
 
-typedef struct { v3 pos, speed; float sediment; int32_t age; } or_seqwind;
+typedef struct { v3 pos, speed; float sediment; int32_t age; int64_t cell; } or_seqwind;
 
 /* Wind::fly (wind.hpp:28-117), sequential: glibc rand(), libm expf */
 static int seq_fly(or_seqmap* m, or_seqwind* w)
@@ -825,6 +825,7 @@ static int seq_fly(or_seqmap* m, or_seqwind* w)
   int32_t ix = (int32_t)rx, iy = (int32_t)rz;
   or_cell* cell = seq_get(m, ix, iy);
   if (cell == NULL) return 0;
+  w->cell = cell - m->cells;
   v3 n = seq_normal(m, ix, iy);
   if (w->age == 0 || w->pos.y < cell->height) w->pos.y = cell->height;
   if (w->age++ > p->max_age) return 0;
@@ -922,7 +923,13 @@ int64_t or_erode_seq(const or_params* p, int32_t nx, int32_t ny, float* height, 
       w.speed = V3(0.0f, 0.0f, 0.0f);
       w.sediment = 0.0f;
       w.age = 0;
+      w.cell = -1;
       while (seq_fly(&m, &w)) steps++;
+      if (p->deposit_on_death && w.cell >= 0) {
+        /* the option the batched method offers: the load stays on the map */
+        cells[w.cell].height += w.sediment;
+        if (p->layers == 2 && sed) sed[w.cell] += w.sediment;
+      }
     }
     for (int64_t c = 0; c < ncell; c++) {
       or_cell* e = &cells[c];

@@ -113,6 +113,9 @@ SIGNATURES = {
     "swe_get_cycle": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "swe_set_cycle": (ctypes.c_int, [_P, ctypes.c_uint64]),
     "swe_set_lost": (ctypes.c_int, [_P, ctypes.c_int64]),
+    "swe_get_size": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "swe_save_state": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "swe_load_state": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "swe_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "swe_stats": (ctypes.c_int, [_P, ctypes.POINTER(SweStats)]),
     "swe_reset_stats": (ctypes.c_int, [_P]),

@@ -734,6 +734,13 @@ int swe_audit(swe_ctx* c, swe_audit_t* out) {
   return 0;
 }
 
+int swe_get_size(const swe_ctx* c, int32_t* nx, int32_t* ny) {
+  if (!c || !nx || !ny) return SWE_EINVAL;
+  *nx = c->nx;
+  *ny = c->ny;
+  return 0;
+}
+
 int swe_get_cycle(const swe_ctx* c, uint64_t* cycle) {
   if (!c || !cycle) return SWE_EINVAL;
   *cycle = c->cycle;

@@ -174,6 +174,24 @@ class World:
     def set_lost(self, v):
         self._check(self.lib.swe_set_lost(self.ctx, int(v)), "swe_set_lost")
 
+    def save(self, path):
+        """Checkpoint the exact state (masters, EMA fields, params, counters) to `path`."""
+        self._check(self.lib.swe_save_state(self.ctx, str(path).encode()), "swe_save_state")
+
+    def load(self, path):
+        """Restore a checkpoint written by save(); the map size must match.  The
+        statics are set from the file so later erode() calls keep its params."""
+        self._check(self.lib.swe_load_state(self.ctx, str(path).encode()), "swe_load_state")
+        p = capi.SweParams()
+        self._check(self.lib.swe_get_params(self.ctx, ctypes.byref(p)), "swe_get_params")
+        Wind.maxAge, Wind.boundarylayer, Wind.suspension, Wind.gravity = (
+            p.max_age, p.boundarylayer, p.suspension, p.gravity)
+        Wind.pspeed = tuple(p.pspeed)
+        World.lrate, World.maxdiff, World.settling, World.SEED = p.lrate, p.maxdiff, p.settling, p.seed
+        self.abrasion, self.mapscale, self.window = p.abrasion, p.mapscale, p.window
+        self.periodic, self.track, self.layers = bool(p.periodic), bool(p.track), p.layers
+        self.deposit_on_death = bool(p.deposit_on_death)
+
     def set_timing(self, on):
         self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")
 

@@ -192,3 +192,33 @@ def test_full_size_one_cycle(statics):
     st.erode(1_000_000, 1)
     assert_same(w, st)
     w.close()
+
+
+def test_checkpoint_resume_is_bit_exact(statics, tmp_path):
+    """save after 2 cycles, continue 2 more; load the checkpoint into a fresh
+    context and run the same 2: identical masters, fields and counters."""
+    say("case checkpoint")
+    Wind.suspension = 1e-2
+    World.settling = 0.05
+    World.SEED = 21
+    rng = np.random.default_rng(4)
+    h = (rng.random((40, 72)) * 0.5).astype(np.float32)
+    sed = np.full((40, 72), 0.003, np.float32)
+    w = World(40, 72, periodic=True, layers=2, abrasion=1e-3)
+    w.upload(h, sed)
+    w.erode(900, n_cycles=2)
+    path = tmp_path / "state.swe"
+    w.save(path)
+    w.erode(900, n_cycles=2)
+    ref = (w.master(), w.download(), w.audit())
+    w.close()
+    World.settling = 0.7  # the checkpoint's params must win
+    w2 = World(40, 72)
+    w2.load(path)
+    w2.erode(900, n_cycles=2)
+    got = (w2.master(), w2.download(), w2.audit())
+    assert np.array_equal(ref[0][0], got[0][0]) and np.array_equal(ref[0][1], got[0][1])
+    for k in ("massflow", "momentum", "height", "sediment"):
+        assert np.array_equal(ref[1][k], got[1][k]), k
+    assert ref[2] == got[2]
+    w2.close()
