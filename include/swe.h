@@ -72,7 +72,8 @@ typedef struct swe_params {
   int32_t track;          /* 1 = massflow/momentum tracking and EMA fields */
   int32_t layers;         /* 1 = height only (reference); 2 = height + loose sediment */
   int32_t deposit_on_death; /* 1 = a dying particle drops its load (closes map mass) */
-  int32_t reserved;       /* must be 0 */
+  int32_t spawn;          /* 0 = uniform over the map (world.hpp:52-54), 1 = on the x = 0 / y = 0
+                             edges (the commented variant, world.hpp:56-61) */
   uint64_t seed;          /* World::SEED: feeds the counter RNG */
 } swe_params;
 
@@ -169,6 +170,13 @@ int swe_load_state(swe_ctx* ctx, const char* path);
 int swe_set_timing(swe_ctx* ctx, int32_t on);
 int swe_stats(swe_ctx* ctx, swe_stats_t* out);
 int swe_reset_stats(swe_ctx* ctx);
+
+/*
+ * The wind map of the reference's map view (SimpleWindErosion.cpp:224-230):
+ * per cell (0.5(1+erf(2 mx)), 0.5(1+erf(2 my)), 0.5(1+erf(2 mz)), 1) from the
+ * EMA momentum field.  out: 4*nx*ny floats, RGBA interleaved per cell.
+ */
+int swe_windmap(swe_ctx* ctx, float* out);
 
 /*
  * Surface normals of the current snapshot, quad::_normal (cellpool.h:200-223)

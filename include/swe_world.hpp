@@ -38,7 +38,7 @@ class World {
   // extensions of the reference's compile-time map (cellpool.h:184-198)
   inline static int nx = 512, ny = 512;
   inline static float mapscale = 80.0f;
-  inline static int periodic = 0, track = 1, layers = 1, window = 1, deposit_on_death = 0;
+  inline static int periodic = 0, track = 1, layers = 1, window = 1, deposit_on_death = 0, spawn = 0;
   inline static float abrasion = 0.0f;
   inline static int device = 0;
 
@@ -111,6 +111,7 @@ class World {
     p.track = track;
     p.layers = layers;
     p.deposit_on_death = deposit_on_death;
+    p.spawn = spawn;
     p.seed = SEED;
     check(swe_set_params(ctx_, &p), "swe_set_params");
   }

@@ -33,7 +33,7 @@ class OrParams(ctypes.Structure):
         ("track", ctypes.c_int32),
         ("layers", ctypes.c_int32),
         ("deposit_on_death", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("spawn", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
     ]
 
@@ -56,7 +56,7 @@ def reference_params(**kw):
     p.track = 1
     p.layers = 1
     p.deposit_on_death = 0
-    p.reserved = 0
+    p.spawn = 0
     p.seed = 0
     for k, v in kw.items():
         if k == "pspeed":
@@ -106,6 +106,7 @@ def load():
         "or_mix64": (ctypes.c_uint64, [ctypes.c_uint64]),
         "or_particle_key": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
         "or_spawn": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _F, _F]),
+        "or_spawn_edge": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, _F, _F]),
         "or_turb": (None, [ctypes.c_uint64, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
         "or_normal": (None, [_F, ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_float, ctypes.c_int32,
                              ctypes.c_int32, _F]),

@@ -58,7 +58,7 @@ typedef struct {
   int32_t track;          /* flow tracking + EMA fields (world.hpp:38-44,69-76)*/
   int32_t layers;         /* 1: height only (reference), 2: + sediment layer */
   int32_t deposit_on_death;
-  int32_t reserved;
+  int32_t spawn;          /* 0: uniform (world.hpp:52-54), 1: edges (world.hpp:56-61) */
   uint64_t seed;
 } or_params;
 
@@ -194,6 +194,16 @@ uint64_t or_particle_key(uint64_t seed, uint64_t cycle, uint64_t particle)
   uint64_t ck = or_mix64(seed + OR_G * (cycle + 1));
   return or_mix64(ck ^ (OR_P * (particle + 1)));
 }
+/* the commented boundary spawn (world.hpp:56-61): a position on the x = 0
+   or y = 0 edge, shift = r % (W + H), no fractional offset */
+void or_spawn_edge(uint64_t pk, int32_t nx, int32_t ny, float* px, float* pz)
+{
+  uint64_t a = or_mix64(pk + 1 * OR_G);
+  uint32_t shift = (uint32_t)a % (uint32_t)(nx + ny);
+  if (shift < (uint32_t)nx) { *px = (float)shift; *pz = 0.0f; }
+  else { *px = 0.0f; *pz = (float)(shift - (uint32_t)nx); }
+}
+
 /* world.hpp:52-54: integer cell + thousandths, restated on the counter RNG */
 void or_spawn(uint64_t pk, int32_t nx, int32_t ny, float* px, float* pz)
 {
@@ -622,7 +632,8 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
     for (int64_t i = 0; i < particles; i++) {          /* world.hpp:52-62 */
       or_wind* w = &W[i];
       w->key = or_particle_key(p->seed, *cycle, (uint64_t)i);
-      or_spawn(w->key, nx, ny, &w->px, &w->pz);
+      if (p->spawn == 1) or_spawn_edge(w->key, nx, ny, &w->px, &w->pz);
+      else or_spawn(w->key, nx, ny, &w->px, &w->pz);
       w->py = 0.0f; w->sx = w->sy = w->sz = 0.0f; w->sed = 0.0f;
       w->age = 0; w->ledger = 0; w->last_cell = 0;
     }

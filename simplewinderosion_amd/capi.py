@@ -37,7 +37,7 @@ class SweParams(ctypes.Structure):
         ("track", ctypes.c_int32),
         ("layers", ctypes.c_int32),
         ("deposit_on_death", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("spawn", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
     ]
 
@@ -120,6 +120,7 @@ SIGNATURES = {
     "swe_stats": (ctypes.c_int, [_P, ctypes.POINTER(SweStats)]),
     "swe_reset_stats": (ctypes.c_int, [_P]),
     "swe_normals": (ctypes.c_int, [_P, _FP]),
+    "swe_windmap": (ctypes.c_int, [_P, _FP]),
     "swe_generate_terrain": (
         ctypes.c_int,
         [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,

@@ -130,7 +130,7 @@ void normalize3(const float v[3], float out[3]) {
 int check_params(swe_ctx* c, const swe_params* p) {
   if (p->layers != 1 && p->layers != 2) return fail(c, SWE_EINVAL, "layers must be 1 or 2");
   if (p->window < 0) return fail(c, SWE_EINVAL, "window must be >= 0");
-  if (p->reserved != 0) return fail(c, SWE_EINVAL, "reserved must be 0");
+  if (p->spawn != 0 && p->spawn != 1) return fail(c, SWE_EINVAL, "spawn must be 0 (uniform) or 1 (edges)");
   if (p->max_age < -1 || p->max_age > (1 << 28)) return fail(c, SWE_EINVAL, "max_age out of range");
   if (p->periodic != 0 && p->periodic != 1) return fail(c, SWE_EINVAL, "periodic must be 0 or 1");
   if (p->track != 0 && p->track != 1) return fail(c, SWE_EINVAL, "track must be 0 or 1");
@@ -396,6 +396,7 @@ int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
   sp.tiles_x = c->tiles_x;
   sp.tiles_y = c->tiles_y;
   sp.periodic = c->p.periodic;
+  sp.spawn_mode = c->p.spawn;
 
   for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
     if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
@@ -481,7 +482,7 @@ void swe_params_default(swe_params* p) {
   p->track = 1;
   p->layers = 1;
   p->deposit_on_death = 0;
-  p->reserved = 0;
+  p->spawn = 0;
   p->seed = 0;
 }
 
@@ -787,6 +788,21 @@ int swe_reset_stats(swe_ctx* c) {
   int r = collect_timing(c);
   if (r) return r;
   std::memset(&c->st, 0, sizeof(c->st));
+  return 0;
+}
+
+int swe_windmap(swe_ctx* c, float* out) {
+  if (!c || !out) return SWE_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  float* tmp = nullptr;
+  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 4 * c->ncell));
+  swe::launch_windmap(c->ema + c->ncell, tmp, c->ncell, c->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 4 * c->ncell, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("windmap: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(c, SWE_EHIP, std::string("windmap sync: ") + hipGetErrorString(e2));
   return 0;
 }
 

@@ -30,16 +30,23 @@ struct FlyConsts {
 // Outcome of one call.
 enum FlyOutcome { kStepped = 0, kDied = 1 };
 
-// world.hpp:52-54,62 / wind.hpp:6-13 on the counter RNG
-__device__ __forceinline__ void spawn(uint64_t pk, int32_t nx, int32_t ny, Particle& w) {
+// world.hpp:52-54,62 / wind.hpp:6-13 on the counter RNG; mode 1 is the
+// commented boundary spawn of world.hpp:56-61 (x = 0 or y = 0 edge)
+__device__ __forceinline__ void spawn(uint64_t pk, int32_t nx, int32_t ny, int32_t mode, Particle& w) {
   const uint64_t ra = mix64(pk + 1 * kGolden);
-  const uint64_t rb = mix64(pk + 2 * kGolden);
-  const int32_t ix = (int32_t)((uint32_t)ra % (uint32_t)nx);
-  const int32_t iy = (int32_t)((uint32_t)(ra >> 32) % (uint32_t)ny);
-  const int32_t fxp = (int32_t)((uint32_t)rb % 1000u);
-  const int32_t fyp = (int32_t)((uint32_t)(rb >> 32) % 1000u);
-  w.px = (float)ix + (float)fxp / 1000.0f;
-  w.pz = (float)iy + (float)fyp / 1000.0f;
+  if (mode == 1) {
+    const uint32_t shift = (uint32_t)ra % (uint32_t)(nx + ny);
+    w.px = shift < (uint32_t)nx ? (float)shift : 0.0f;
+    w.pz = shift < (uint32_t)nx ? 0.0f : (float)(shift - (uint32_t)nx);
+  } else {
+    const uint64_t rb = mix64(pk + 2 * kGolden);
+    const int32_t ix = (int32_t)((uint32_t)ra % (uint32_t)nx);
+    const int32_t iy = (int32_t)((uint32_t)(ra >> 32) % (uint32_t)ny);
+    const int32_t fxp = (int32_t)((uint32_t)rb % 1000u);
+    const int32_t fyp = (int32_t)((uint32_t)(rb >> 32) % 1000u);
+    w.px = (float)ix + (float)fxp / 1000.0f;
+    w.pz = (float)iy + (float)fyp / 1000.0f;
+  }
   w.py = 0.0f;
   w.sed = 0.0f;
   w.sx = w.sy = w.sz = 0.0f;

@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
     const uint64_t pk = particle_key(cycle_key(a.p.seed, *a.cycle), (uint64_t)i);
     Particle w;
     if (FIRST) {
-      spawn(pk, a.nx, a.ny, w);
+      spawn(pk, a.nx, a.ny, a.p.spawn, w);
     } else {
       const float4 q0 = a.P0[i];
       const float4 q1 = a.P1[i];
@@ -302,6 +302,17 @@ __global__ void k_ema(float* ema, const int64_t* trk, int64_t n4, float lrate) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
     ema[i] = a * ema[i] + lrate * f32_from_fx(trk[i]);
 }
+// SimpleWindErosion.cpp:224-230: colour = 0.5 (1 + erf(2 m)) per momentum axis
+__global__ void k_windmap(const float* mom, float* out, int64_t n) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const float mx = mom[c], my = mom[n + c], mz = mom[2 * n + c];
+    out[4 * c + 0] = 0.5f * (1.0f + erff(2.0f * mx));
+    out[4 * c + 1] = 0.5f * (1.0f + erff(2.0f * my));
+    out[4 * c + 2] = 0.5f * (1.0f + erff(2.0f * mz));
+    out[4 * c + 3] = 1.0f;
+  }
+}
+
 __global__ void k_bump(uint64_t* cycle) {
   if (threadIdx.x == 0 && blockIdx.x == 0) cycle[0] = cycle[0] + 1;
 }
@@ -407,6 +418,9 @@ void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s) {
 }
 void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s) {
   if (n4 > 0) hipLaunchKernelGGL(k_ema, dim3(grid_for(n4, 256)), dim3(256), 0, s, ema, trk, n4, lrate);
+}
+void launch_windmap(const float* mom, float* out, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_windmap, dim3(grid_for(n, 256)), dim3(256), 0, s, mom, out, n);
 }
 void launch_bump(uint64_t* cycle, hipStream_t s) {
   hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, cycle);

@@ -84,7 +84,7 @@ struct SpawnArgs {
   const uint64_t* cycle;
   uint64_t seed;
   int64_t n;
-  int32_t nx, ny, tiles_x, tiles_y, periodic;
+  int32_t nx, ny, tiles_x, tiles_y, periodic, spawn_mode;
 };
 
 void launch_step(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,

@@ -502,7 +502,7 @@ __global__ __launch_bounds__(256) void k_spawn_count(SpawnArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= a.n) return;
   Particle w;
-  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, w);
+  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, a.spawn_mode, w);
   // the spawn cell is always on the map: round() of a position in
   // [ix, ix + 0.999] is ix or ix + 1, wrapped (periodic) or, on a bounded map,
   // possibly nx / ny, in which case the particle dies at its first call;
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= a.n) return;
   Particle w;
-  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, w);
+  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, a.spawn_mode, w);
   const int64_t o = a.base1[a.tile[i]] + a.rank[i];
   a.out.P0[o] = make_float4(w.px, w.py, w.pz, w.sed);
   a.out.P1[o] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));

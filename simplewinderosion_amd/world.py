@@ -37,7 +37,7 @@ class SweError(RuntimeError):
 
 
 def build_params(abrasion=0.0, mapscale=80.0, window=1, periodic=False, track=True, layers=1,
-                 deposit_on_death=False):
+                 deposit_on_death=False, spawn=0):
     """swe_params from the Wind/World statics plus per-map options (no device needed)."""
     p = capi.SweParams()
     capi.load().swe_params_default(ctypes.byref(p))
@@ -47,7 +47,7 @@ def build_params(abrasion=0.0, mapscale=80.0, window=1, periodic=False, track=Tr
         lrate=float(World.lrate), maxdiff=float(World.maxdiff), settling=float(World.settling),
         abrasion=float(abrasion), mapscale=float(mapscale), window=int(window), periodic=int(bool(periodic)),
         track=int(bool(track)), layers=int(layers), deposit_on_death=int(bool(deposit_on_death)),
-        seed=int(World.SEED) & 0xFFFFFFFFFFFFFFFF,
+        spawn=int(spawn), seed=int(World.SEED) & 0xFFFFFFFFFFFFFFFF,
     )
     return p
 
@@ -69,7 +69,7 @@ class World:
     settling = 1.0
 
     def __init__(self, nx=512, ny=512, device=0, periodic=False, track=True, layers=1, window=1,
-                 abrasion=0.0, deposit_on_death=False, mapscale=80.0):
+                 abrasion=0.0, deposit_on_death=False, mapscale=80.0, spawn=0):
         self.lib = capi.load()
         self.nx, self.ny = int(nx), int(ny)
         self.periodic = bool(periodic)
@@ -79,6 +79,7 @@ class World:
         self.abrasion = float(abrasion)
         self.deposit_on_death = bool(deposit_on_death)
         self.mapscale = float(mapscale)
+        self.spawn = int(spawn)
         self.ctx = ctypes.c_void_p()
         rc = self.lib.swe_create(ctypes.byref(self.ctx), self.nx, self.ny, int(device))
         if rc != 0:
@@ -89,7 +90,7 @@ class World:
     def options(self):
         return dict(abrasion=self.abrasion, mapscale=self.mapscale, window=self.window,
                     periodic=self.periodic, track=self.track, layers=self.layers,
-                    deposit_on_death=self.deposit_on_death)
+                    deposit_on_death=self.deposit_on_death, spawn=self.spawn)
 
     def params(self):
         return build_params(**self.options())
@@ -150,6 +151,12 @@ class World:
         self._check(self.lib.swe_download_master(self.ctx, _ip(h), _ip(s)), "swe_download_master")
         return h.reshape(self.nx, self.ny), s.reshape(self.nx, self.ny)
 
+    def windmap(self):
+        """RGBA wind map of the EMA momentum (SimpleWindErosion.cpp:224-230), [nx, ny, 4]."""
+        out = np.empty(4 * self.nx * self.ny, np.float32)
+        self._check(self.lib.swe_windmap(self.ctx, _fp(out)), "swe_windmap")
+        return out.reshape(self.nx, self.ny, 4)
+
     def normals(self):
         out = np.empty(3 * self.nx * self.ny, np.float32)
         self._check(self.lib.swe_normals(self.ctx, _fp(out)), "swe_normals")
@@ -191,6 +198,7 @@ class World:
         self.abrasion, self.mapscale, self.window = p.abrasion, p.mapscale, p.window
         self.periodic, self.track, self.layers = bool(p.periodic), bool(p.track), p.layers
         self.deposit_on_death = bool(p.deposit_on_death)
+        self.spawn = p.spawn
 
     def set_timing(self, on):
         self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")

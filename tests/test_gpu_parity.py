@@ -222,3 +222,38 @@ def test_checkpoint_resume_is_bit_exact(statics, tmp_path):
         assert np.array_equal(ref[1][k], got[1][k]), k
     assert ref[2] == got[2]
     w2.close()
+
+
+@pytest.mark.parametrize("window", [1, 3])
+def test_parity_edge_spawn(statics, window):
+    """The commented boundary spawn of world.hpp:56-61 (spawn = 1)."""
+    say("case edge spawn", window)
+    rng = np.random.default_rng(8)
+    h = (rng.random((60, 50)) * 0.3).astype(np.float32)
+    World.settling = 0.05
+    World.SEED = 4
+    w = World(60, 50, periodic=False, window=window, spawn=1)
+    w.upload(h)
+    st = O.OracleState(60, 50, h, params=O.params_from(w.params()))
+    st.erode(400, 2)
+    w.erode(400, n_cycles=2)
+    assert_same(w, st)
+    w.close()
+
+
+def test_windmap_matches_erf(statics):
+    """SimpleWindErosion.cpp:224-230 colour of the EMA momentum field."""
+    import math
+    say("case windmap")
+    rng = np.random.default_rng(3)
+    w = World(21, 34)
+    mom = (rng.standard_normal(3 * 21 * 34) * 0.7).astype(np.float32)
+    w.upload(np.zeros((21, 34), np.float32))
+    w.upload_fields(momentum=mom)
+    got = w.windmap()
+    m = mom.reshape(3, 21, 34)
+    for axis in range(3):
+        want = np.vectorize(lambda v: 0.5 * (1.0 + math.erf(2.0 * float(np.float32(v)))))(m[axis])
+        assert np.abs(got[..., axis] - want).max() < 3e-7
+    assert np.all(got[..., 3] == 1.0)
+    w.close()

@@ -151,3 +151,18 @@ def test_sequential_port_runs_and_loses_only_airborne_mass():
     assert steps == 50 * 513
     assert not np.array_equal(h, h0)
     assert abs(float(h.astype(np.float64).sum()) - float(h0.astype(np.float64).sum())) < 5.0
+
+
+def test_edge_spawn_positions():
+    import ctypes
+    lib = O.load()
+    px, pz = ctypes.c_float(), ctypes.c_float()
+    seen_x = seen_y = 0
+    for i in range(3000):
+        k = lib.or_particle_key(5, 0, i)
+        lib.or_spawn_edge(k, 10, 7, ctypes.byref(px), ctypes.byref(pz))
+        assert (pz.value == 0.0 and 0 <= px.value < 10) or (px.value == 0.0 and 0 <= pz.value < 7)
+        assert px.value == int(px.value) and pz.value == int(pz.value)
+        seen_x += pz.value == 0.0
+        seen_y += px.value == 0.0
+    assert 1500 < seen_x < 2100 and 900 < seen_y < 1500   # shift % (10 + 7)
