@@ -179,6 +179,14 @@ int swe_reset_stats(swe_ctx* ctx);
 int swe_windmap(swe_ctx* ctx, float* out);
 
 /*
+ * Render-mesh vertex attributes as updatenode writes them (cellpool.h:302-333):
+ * per cell P = (x, mapscale*h, y), normal, T - P, B - P (T, B the +x / +y
+ * neighbours; height 0 off a bounded map).  out: 12*nx*ny floats per cell
+ * in that order.
+ */
+int swe_mesh(swe_ctx* ctx, float* out);
+
+/*
  * Surface normals of the current snapshot, quad::_normal (cellpool.h:200-223)
  * for every cell, as used by updatenode (cellpool.h:302-333).  out: 3*nx*ny
  * floats, planes x, y, z.

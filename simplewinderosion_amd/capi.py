@@ -121,6 +121,7 @@ SIGNATURES = {
     "swe_reset_stats": (ctypes.c_int, [_P]),
     "swe_normals": (ctypes.c_int, [_P, _FP]),
     "swe_windmap": (ctypes.c_int, [_P, _FP]),
+    "swe_mesh": (ctypes.c_int, [_P, _FP]),
     "swe_generate_terrain": (
         ctypes.c_int,
         [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_float, ctypes.c_int32,

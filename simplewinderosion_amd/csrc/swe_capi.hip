@@ -791,6 +791,21 @@ int swe_reset_stats(swe_ctx* c) {
   return 0;
 }
 
+int swe_mesh(swe_ctx* c, float* out) {
+  if (!c || !out) return SWE_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  float* tmp = nullptr;
+  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 12 * c->ncell));
+  swe::launch_mesh(c->zs[c->cur()], tmp, c->nx, c->ny, c->p.mapscale, c->p.periodic != 0, c->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 12 * c->ncell, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("mesh: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(c, SWE_EHIP, std::string("mesh sync: ") + hipGetErrorString(e2));
+  return 0;
+}
+
 int swe_windmap(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
   HIPCHK(c, hipSetDevice(c->device));

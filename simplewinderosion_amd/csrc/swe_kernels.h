@@ -62,6 +62,7 @@ void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s);
 void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s);
 void launch_bump(uint64_t* cycle, hipStream_t s);
 void launch_windmap(const float* mom, float* out, int64_t n, hipStream_t s);
+void launch_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic, hipStream_t s);
 void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic,
                     hipStream_t s);
 

@@ -302,6 +302,51 @@ __global__ void k_ema(float* ema, const int64_t* trk, int64_t n4, float lrate) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
     ema[i] = a * ema[i] + lrate * f32_from_fx(trk[i]);
 }
+// updatenode (cellpool.h:302-333): per cell the vertex position P =
+// (x, mapscale*h, y), the normal, T - P and B - P with T, B the +x / +y
+// neighbours (height 0 off a bounded map, wrapped on a periodic one)
+template <bool PERIODIC>
+__global__ void k_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapscale) {
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t cx = (int32_t)(c / ny), cy = (int32_t)(c % ny);
+    const int64_t row = (int64_t)cx * ny;
+    const float h = S[c];
+    float hyp = 0.0f, hym = 0.0f, hxp = 0.0f, hxm = 0.0f;
+    bool g1, g2, g3, g4;
+    if (PERIODIC) {
+      const int32_t xp = cx + 1 == nx ? 0 : cx + 1, xm = cx == 0 ? nx - 1 : cx - 1;
+      const int32_t yp = cy + 1 == ny ? 0 : cy + 1, ym = cy == 0 ? ny - 1 : cy - 1;
+      hyp = S[row + yp];
+      hym = S[row + ym];
+      hxp = S[(int64_t)xp * ny + cy];
+      hxm = S[(int64_t)xm * ny + cy];
+      g1 = g2 = g3 = g4 = true;
+    } else {
+      const bool bxp = cx + 1 < nx, bxm = cx > 0, byp = cy + 1 < ny, bym = cy > 0;
+      g1 = bxp && byp;
+      g2 = bxm && bym;
+      g3 = bxp && bym;
+      g4 = bxm && byp;
+      if (byp) hyp = S[row + cy + 1];
+      if (bym) hym = S[row + cy - 1];
+      if (bxp) hxp = S[row + ny + cy];
+      if (bxm) hxm = S[row - ny + cy];
+    }
+    const f3 nn = surface_normal(h, hyp, hxp, hym, hxm, g1, g2, g3, g4, mapscale);
+    const float px = (float)cx, pz = (float)cy;
+    const f3 P = mk(px, mapscale * h, pz);
+    const f3 T = mk((float)(cx + 1), mapscale * hxp, pz);
+    const f3 B = mk(px, mapscale * hyp, (float)(cy + 1));
+    const f3 t = sub(T, P), b = sub(B, P);
+    float* o = out + 12 * c;
+    o[0] = P.x; o[1] = P.y; o[2] = P.z;
+    o[3] = nn.x; o[4] = nn.y; o[5] = nn.z;
+    o[6] = t.x; o[7] = t.y; o[8] = t.z;
+    o[9] = b.x; o[10] = b.y; o[11] = b.z;
+  }
+}
+
 // SimpleWindErosion.cpp:224-230: colour = 0.5 (1 + erf(2 m)) per momentum axis
 __global__ void k_windmap(const float* mom, float* out, int64_t n) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
@@ -418,6 +463,11 @@ void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s) {
 }
 void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s) {
   if (n4 > 0) hipLaunchKernelGGL(k_ema, dim3(grid_for(n4, 256)), dim3(256), 0, s, ema, trk, n4, lrate);
+}
+void launch_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic, hipStream_t s) {
+  const int64_t n = (int64_t)nx * ny;
+  if (periodic) hipLaunchKernelGGL((k_mesh<true>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
+  else hipLaunchKernelGGL((k_mesh<false>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
 }
 void launch_windmap(const float* mom, float* out, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_windmap, dim3(grid_for(n, 256)), dim3(256), 0, s, mom, out, n);

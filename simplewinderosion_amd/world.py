@@ -157,6 +157,12 @@ class World:
         self._check(self.lib.swe_windmap(self.ctx, _fp(out)), "swe_windmap")
         return out.reshape(self.nx, self.ny, 4)
 
+    def mesh(self):
+        """updatenode's vertex attributes (cellpool.h:302-333): [nx, ny, 12] = P, normal, T-P, B-P."""
+        out = np.empty(12 * self.nx * self.ny, np.float32)
+        self._check(self.lib.swe_mesh(self.ctx, _fp(out)), "swe_mesh")
+        return out.reshape(self.nx, self.ny, 12)
+
     def normals(self):
         out = np.empty(3 * self.nx * self.ny, np.float32)
         self._check(self.lib.swe_normals(self.ctx, _fp(out)), "swe_normals")

@@ -257,3 +257,31 @@ def test_windmap_matches_erf(statics):
         assert np.abs(got[..., axis] - want).max() < 3e-7
     assert np.all(got[..., 3] == 1.0)
     w.close()
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_mesh_attributes(statics, periodic):
+    """updatenode (cellpool.h:302-333): P, normal, T - P, B - P per cell."""
+    import ctypes
+    say("case mesh", periodic)
+    rng = np.random.default_rng(6)
+    nx, ny = 13, 19
+    h = (rng.random((nx, ny)) * 0.2).astype(np.float32)
+    w = World(nx, ny, periodic=periodic)
+    w.upload(h)
+    m = w.mesh()
+    S = O.unfx_vec(O._fx_vec(h)).reshape(nx, ny)
+    F = np.float32
+    out = np.zeros(3, np.float32)
+    for x in range(nx):
+        for y in range(ny):
+            hp = S[x, y]
+            hx = S[(x + 1) % nx, y] if (periodic or x + 1 < nx) else F(0)
+            hy = S[x, (y + 1) % ny] if (periodic or y + 1 < ny) else F(0)
+            P = np.array([x, F(80) * hp, y], np.float32)
+            T = np.array([x + 1, F(80) * hx, y], np.float32) - P
+            B = np.array([x, F(80) * hy, y + 1], np.float32) - P
+            O.load().or_normal(S.ravel().ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nx, ny, int(periodic),
+                               80.0, x, y, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+            assert np.array_equal(m[x, y], np.concatenate([P, out, T, B])), (x, y)
+    w.close()
