@@ -146,6 +146,27 @@ int swe_synchronize(swe_ctx* ctx);
  */
 int swe_set_path(swe_ctx* ctx, int32_t path);
 
+/*
+ * Batches: many terrains of different sizes in one context, eroded together
+ * by one launch per snapshot window (window == 1).  Terrain k's cells, tiles
+ * and particles are contiguous ranges of the context's buffers; results of
+ * each terrain equal those of a single-terrain context with the same size,
+ * inputs, seed and wind.  Shared parameters come from swe_set_params; each
+ * terrain may override the seed and the prevailing wind (pspeed may be
+ * NULL).  Single-map calls (swe_upload_layers, swe_erode, ...) act on a
+ * context of one terrain; batch contexts use the *_terrain / *_batch calls.
+ * No reference counterpart: the reference erodes one map.
+ */
+int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const int32_t* ny, int32_t device);
+int swe_get_terrains(const swe_ctx* ctx, int32_t* n_terrains);
+int swe_set_terrain(swe_ctx* ctx, int32_t k, uint64_t seed, const float* pspeed);
+int swe_upload_terrain(swe_ctx* ctx, int32_t k, const float* height, const float* sediment);
+int swe_download_terrain(swe_ctx* ctx, int32_t k, float* height, float* sediment, float* massflow, float* momentum);
+int swe_download_terrain_master(swe_ctx* ctx, int32_t k, int64_t* height_fx, int64_t* sediment_fx);
+int swe_audit_terrain(swe_ctx* ctx, int32_t k, struct swe_audit_t* out);
+/* One map-cycle per n_cycles for every terrain; particles[k] per terrain. */
+int swe_erode_batch(swe_ctx* ctx, const int64_t* particles, int32_t n_cycles);
+
 /* Exact mass audit of the current state (synchronizes). */
 int swe_audit(swe_ctx* ctx, swe_audit_t* out);
 /* RNG cycle counter (for resume); setting it does not touch the map. */

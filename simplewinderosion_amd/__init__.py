@@ -5,6 +5,6 @@ libswe.so behind the C-ABI of include/swe.h; this package is the Python mirror
 of the reference's World/Wind interface over that library.
 """
 from . import capi
-from .world import SweError, Wind, World, generate_terrain, glm_normalize
+from .world import Batch, SweError, Wind, World, generate_terrain, glm_normalize
 
-__all__ = ["capi", "SweError", "Wind", "World", "generate_terrain", "glm_normalize"]
+__all__ = ["capi", "Batch", "SweError", "Wind", "World", "generate_terrain", "glm_normalize"]

@@ -128,6 +128,15 @@ SIGNATURES = {
          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _FP],
     ),
     "swe_version": (ctypes.c_char_p, []),
+    "swe_create_batch": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
+    "swe_get_terrains": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "swe_set_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_uint64, _FP]),
+    "swe_upload_terrain": (ctypes.c_int, [_P, ctypes.c_int32, _FP, _FP]),
+    "swe_download_terrain": (ctypes.c_int, [_P, ctypes.c_int32, _FP, _FP, _FP, _FP]),
+    "swe_download_terrain_master": (ctypes.c_int, [_P, ctypes.c_int32, _I64P, _I64P]),
+    "swe_audit_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(SweAudit)]),
+    "swe_erode_batch": (ctypes.c_int, [_P, _I64P, ctypes.c_int32]),
 }
 
 _lib = None

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <array>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -51,8 +52,18 @@ struct swe_ctx {
   int64_t* L = nullptr;
   int32_t* LC = nullptr;
 
+  // terrains packed into the buffers (a plain map is one terrain)
+  int32_t nterr = 1;
+  std::vector<int32_t> tnx, tny, ttx, tty, ttile0;
+  std::vector<int64_t> tcell0;
+  std::vector<uint64_t> tseed;
+  std::vector<std::array<float, 3>> tps;
+  std::vector<char> tseed_set, tps_set;
+  int32_t ntiles = 0;
+  swe::Terrain* dterr = nullptr;
+  int32_t* dtile_terrain = nullptr;
+
   // tile-owned step path
-  int32_t tiles_x = 0, tiles_y = 0;
   int64_t* fD[2] = {nullptr, nullptr};
   int64_t* fDS[2] = {nullptr, nullptr};
   uint32_t* fV[2] = {nullptr, nullptr};
@@ -66,7 +77,7 @@ struct swe_ctx {
   uint32_t* sp_rank = nullptr;
 
   // counters (device)
-  int64_t* lost = nullptr;     // swe::kLostSlots
+  int64_t* lost = nullptr;     // swe::kLostSlots per terrain
   uint64_t* dcycle = nullptr;  // RNG cycle counter
   uint64_t cycle = 0;          // host mirror
 
@@ -190,7 +201,7 @@ int alloc_pbuf(swe_ctx* c, swe::ParticleBuf& b, int64_t n) {
 int ensure_step(swe_ctx* c, int64_t n) {
   int r;
   if (!c->fD[0]) {
-    const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
+    const int64_t nt = c->ntiles;
     if ((r = dalloc(c, &c->fD[0], c->ncell)) || (r = dalloc(c, &c->fD[1], c->ncell)) ||
         (r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
         (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->tbase[0], nt)) || (r = dalloc(c, &c->tbase[1], nt)) ||
@@ -351,25 +362,52 @@ int erode_global(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin, 
   return 0;
 }
 
-// World::erode with one-call windows on the tile-owned step kernel: spawn and
-// bin by tile, then one k_step per window (relax of the previous window +
-// one fly() call of every particle), then a relax-only k_step
-int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
+// terrain records for the device: seed / pspeed from the params unless a
+// terrain overrides them
+int push_terrains(swe_ctx* c, const int64_t* particles) {
+  std::vector<swe::Terrain> t((size_t)c->nterr);
+  int64_t part0 = 0;
+  for (int32_t k = 0; k < c->nterr; k++) {
+    swe::Terrain& T = t[(size_t)k];
+    T.cell0 = c->tcell0[(size_t)k];
+    T.part0 = part0;
+    T.npart = particles[k];
+    part0 += particles[k];
+    T.nx = c->tnx[(size_t)k];
+    T.ny = c->tny[(size_t)k];
+    T.tiles_x = c->ttx[(size_t)k];
+    T.tiles_y = c->tty[(size_t)k];
+    T.tile0 = c->ttile0[(size_t)k];
+    T.rcp_nx = 1.0f / (float)T.nx;
+    T.rcp_ny = 1.0f / (float)T.ny;
+    T.seed = c->tseed_set[(size_t)k] ? c->tseed[(size_t)k] : c->p.seed;
+    const float* ps = c->tps_set[(size_t)k] ? c->tps[(size_t)k].data() : c->p.pspeed;
+    for (int q = 0; q < 3; q++) T.ps[q] = ps[q];
+    normalize3(ps, T.nps);
+  }
+  HIPCHK(c, hipMemcpyAsync(c->dterr, t.data(), sizeof(swe::Terrain) * t.size(), hipMemcpyHostToDevice, c->stream));
+  return 0;
+}
+
+// World::erode with one-call windows on the tile-owned step kernel, for every
+// terrain of the context at once: spawn and bin by tile, then one k_step per
+// window (relax of the previous window + one fly() call of every particle),
+// then a relax-only k_step
+int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t nwin) {
   int r;
-  if ((r = ensure_step(c, particles > 0 ? particles : 1))) return r;
+  int64_t total = 0;
+  for (int32_t k = 0; k < c->nterr; k++) total += particles[k];
+  if ((r = ensure_step(c, total > 0 ? total : 1))) return r;
+  if ((r = push_terrains(c, particles))) return r;
   const bool layers2 = c->p.layers == 2;
   swe::StepArgs sa{};
   sa.trk = c->trk;
   sa.lost = c->lost;
   sa.cycle = c->dcycle;
-  sa.seed = c->p.seed;
+  sa.terr = c->dterr;
+  sa.tile_terrain = c->dtile_terrain;
   sa.ncell = c->ncell;
-  sa.nx = c->nx;
-  sa.ny = c->ny;
-  sa.tiles_x = c->tiles_x;
-  sa.tiles_y = c->tiles_y;
-  sa.rcp_nx = 1.0f / (float)c->nx;
-  sa.rcp_ny = 1.0f / (float)c->ny;
+  sa.ntiles = c->ntiles;
   sa.mapscale = c->p.mapscale;
   sa.boundarylayer = c->p.boundarylayer;
   sa.suspension = c->p.suspension;
@@ -378,9 +416,8 @@ int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
   sa.maxdiff = c->p.maxdiff;
   sa.settling = c->p.settling;
   sa.max_age = c->p.max_age;
-  fill_consts(c, sa.ps, sa.nps);
 
-  const int64_t nt = (int64_t)c->tiles_x * c->tiles_y;
+  const int64_t nt = c->ntiles;
   swe::SpawnArgs sp{};
   sp.out = c->PB[0];
   sp.tile = c->sp_tile;
@@ -389,18 +426,16 @@ int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
   sp.base1 = c->tbase[1];
   sp.cnt = c->tcnt[1];
   sp.cycle = c->dcycle;
-  sp.seed = c->p.seed;
-  sp.n = particles;
-  sp.nx = c->nx;
-  sp.ny = c->ny;
-  sp.tiles_x = c->tiles_x;
-  sp.tiles_y = c->tiles_y;
+  sp.terr = c->dterr;
+  sp.nterr = c->nterr;
+  sp.n = total;
+  sp.ntiles = c->ntiles;
   sp.periodic = c->p.periodic;
   sp.spawn_mode = c->p.spawn;
 
   for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
     if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
-    if (particles > 0) {
+    if (total > 0) {
       swe::launch_zero32(c->nin, 3 * nt, c->stream);
       swe::launch_zero32(c->farcnt, 3, c->stream);
       swe::launch_spawn(sp, c->stream);
@@ -452,6 +487,7 @@ int erode_tiled(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin) {
     HIPCHK(c, hipGetLastError());
     c->cycle++;
     c->st.cycles++;
+    c->st.particle_steps += 0;
     if (c->timing && c->pending.size() > 512) {
       if ((r = collect_timing(c))) return r;
     }
@@ -522,27 +558,56 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->sp_rank);
   dfree(c->lost);
   dfree(c->dcycle);
+  dfree(c->dterr);
+  dfree(c->dtile_terrain);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
 
-int swe_create(swe_ctx** out, int32_t nx, int32_t ny, int32_t device) {
+int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const int32_t* ny, int32_t device) {
   if (!out) return SWE_EINVAL;
   *out = nullptr;
-  if (nx < 1 || ny < 1) return SWE_EINVAL;
-  if ((int64_t)nx * ny > ((int64_t)1 << 31) - 1) return SWE_EINVAL;  // cell index fits int32
+  if (n_terrains < 1 || n_terrains > 65536 || !nx || !ny) return SWE_EINVAL;
+  int64_t cells = 0, tiles = 0;
+  for (int32_t k = 0; k < n_terrains; k++) {
+    if (nx[k] < 1 || ny[k] < 1) return SWE_EINVAL;
+    if ((int64_t)nx[k] * ny[k] > ((int64_t)1 << 31) - 1) return SWE_EINVAL;  // cell index fits int32
+    cells += (int64_t)nx[k] * ny[k];
+    tiles += (int64_t)((nx[k] + swe::kStepTileX - 1) / swe::kStepTileX) * ((ny[k] + swe::kStepTileY - 1) / swe::kStepTileY);
+  }
+  if (tiles > ((int64_t)1 << 30)) return SWE_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SWE_EHIP;
   if (device < 0 || device >= ndev) return SWE_EINVAL;
   swe_ctx* c = new swe_ctx();
   c->device = device;
-  c->nx = nx;
-  c->ny = ny;
-  c->ncell = (int64_t)nx * ny;
-  c->tiles_x = (nx + swe::kStepTileX - 1) / swe::kStepTileX;
-  c->tiles_y = (ny + swe::kStepTileY - 1) / swe::kStepTileY;
+  c->nterr = n_terrains;
+  c->nx = nx[0];
+  c->ny = ny[0];
+  c->ncell = cells;
+  c->ntiles = (int32_t)tiles;
+  std::vector<int32_t> tile_terrain((size_t)tiles);
+  int64_t cell0 = 0;
+  int32_t tile0 = 0;
+  for (int32_t k = 0; k < n_terrains; k++) {
+    const int32_t tx = (nx[k] + swe::kStepTileX - 1) / swe::kStepTileX;
+    const int32_t ty = (ny[k] + swe::kStepTileY - 1) / swe::kStepTileY;
+    c->tnx.push_back(nx[k]);
+    c->tny.push_back(ny[k]);
+    c->ttx.push_back(tx);
+    c->tty.push_back(ty);
+    c->ttile0.push_back(tile0);
+    c->tcell0.push_back(cell0);
+    for (int32_t t = 0; t < tx * ty; t++) tile_terrain[(size_t)(tile0 + t)] = k;
+    cell0 += (int64_t)nx[k] * ny[k];
+    tile0 += tx * ty;
+  }
+  c->tseed.assign((size_t)n_terrains, 0);
+  c->tps.assign((size_t)n_terrains, {1.0f, 0.0f, 0.0f});
+  c->tseed_set.assign((size_t)n_terrains, 0);
+  c->tps_set.assign((size_t)n_terrains, 0);
   swe_params_default(&c->p);
   int r = 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -553,16 +618,41 @@ int swe_create(swe_ctx** out, int32_t nx, int32_t ny, int32_t device) {
       (r = dalloc(c, &c->zs[0], c->ncell)) || (r = dalloc(c, &c->zs[1], c->ncell)) ||
       (r = dalloc(c, &c->dz, c->ncell)) ||
       (r = dalloc(c, &c->vis[0], c->ncell)) || (r = dalloc(c, &c->vis[1], c->ncell)) ||
-      (r = dalloc(c, &c->ema, 4 * c->ncell)) || (r = dalloc(c, &c->lost, swe::kLostSlots)) ||
-      (r = dalloc(c, &c->dcycle, 1)) || (r = ensure_layers(c))) {
+      (r = dalloc(c, &c->ema, 4 * c->ncell)) || (r = dalloc(c, &c->lost, (int64_t)swe::kLostSlots * n_terrains)) ||
+      (r = dalloc(c, &c->dcycle, 1)) || (r = dalloc(c, &c->dterr, n_terrains)) ||
+      (r = dalloc(c, &c->dtile_terrain, tiles)) || (r = ensure_layers(c))) {
     swe_destroy(c);
     return r;
   }
-  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+  if (hipMemcpyAsync(c->dtile_terrain, tile_terrain.data(), sizeof(int32_t) * tile_terrain.size(),
+                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
     swe_destroy(c);
     return SWE_EHIP;
   }
   *out = c;
+  return 0;
+}
+
+int swe_create(swe_ctx** out, int32_t nx, int32_t ny, int32_t device) {
+  return swe_create_batch(out, 1, &nx, &ny, device);
+}
+
+int swe_set_terrain(swe_ctx* c, int32_t k, uint64_t seed, const float* pspeed) {
+  if (!c) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
+  c->tseed[(size_t)k] = seed;
+  c->tseed_set[(size_t)k] = 1;
+  if (pspeed) {
+    for (int q = 0; q < 3; q++) c->tps[(size_t)k][q] = pspeed[q];
+    c->tps_set[(size_t)k] = 1;
+  }
+  return 0;
+}
+
+int swe_get_terrains(const swe_ctx* c, int32_t* n_terrains) {
+  if (!c || !n_terrains) return SWE_EINVAL;
+  *n_terrains = c->nterr;
   return 0;
 }
 
@@ -587,30 +677,30 @@ int swe_get_params(const swe_ctx* c, swe_params* p) {
   return 0;
 }
 
-int swe_upload_layers(swe_ctx* c, const float* height, const float* sediment) {
+int swe_upload_terrain(swe_ctx* c, int32_t k, const float* height, const float* sediment) {
   if (!c || !height) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  if (sediment && !c->smv[0]) {
+    // a sediment layer can be uploaded before layers == 2 is selected
+    int r;
+    if ((r = dalloc(c, &c->smv[0], c->ncell)) || (r = dalloc(c, &c->smv[1], c->ncell)) ||
+        (r = dalloc(c, &c->ss[0], c->ncell)) || (r = dalloc(c, &c->ss[1], c->ncell)) ||
+        (r = dalloc(c, &c->ds, c->ncell)))
+      return r;
+  }
   float* tmp = nullptr;
-  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * c->ncell));
-  hipError_t e = hipMemcpyAsync(tmp, height, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream);
+  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * n));
+  hipError_t e = hipMemcpyAsync(tmp, height, sizeof(float) * n, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
-    swe::launch_from_f32(tmp, c->zmv[c->cur()], c->zs[c->cur()], c->ncell, c->stream);
+    swe::launch_from_f32(tmp, c->zmv[c->cur()] + c0, c->zs[c->cur()] + c0, n, c->stream);
     e = hipGetLastError();
   }
   if (e == hipSuccess && sediment) {
-    int r = 0;
-    if (!c->smv[0]) {
-      // a sediment layer can be uploaded before layers == 2 is selected
-      if ((r = dalloc(c, &c->smv[0], c->ncell)) || (r = dalloc(c, &c->smv[1], c->ncell)) ||
-          (r = dalloc(c, &c->ss[0], c->ncell)) || (r = dalloc(c, &c->ss[1], c->ncell)) ||
-          (r = dalloc(c, &c->ds, c->ncell))) {
-        (void)hipFree(tmp);
-        return r;
-      }
-    }
-    e = hipMemcpyAsync(tmp, sediment, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream);
+    e = hipMemcpyAsync(tmp, sediment, sizeof(float) * n, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-      swe::launch_from_f32(tmp, c->smv[c->cur()], c->ss[c->cur()], c->ncell, c->stream);
+      swe::launch_from_f32(tmp, c->smv[c->cur()] + c0, c->ss[c->cur()] + c0, n, c->stream);
       e = hipGetLastError();
     }
   }
@@ -621,8 +711,15 @@ int swe_upload_layers(swe_ctx* c, const float* height, const float* sediment) {
   return 0;
 }
 
+int swe_upload_layers(swe_ctx* c, const float* height, const float* sediment) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context uploads with swe_upload_terrain");
+  return swe_upload_terrain(c, 0, height, sediment);
+}
+
 int swe_upload_master(swe_ctx* c, const int64_t* height_fx, const int64_t* sediment_fx) {
   if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   if (height_fx) {
     HIPCHK(c, hipMemcpyAsync(c->zmv[c->cur()], height_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
@@ -641,6 +738,7 @@ int swe_upload_master(swe_ctx* c, const int64_t* height_fx, const int64_t* sedim
 
 int swe_upload_fields(swe_ctx* c, const float* massflow, const float* momentum) {
   if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   if (massflow) HIPCHK(c, hipMemcpyAsync(c->ema, massflow, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream));
   if (momentum)
@@ -649,50 +747,82 @@ int swe_upload_fields(swe_ctx* c, const float* massflow, const float* momentum) 
   return 0;
 }
 
-int swe_download_layers(swe_ctx* c, float* height, float* sediment, float* massflow, float* momentum) {
+int swe_download_terrain(swe_ctx* c, int32_t k, float* height, float* sediment, float* massflow, float* momentum) {
   if (!c) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
-  if (height) HIPCHK(c, hipMemcpyAsync(height, c->zs[c->cur()], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  if (height) HIPCHK(c, hipMemcpyAsync(height, c->zs[c->cur()] + c0, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
   if (sediment) {
     if (c->ss[c->cur()])
-      HIPCHK(c, hipMemcpyAsync(sediment, c->ss[c->cur()], sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(sediment, c->ss[c->cur()] + c0, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     else
-      std::memset(sediment, 0, sizeof(float) * c->ncell);
+      std::memset(sediment, 0, sizeof(float) * n);
   }
-  if (massflow) HIPCHK(c, hipMemcpyAsync(massflow, c->ema, sizeof(float) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  if (massflow) HIPCHK(c, hipMemcpyAsync(massflow, c->ema + c0, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
   if (momentum)
-    HIPCHK(c, hipMemcpyAsync(momentum, c->ema + c->ncell, sizeof(float) * 3 * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    for (int q = 0; q < 3; q++)
+      HIPCHK(c, hipMemcpyAsync(momentum + q * n, c->ema + (q + 1) * c->ncell + c0, sizeof(float) * n,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int swe_download_layers(swe_ctx* c, float* height, float* sediment, float* massflow, float* momentum) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context downloads with swe_download_terrain");
+  return swe_download_terrain(c, 0, height, sediment, massflow, momentum);
+}
+
+int swe_download_terrain_master(swe_ctx* c, int32_t k, int64_t* height_fx, int64_t* sediment_fx) {
+  if (!c) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  if (height_fx)
+    HIPCHK(c, hipMemcpyAsync(height_fx, c->zmv[c->cur()] + c0, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (sediment_fx) {
+    if (c->smv[0])
+      HIPCHK(c, hipMemcpyAsync(sediment_fx, c->smv[c->cur()] + c0, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                               c->stream));
+    else
+      std::memset(sediment_fx, 0, sizeof(int64_t) * n);
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
 int swe_download_master(swe_ctx* c, int64_t* height_fx, int64_t* sediment_fx) {
   if (!c) return SWE_EINVAL;
-  HIPCHK(c, hipSetDevice(c->device));
-  if (height_fx) HIPCHK(c, hipMemcpyAsync(height_fx, c->zmv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
-  if (sediment_fx) {
-    if (c->smv[0])
-      HIPCHK(c, hipMemcpyAsync(sediment_fx, c->smv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
-    else
-      std::memset(sediment_fx, 0, sizeof(int64_t) * c->ncell);
-  }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return 0;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context downloads with swe_download_terrain_master");
+  return swe_download_terrain_master(c, 0, height_fx, sediment_fx);
 }
 
-int swe_erode(swe_ctx* c, int64_t particles, int32_t n_cycles) {
-  if (!c) return SWE_EINVAL;
-  if (particles < 0 || n_cycles < 0) return fail(c, SWE_EINVAL, "particles and n_cycles must be >= 0");
-  if (particles > ((int64_t)1 << 31) - 256) return fail(c, SWE_EINVAL, "particles per cycle above 2^31");
+int swe_erode_batch(swe_ctx* c, const int64_t* particles, int32_t n_cycles) {
+  if (!c || !particles) return SWE_EINVAL;
+  if (n_cycles < 0) return fail(c, SWE_EINVAL, "n_cycles must be >= 0");
+  int64_t total = 0;
+  for (int32_t k = 0; k < c->nterr; k++) {
+    if (particles[k] < 0) return fail(c, SWE_EINVAL, "particles must be >= 0");
+    total += particles[k];
+  }
+  if (total > ((int64_t)1 << 31) - 256) return fail(c, SWE_EINVAL, "particles per cycle above 2^31");
   HIPCHK(c, hipSetDevice(c->device));
   int r;
   if ((r = ensure_layers(c))) return r;
   int32_t R;
   const int64_t nwin = windows_of(c->p, &R);
-  const bool tiled = c->path == 2 || (c->path == 0 && R == 1);
-  if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path needs window == 1");
+  const bool tiled = c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1;
+  if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path (and batches) need window == 1");
   if (tiled) return erode_tiled(c, particles, n_cycles, nwin);
-  return erode_global(c, particles, n_cycles, nwin, R);
+  return erode_global(c, particles[0], n_cycles, nwin, R);
+}
+
+int swe_erode(swe_ctx* c, int64_t particles, int32_t n_cycles) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context erodes with swe_erode_batch");
+  if (particles < 0) return fail(c, SWE_EINVAL, "particles and n_cycles must be >= 0");
+  return swe_erode_batch(c, &particles, n_cycles);
 }
 
 int swe_set_path(swe_ctx* c, int32_t path) {
@@ -709,23 +839,26 @@ int swe_synchronize(swe_ctx* c) {
   return collect_timing(c);
 }
 
-int swe_audit(swe_ctx* c, swe_audit_t* out) {
+int swe_audit_terrain(swe_ctx* c, int32_t k, swe_audit_t* out) {
   if (!c || !out) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
-  std::vector<int64_t> z((size_t)c->ncell), s;
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  std::vector<int64_t> z((size_t)n), s;
   int64_t lost[swe::kLostSlots];
-  HIPCHK(c, hipMemcpyAsync(z.data(), c->zmv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(z.data(), c->zmv[c->cur()] + c0, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
   if (c->smv[0]) {
-    s.resize((size_t)c->ncell);
-    HIPCHK(c, hipMemcpyAsync(s.data(), c->smv[c->cur()], sizeof(int64_t) * c->ncell, hipMemcpyDeviceToHost, c->stream));
+    s.resize((size_t)n);
+    HIPCHK(c, hipMemcpyAsync(s.data(), c->smv[c->cur()] + c0, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
   }
-  HIPCHK(c, hipMemcpyAsync(lost, c->lost, sizeof(lost), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lost, c->lost + (int64_t)k * swe::kLostSlots, sizeof(lost), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   __int128 hz = 0, hs = 0;
   for (int64_t v : z) hz += v;
   for (int64_t v : s) hs += v;
   int64_t l = 0;
-  for (int k = 0; k < swe::kLostSlots; k++) l += lost[k];
+  for (int q = 0; q < swe::kLostSlots; q++) l += lost[q];
   out->height_sum_lo = (int64_t)(uint64_t)hz;
   out->height_sum_hi = (int64_t)(hz >> 64);
   out->sediment_sum_lo = (int64_t)(uint64_t)hs;
@@ -733,6 +866,12 @@ int swe_audit(swe_ctx* c, swe_audit_t* out) {
   out->lost = l;
   out->cycle = c->cycle;
   return 0;
+}
+
+int swe_audit(swe_ctx* c, swe_audit_t* out) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context audits with swe_audit_terrain");
+  return swe_audit_terrain(c, 0, out);
 }
 
 int swe_get_size(const swe_ctx* c, int32_t* nx, int32_t* ny) {
@@ -759,6 +898,7 @@ int swe_set_cycle(swe_ctx* c, uint64_t cycle) {
 
 int swe_set_lost(swe_ctx* c, int64_t lost) {
   if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   int64_t slots[swe::kLostSlots] = {0};
   slots[0] = lost;
@@ -793,6 +933,7 @@ int swe_reset_stats(swe_ctx* c) {
 
 int swe_mesh(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   float* tmp = nullptr;
   HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 12 * c->ncell));
@@ -808,6 +949,7 @@ int swe_mesh(swe_ctx* c, float* out) {
 
 int swe_windmap(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   float* tmp = nullptr;
   HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 4 * c->ncell));
@@ -823,6 +965,7 @@ int swe_windmap(swe_ctx* c, float* out) {
 
 int swe_normals(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
   HIPCHK(c, hipSetDevice(c->device));
   float* tmp = nullptr;
   HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 3 * c->ncell));

@@ -24,6 +24,18 @@ constexpr int kStepTileY = SWE_STEP_TILE_Y;    // tile extent along y (fast axis
 constexpr int kStepTileThreads = SWE_STEP_THREADS;
 constexpr int kStepListCap = SWE_STEP_LIST;    // particles gathered in LDS per tile
 
+// One terrain of a context: its cells, tiles and particles are contiguous
+// ranges of the context's buffers
+struct Terrain {
+  int64_t cell0;   // first cell in the map arrays
+  int64_t part0;   // first particle index of a cycle
+  int64_t npart;   // particles per cycle
+  int32_t nx, ny, tiles_x, tiles_y, tile0;
+  float rcp_nx, rcp_ny;
+  uint64_t seed;
+  float ps[3], nps[3];  // pspeed and normalize(pspeed)
+};
+
 // One particle buffer (structure of arrays)
 struct ParticleBuf {
   float4* P0;     // pos.xyz, sediment
@@ -65,13 +77,12 @@ struct StepArgs {
   uint32_t* nin_next;        // ... next window (accumulated here)
   uint32_t* nin_clear;       // ... the window after next (zeroed here)
   const uint64_t* cycle;
-  uint64_t seed;
-  int64_t ncell;
-  int32_t nx, ny, tiles_x, tiles_y;
-  float rcp_nx, rcp_ny;
+  const Terrain* terr;
+  const int32_t* tile_terrain;  // per global tile
+  int64_t ncell;                // cells of all terrains (track plane stride)
+  int32_t ntiles;
   float mapscale, boundarylayer, suspension, gravity, abrasion, maxdiff, settling;
   int32_t max_age;
-  float ps[3], nps[3];
 };
 
 struct SpawnArgs {
@@ -82,9 +93,11 @@ struct SpawnArgs {
   int64_t* base1;    // per tile: segment base of the spawned particles
   int64_t* cnt;      // per tile: segment length
   const uint64_t* cycle;
-  uint64_t seed;
-  int64_t n;
-  int32_t nx, ny, tiles_x, tiles_y, periodic, spawn_mode;
+  const Terrain* terr;
+  int32_t nterr;
+  int64_t n;         // particles of all terrains
+  int32_t ntiles;
+  int32_t periodic, spawn_mode;
 };
 
 void launch_step(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,

@@ -93,15 +93,36 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   __shared__ uint32_t s_list[kList];
   __shared__ uint32_t s_nlist;
 
-  const int32_t nx = a.nx, ny = a.ny;
-  const int32_t tiles_x = a.tiles_x, tiles_y = a.tiles_y;
   // XCD-aware order: XCD k (workgroups b with b % 8 == k) takes a contiguous
   // range of tiles, so neighbouring tiles share halo lines in one L2.
   const uint32_t nb = gridDim.x, b = blockIdx.x;
   const uint32_t q8 = nb / 8, r8 = nb % 8, xcd = b % 8;
-  const int32_t u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);
-  const int32_t tx = u / tiles_y, ty = u % tiles_y;
+  const int32_t u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);  // global tile
+  // the terrain this tile belongs to; all map indices below are terrain-local
+  const Terrain T = a.terr[a.tile_terrain[u]];
+  const int32_t nx = T.nx, ny = T.ny;
+  const int32_t tiles_x = T.tiles_x, tiles_y = T.tiles_y;
+  const int32_t t0 = T.tile0;
+  const int32_t ul = u - t0;
+  const int32_t tx = ul / tiles_y, ty = ul % tiles_y;
   const int32_t x0 = tx * kSX, y0 = ty * kSY;
+  const float rcp_nx = T.rcp_nx, rcp_ny = T.rcp_ny;
+  const int64_t c0 = T.cell0;
+  const int64_t* zm_in = a.zm_in + c0;
+  int64_t* zm_out = a.zm_out + c0;
+  const float* snap_in = a.snap_in + c0;
+  float* snap_out = a.snap_out + c0;
+  const int64_t* sm_in = LAYERS == 2 ? a.sm_in + c0 : nullptr;
+  int64_t* sm_out = LAYERS == 2 ? a.sm_out + c0 : nullptr;
+  const float* ssnap_in = LAYERS == 2 ? a.ssnap_in + c0 : nullptr;
+  float* ssnap_out = LAYERS == 2 ? a.ssnap_out + c0 : nullptr;
+  const int64_t* d_in = a.d_in + c0;
+  const int64_t* ds_in = LAYERS == 2 ? a.ds_in + c0 : nullptr;
+  const uint32_t* v_in = a.v_in + c0;
+  int64_t* d_out = a.d_out + c0;
+  int64_t* ds_out = LAYERS == 2 ? a.ds_out + c0 : nullptr;
+  uint32_t* v_out = a.v_out + c0;
+  int64_t* trk = TRACK ? a.trk + c0 : nullptr;
 
   // distinct source tiles and, for each, the direction codes that lead here
   if (threadIdx.x == 0) {
@@ -115,7 +136,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       } else if (sx < 0 || sy < 0 || sx >= tiles_x || sy >= tiles_y) {
         continue;
       }
-      const int32_t src = sx * tiles_y + sy;
+      const int32_t src = t0 + sx * tiles_y + sy;
       int j = 0;
       while (j < n && s_src[j] != src) j++;
       if (j == n) {
@@ -165,15 +186,15 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       float v = 0.0f, vs = 0.0f;
       bool ok = true;
       if (PERIODIC) {
-        gx = wrap(gx, nx, a.rcp_nx);
-        gy = wrap(gy, ny, a.rcp_ny);
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
       if (ok) {
         const int64_t g = (int64_t)gx * ny + gy;
-        v = a.snap_in[g];
-        if (LAYERS == 2) vs = a.ssnap_in[g];
+        v = snap_in[g];
+        if (LAYERS == 2) vs = ssnap_in[g];
       }
       s_old[e] = v;
       if (LAYERS == 2) s_sold[e] = vs;
@@ -183,12 +204,12 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       uint32_t v = 0;
       bool ok = true;
       if (PERIODIC) {
-        gx = wrap(gx, nx, a.rcp_nx);
-        gy = wrap(gy, ny, a.rcp_ny);
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
-      if (ok) v = a.v_in[(int64_t)gx * ny + gy];
+      if (ok) v = v_in[(int64_t)gx * ny + gy];
       s_vold[e] = v;
     }
     for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
@@ -239,8 +260,8 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       int32_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
       bool ok = true;
       if (PERIODIC) {
-        gx = wrap(gx, nx, a.rcp_nx);
-        gy = wrap(gy, ny, a.rcp_ny);
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -251,22 +272,22 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       }
       const int64_t g = (int64_t)gx * ny + gy;
       const int64_t cas = s_casc[e];
-      const int64_t z = a.zm_in[g] + a.d_in[g] + cas;
+      const int64_t z = zm_in[g] + d_in[g] + cas;
       const float zf = f32_from_fx(z);
       s_new[e] = zf;
       const bool mine = lx >= 1 && ly >= 1 && lx <= kSX && ly <= kSY && x0 - 1 + lx < nx && y0 - 1 + ly < ny;
       if (mine) {
-        a.zm_out[g] = z;
-        a.snap_out[g] = zf;
+        zm_out[g] = z;
+        snap_out[g] = zf;
       }
       if (LAYERS == 2) {
-        int64_t s = a.sm_in[g] + a.ds_in[g] + cas;
+        int64_t s = sm_in[g] + ds_in[g] + cas;
         if (s < 0) s = 0;
         const float sf = f32_from_fx(s);
         s_snew[e] = sf;
         if (mine) {
-          a.sm_out[g] = s;
-          a.ssnap_out[g] = sf;
+          sm_out[g] = s;
+          ssnap_out[g] = sf;
         }
       }
     }
@@ -309,17 +330,18 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     FlyConsts k;
     k.nx = nx;
     k.ny = ny;
-    k.rcp_nx = a.rcp_nx;
-    k.rcp_ny = a.rcp_ny;
+    k.rcp_nx = rcp_nx;
+    k.rcp_ny = rcp_ny;
     k.mapscale = a.mapscale;
     k.max_age = a.max_age;
     k.boundarylayer = a.boundarylayer;
     k.suspension = a.suspension;
     k.gravity = a.gravity;
     k.abrasion = a.abrasion;
-    k.ps = mk(a.ps[0], a.ps[1], a.ps[2]);
-    k.nps = mk(a.nps[0], a.nps[1], a.nps[2]);
-    const uint64_t ck = cycle_key(a.seed, *a.cycle);
+    k.ps = mk(T.ps[0], T.ps[1], T.ps[2]);
+    k.nps = mk(T.nps[0], T.nps[1], T.nps[2]);
+    const uint64_t ck = cycle_key(T.seed, *a.cycle);
+    const int64_t p0 = T.part0;
     const int64_t out_base = s_base;
     const int32_t nsrc = s_nsrc;
     const uint32_t far_n = *a.far_cnt_in;
@@ -335,7 +357,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       w.ledger = in.L[j];
       w.lastc = in.LC[j];
       const uint32_t pid = in.PID[j];
-      const uint64_t pk = particle_key(ck, pid);
+      const uint64_t pk = particle_key(ck, (uint64_t)((int64_t)pid - p0));
       int32_t cx, cy;
       const bool ok = cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);
       int64_t dead_cell = -1;
@@ -433,6 +455,8 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
+      const int64_t fp = (int64_t)a.far_in.PID[j];
+      if (fp < p0 || fp >= p0 + T.npart) continue;  // another terrain's particle
       int32_t cx, cy;
       const float4 q0 = a.far_in.P0[j];
       int32_t ctx, cty;
@@ -454,14 +478,14 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
     if (gx >= nx || gy >= ny) continue;
     const int64_t g = (int64_t)gx * ny + gy;
-    a.d_out[g] = acc_d[e];
-    a.v_out[g] = acc_v[e];
-    if (LAYERS == 2) a.ds_out[g] = acc_ds[e];
+    d_out[g] = acc_d[e];
+    v_out[g] = acc_v[e];
+    if (LAYERS == 2) ds_out[g] = acc_ds[e];
     if (TRACK) {
 #pragma unroll
       for (int f = 0; f < 4; f++) {
         const int64_t t = acc_t[f * kCell + e];
-        if (t != 0) a.trk[f * a.ncell + g] += t;
+        if (t != 0) trk[f * a.ncell + g] += t;
       }
     }
   }
@@ -476,7 +500,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
         dtx = wrapt(dtx, tiles_x);
         dty = wrapt(dty, tiles_y);
       }
-      atomicAdd(&a.nin_next[dtx * tiles_y + dty], n);
+      atomicAdd(&a.nin_next[t0 + dtx * tiles_y + dty], n);
     }
   }
   if (threadIdx.x == 0) {
@@ -491,31 +515,45 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   if (!DEPOSIT && threadIdx.x == 0) {
     int64_t l = 0;
     for (int i = 0; i < kStepThreads / 64; i++) l += s_lost[i];
-    if (l != 0) atomicAdd((unsigned long long*)&a.lost[u & (kLostSlots - 1)], (unsigned long long)l);
+    if (l != 0)
+      atomicAdd((unsigned long long*)&a.lost[(int64_t)a.tile_terrain[u] * kLostSlots + (u & (kLostSlots - 1))],
+                (unsigned long long)l);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Spawn and bin the particles of a cycle by tile (world.hpp:52-62)
 // ---------------------------------------------------------------------------
+// terrain of a global particle index: last k with part0 <= i
+__device__ __forceinline__ int32_t terrain_of(const Terrain* terr, int32_t nterr, int64_t i) {
+  int32_t lo = 0, hi = nterr - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (terr[mid].part0 <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void k_spawn_count(SpawnArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= a.n) return;
+  const Terrain T = a.terr[terrain_of(a.terr, a.nterr, i)];
   Particle w;
-  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, a.spawn_mode, w);
+  spawn(particle_key(cycle_key(T.seed, *a.cycle), (uint64_t)(i - T.part0)), T.nx, T.ny, a.spawn_mode, w);
   // the spawn cell is always on the map: round() of a position in
   // [ix, ix + 0.999] is ix or ix + 1, wrapped (periodic) or, on a bounded map,
   // possibly nx / ny, in which case the particle dies at its first call;
   // bin it by the spawn cell clamped to the map
   int32_t ix = (int32_t)__builtin_roundf(w.px), iy = (int32_t)__builtin_roundf(w.pz);
   if (a.periodic) {
-    if (ix >= a.nx) ix -= a.nx;
-    if (iy >= a.ny) iy -= a.ny;
+    if (ix >= T.nx) ix -= T.nx;
+    if (iy >= T.ny) iy -= T.ny;
   } else {
-    if (ix >= a.nx) ix = a.nx - 1;
-    if (iy >= a.ny) iy = a.ny - 1;
+    if (ix >= T.nx) ix = T.nx - 1;
+    if (iy >= T.ny) iy = T.ny - 1;
   }
-  const int32_t t = (ix / kSX) * a.tiles_y + (iy / kSY);
+  const int32_t t = T.tile0 + (ix / kSX) * T.tiles_y + (iy / kSY);
   a.tile[i] = t;
   a.rank[i] = atomicAdd(&a.count[t], 1u);  // count is window 0's nin
 }
@@ -523,7 +561,7 @@ __global__ __launch_bounds__(256) void k_spawn_count(SpawnArgs a) {
 __global__ __launch_bounds__(256) void k_spawn_scan(SpawnArgs a) {
   // one workgroup: exclusive prefix of the per-tile counts
   __shared__ int64_t s_part[256];
-  const int32_t ntiles = a.tiles_x * a.tiles_y;
+  const int32_t ntiles = a.ntiles;
   const int32_t per = (ntiles + 255) / 256;
   const int32_t t0 = threadIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
   int64_t run = 0;
@@ -552,8 +590,9 @@ __global__ __launch_bounds__(256) void k_spawn_scan(SpawnArgs a) {
 __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= a.n) return;
+  const Terrain T = a.terr[terrain_of(a.terr, a.nterr, i)];
   Particle w;
-  spawn(particle_key(cycle_key(a.seed, *a.cycle), (uint64_t)i), a.nx, a.ny, a.spawn_mode, w);
+  spawn(particle_key(cycle_key(T.seed, *a.cycle), (uint64_t)(i - T.part0)), T.nx, T.ny, a.spawn_mode, w);
   const int64_t o = a.base1[a.tile[i]] + a.rank[i];
   a.out.P0[o] = make_float4(w.px, w.py, w.pz, w.sed);
   a.out.P1[o] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
@@ -568,7 +607,7 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
 // ---------------------------------------------------------------------------
 template <bool P, bool T, int L, bool D>
 static void step4(const StepArgs& a, bool relax_only, hipStream_t s) {
-  const unsigned g = (unsigned)(a.tiles_x * a.tiles_y);
+  const unsigned g = (unsigned)a.ntiles;
   if (relax_only)
     hipLaunchKernelGGL((k_step<P, T, L, D, true>), dim3(g), dim3(kStepThreads), 0, s, a);
   else

@@ -20,6 +20,8 @@ Surface height = fBm (+ pyramids) + sediment thickness: the library's
 height layer is the surface the reference's cell->height describes
 (cellpool.h:228), the sediment layer the loose part on top of bedrock.
 """
+import numpy as np
+
 from .world import World, Wind, generate_terrain, glm_normalize
 
 WORKLOADS = {
@@ -93,3 +95,48 @@ def make_world(name, seed=1, nx=None, ny=None, device=0, **over):
     h, s = make_inputs(name, seed, nx, ny)
     world.upload(h, s)
     return world
+
+
+BATCH_SIDES = (128, 192, 256, 384, 512, 768, 1024)
+
+
+def batch64(seed=1, n=64, sides=BATCH_SIDES, elevation_deg=45.0):
+    """The multi-terrain workload: `n` terrains whose sides are drawn
+    log-uniformly from `sides` (rectangles allowed), each with its own seed
+    and a prevailing wind of uniform azimuth at `elevation_deg` above the
+    horizon: pspeed = (cos e cos a, sin e, cos e sin a); particles per cycle =
+    area.  Returns a list of dicts (nx, ny, seed, pspeed, particles)."""
+    import math
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    out = []
+    e = math.radians(elevation_deg)
+    for k in range(n):
+        nx = int(sides[rng.integers(len(sides))])
+        ny = int(sides[rng.integers(len(sides))])
+        az = float(rng.uniform(0.0, 2.0 * math.pi))
+        ps = glm_normalize((math.cos(e) * math.cos(az), math.sin(e), math.cos(e) * math.sin(az)))
+        out.append(dict(nx=nx, ny=ny, seed=int(rng.integers(1, 2 ** 31)), pspeed=ps, particles=nx * ny))
+    return out
+
+
+def make_batch(terrains, device=0, layers=2, sediment=0.05, **opts):
+    """A Batch loaded with fBm terrains (+ a uniform sediment layer) and
+    the cpu256 workload's rates."""
+    from .world import Batch
+
+    for k, v in WORKLOADS["cpu256"]["params"].items():
+        if k in ("suspension", "gravity", "boundarylayer"):
+            setattr(Wind, k, v)
+        elif k in ("lrate", "maxdiff", "settling"):
+            setattr(World, k, v)
+    b = Batch([(t["nx"], t["ny"]) for t in terrains], device=device, layers=layers,
+              abrasion=WORKLOADS["cpu256"]["params"]["abrasion"], **opts)
+    for k, t in enumerate(terrains):
+        b.set_terrain(k, t["seed"], t["pspeed"])
+        h = generate_terrain(t["nx"], t["ny"], t["seed"])
+        s = np.full((t["nx"], t["ny"]), np.float32(sediment), np.float32) if layers == 2 else None
+        b.upload(k, (h + (s if s is not None else 0)).astype(np.float32), s)
+    return b

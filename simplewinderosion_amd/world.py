@@ -246,3 +246,95 @@ def generate_terrain(nx, ny, seed, octaves=6, base_cycles=4.0, n_pyramids=0, pyr
     if rc != 0:
         raise SweError("swe_generate_terrain failed (%d)" % rc)
     return out.reshape(int(nx), int(ny))
+
+
+class Batch:
+    """Many terrains of different sizes eroded together: one launch per
+    snapshot window covers every terrain (window 1).  Shared parameters come
+    from the Wind / World statics and the options; each terrain has its own
+    seed and prevailing wind."""
+
+    def __init__(self, sizes, device=0, periodic=True, track=True, layers=1, abrasion=0.0,
+                 deposit_on_death=False, mapscale=80.0, spawn=0):
+        self.lib = capi.load()
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        n = len(self.sizes)
+        nx = (ctypes.c_int32 * n)(*[a for a, _ in self.sizes])
+        ny = (ctypes.c_int32 * n)(*[b for _, b in self.sizes])
+        self.opts = dict(abrasion=abrasion, mapscale=mapscale, window=1, periodic=periodic, track=track,
+                         layers=layers, deposit_on_death=deposit_on_death, spawn=spawn)
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.swe_create_batch(ctypes.byref(self.ctx), n, nx, ny, int(device))
+        if rc != 0:
+            raise SweError("swe_create_batch(%d terrains) failed with %d" % (n, rc))
+        self._push()
+
+    _check = World._check
+
+    def _push(self):
+        p = build_params(**self.opts)
+        self._check(self.lib.swe_set_params(self.ctx, ctypes.byref(p)), "swe_set_params")
+
+    def params(self):
+        return build_params(**self.opts)
+
+    def set_terrain(self, k, seed, pspeed=None):
+        ps = None if pspeed is None else (ctypes.c_float * 3)(*[float(v) for v in pspeed])
+        self._check(self.lib.swe_set_terrain(self.ctx, int(k), int(seed) & 0xFFFFFFFFFFFFFFFF, ps),
+                    "swe_set_terrain")
+
+    def upload(self, k, height, sediment=None):
+        nx, ny = self.sizes[k]
+        h = np.ascontiguousarray(height, dtype=np.float32).reshape(nx * ny)
+        s = None if sediment is None else np.ascontiguousarray(sediment, dtype=np.float32).reshape(nx * ny)
+        self._check(self.lib.swe_upload_terrain(self.ctx, int(k), _fp(h), _fp(s)), "swe_upload_terrain")
+
+    def erode(self, particles, n_cycles=1):
+        """particles: per-terrain counts (one World::erode(N_k) each per cycle)."""
+        self._push()
+        arr = (ctypes.c_int64 * len(self.sizes))(*[int(v) for v in particles])
+        self._check(self.lib.swe_erode_batch(self.ctx, arr, int(n_cycles)), "swe_erode_batch")
+
+    def synchronize(self):
+        self._check(self.lib.swe_synchronize(self.ctx), "swe_synchronize")
+
+    def set_timing(self, on):
+        self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")
+
+    def stats(self):
+        s = capi.SweStats()
+        self._check(self.lib.swe_stats(self.ctx, ctypes.byref(s)), "swe_stats")
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def download(self, k):
+        nx, ny = self.sizes[k]
+        n = nx * ny
+        h, s, mf, mo = (np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.float32),
+                        np.empty(3 * n, np.float32))
+        self._check(self.lib.swe_download_terrain(self.ctx, int(k), _fp(h), _fp(s), _fp(mf), _fp(mo)),
+                    "swe_download_terrain")
+        return {"height": h.reshape(nx, ny), "sediment": s.reshape(nx, ny), "massflow": mf.reshape(nx, ny),
+                "momentum": mo.reshape(3, nx, ny)}
+
+    def master(self, k):
+        nx, ny = self.sizes[k]
+        h, s = np.empty(nx * ny, np.int64), np.empty(nx * ny, np.int64)
+        self._check(self.lib.swe_download_terrain_master(self.ctx, int(k), _ip(h), _ip(s)),
+                    "swe_download_terrain_master")
+        return h.reshape(nx, ny), s.reshape(nx, ny)
+
+    def audit(self, k):
+        a = capi.SweAudit()
+        self._check(self.lib.swe_audit_terrain(self.ctx, int(k), ctypes.byref(a)), "swe_audit_terrain")
+        return {"height_sum": a.height_sum, "sediment_sum": a.sediment_sum, "lost": a.lost, "cycle": a.cycle}
+
+    def close(self):
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.swe_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
