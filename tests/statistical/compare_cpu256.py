@@ -17,7 +17,7 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 import numpy as np  # noqa: E402
 
 import oracle as O  # noqa: E402
-from simplewinderosion_amd import stats, workloads  # noqa: E402
+from simplewinderosion_amd import World, stats, workloads  # noqa: E402
 
 
 def main():
@@ -40,7 +40,10 @@ def main():
             steps = O.erode_seq(p, nx, nx, hh, ss, ema, particles=args.particles, n_cycles=args.cycles, seed=seed)
             ts = time.perf_counter() - t0
             runs["seq"].append(hh.reshape(nx, nx))
-            w = workloads.make_world("cpu256", seed=seed, deposit_on_death=bool(deposit))
+            # the same initial map (terrain seed 1) for every run; the seed
+            # only changes the particles' random streams
+            w = workloads.make_world("cpu256", seed=1, deposit_on_death=bool(deposit))
+            World.SEED = seed
             t0 = time.perf_counter()
             w.erode(args.particles, n_cycles=args.cycles)
             w.synchronize()
