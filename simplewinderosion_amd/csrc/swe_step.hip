@@ -92,6 +92,8 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   __shared__ int64_t s_base;
   __shared__ uint32_t s_list[kList];
   __shared__ uint32_t s_nlist;
+  __shared__ int64_t s_sbase[9];
+  __shared__ int64_t s_slen[9];
 
   // XCD-aware order: XCD k (workgroups b with b % 8 == k) takes a contiguous
   // range of tiles, so neighbouring tiles share halo lines in one L2.
@@ -155,21 +157,50 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   // here and summed after the relax phase
   int64_t part = 0;
   for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
-  // gather the indices of this tile's particles into LDS (when they fit)
+  // gather the indices of this tile's particles into LDS (when they fit):
+  // the source segments' bases and lengths are loaded in parallel, then the
+  // concatenated segments are scanned with kScanPer independent code loads
+  // per thread in flight
   const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
   const bool listed = nin_u <= (uint32_t)kList;
   if (!RELAX_ONLY && listed) {
     const int32_t nsrc = s_nsrc;
-    for (int si = 0; si < nsrc; si++) {
-      const int32_t src = s_src[si];
-      const uint32_t mask = s_srcmask[si];
-      const int64_t b0 = a.base_prev[src];
-      const int64_t n = a.cnt_prev[src];
-      for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
-        const uint32_t code = a.in.CODE[j];
-        if (code < 9 && ((mask >> code) & 1u)) {
+    if (threadIdx.x < (unsigned)nsrc) {
+      s_sbase[threadIdx.x] = a.base_prev[s_src[threadIdx.x]];
+      s_slen[threadIdx.x] = a.cnt_prev[s_src[threadIdx.x]];
+    }
+    __syncthreads();
+    int64_t pre[9];
+    int64_t total = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      pre[q] = total;
+      if (q < nsrc) total += s_slen[q];
+    }
+    constexpr int kScanPer = 16;
+    for (int64_t base = 0; base < total; base += (int64_t)kStepThreads * kScanPer) {
+      uint32_t code[kScanPer], mask[kScanPer];
+      int64_t idx[kScanPer];
+#pragma unroll
+      for (int q = 0; q < kScanPer; q++) {
+        const int64_t f = base + (int64_t)q * kStepThreads + threadIdx.x;
+        code[q] = 255;
+        mask[q] = 0;
+        idx[q] = 0;
+        if (f < total) {
+          int si = 0;
+#pragma unroll
+          for (int r = 1; r < 9; r++) si = (r < nsrc && f >= pre[r]) ? r : si;
+          idx[q] = s_sbase[si] + (f - pre[si]);
+          mask[q] = s_srcmask[si];
+          code[q] = a.in.CODE[idx[q]];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kScanPer; q++) {
+        if (code[q] < 9 && ((mask[q] >> code[q]) & 1u)) {
           const uint32_t pos = atomicAdd(&s_nlist, 1u);
-          if (pos < (uint32_t)kList) s_list[pos] = (uint32_t)j;  // pos < nin_u <= kList
+          if (pos < (uint32_t)kList) s_list[pos] = (uint32_t)idx[q];  // pos < nin_u <= kList
         }
       }
     }
