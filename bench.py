@@ -2,14 +2,18 @@
 
 A step is one map-cycle -- World::erode(N) (world.hpp:36-77) -- of the
 1024x1024 workload: periodic two-layer map (fBm + 8 pyramids + 0.05
-sediment), 1,000,000 particles per cycle, 16-call snapshot windows.  Inputs
-are resident in HBM before the timed region; the timed region is K whole
-cycles bracketed by device synchronisation.
+sediment), 1,000,000 particles per cycle, step-synchronous windows (one
+fly() call per particle per window, 514 windows + a closing relax per
+cycle), flow tracking on.  Inputs are resident in HBM before the timed
+region; the timed region is K whole cycles bracketed by device
+synchronisation.
 
 Prints one JSON line with value = particle-steps/s, the roofline of the
-particle-march kernel measured with HIP events on the library's stream, and
-the CPU baseline (the reference's sequential algorithm, one core) timed on
-a bounded sample of the same workload.
+tile step kernel (k_step) -- algorithmic bytes per launch over its average
+launch duration, measured with HIP events on the library's stream around
+every 16th launch inside the timed region -- and the CPU baseline (the
+reference's sequential algorithm, one core) timed on a bounded sample of
+the same workload.
 """
 import argparse
 import json
@@ -23,21 +27,38 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def march_bytes_per_step(p, R):
-    """Algorithmic bytes the march kernel moves per particle-step.
+def algorithmic_bytes_per_cycle(p, steps, cells):
+    """Algorithmic HBM bytes of one map-cycle of the plain method.
 
-    5-point float32 stencil read (20 B) + int64 pickup delta (8 B) + uint32
-    cascade-event count (4 B); sediment layer: + float32 sediment read (4 B)
-    + int64 sediment delta (8 B); tracking: + four int64 tracks (32 B);
-    particle state load + store once per window: 2 x 44 B / R.
+    Per particle-step 44 B: the 3x3 float32 snapshot neighbourhood read by
+    the normal and the cascade (36 B) and one int64 pickup delta (8 B).  Per
+    cell per cycle 36 B: read and zero the int64 delta (16 B), read-modify-
+    write the int64 master (16 B), write the float32 snapshot (4 B).  Flow
+    tracking adds 32 B per step (four int64 tracks) and 96 B per cell per
+    cycle.  The sediment layer's own bytes are not counted.
     """
-    b = 20 + 8 + 4
-    if p.layers == 2:
-        b += 4 + 8
-    if p.track:
-        b += 32
-    b += 2 * 44.0 / R
-    return b
+    per_step = 44 + (32 if p.track else 0)
+    per_cell = 36 + (96 if p.track else 0)
+    return per_step * steps + per_cell * cells, per_step, per_cell
+
+
+def measured_traffic(workload, p):
+    """HBM bytes per k_step launch from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's reading), when it
+    covers this configuration."""
+    path = os.path.join(ROOT, "profiles", "k_step_%s_hbm_traffic.txt" % workload)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for ln in f:
+            if ":" in ln and not ln.startswith("#"):
+                k, v = ln.split(":", 1)
+                vals[k.strip()] = v.strip()
+    cfg = "window=%d layers=%d track=%d" % (p.window, p.layers, int(p.track))
+    if vals.get("config") != cfg or "traffic_bytes_per_launch" not in vals:
+        return None
+    return float(vals["traffic_bytes_per_launch"])
 
 
 def cpu_baseline(workload, seed, target_s):
@@ -99,6 +120,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--window", type=int, default=None)
     ap.add_argument("--track", type=int, default=None)
+    ap.add_argument("--tile-order", type=int, default=None,
+                    help="SWE_TUNE_TILE_ORDER for the step kernel (default: the library's)")
+    ap.add_argument("--timing-stride", type=int, default=16,
+                    help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -122,6 +147,9 @@ def main():
         over["track"] = bool(args.track)
     world = workloads.make_world(args.workload, seed=args.seed, **over)
     p = world.params()
+    if args.tile_order is not None:
+        from simplewinderosion_amd import capi
+        world.set_tune(capi.SWE_TUNE_TILE_ORDER, args.tile_order)
     if not p.periodic:
         raise SystemExit("bench counts particle-steps analytically: periodic workloads only")
     N = wl["particles"]
@@ -134,7 +162,7 @@ def main():
           file=sys.stderr, flush=True)
     world.erode(N, n_cycles=args.warmup)
     world.synchronize()
-    world.set_timing(True)
+    world.set_timing(args.timing_stride)
     world.reset_stats()
     world.synchronize()
     t0 = time.perf_counter()
@@ -144,18 +172,28 @@ def main():
     st = world.stats()
     audit = world.audit()
     elapsed = t1 - t0
-    total_steps = steps_per_cycle * args.steps
+    total_steps = st["particle_steps"]  # counted on the device
+    if total_steps != steps_per_cycle * args.steps:
+        raise SystemExit("device step count %d != %d particles x %d steps x %d cycles"
+                         % (total_steps, N, p.max_age + 1, args.steps))
     value = total_steps / elapsed
 
-    # march kernel: algorithmic bytes / HIP-event kernel time
-    bps = march_bytes_per_step(p, R)
-    march_s = st["march_ms"] / 1e3
-    achieved = bps * total_steps / march_s / 1e9 if march_s > 0 else None
+    # step kernel: algorithmic bytes per launch / HIP-event launch time
+    launches_per_cycle = (p.max_age + 2) + 1 if R == 1 else None
+    cyc_bytes, per_step, per_cell = algorithmic_bytes_per_cycle(p, steps_per_cycle, world.nx * world.ny)
+    avg_launch_ms = st["march_ms"] / st["march_launches"] if st["march_launches"] else None
+    achieved = None
+    bytes_per_launch = None
+    if avg_launch_ms and launches_per_cycle:
+        bytes_per_launch = cyc_bytes / launches_per_cycle
+        achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
+    traffic = measured_traffic(args.workload, p)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                "kernel": "k_march", "bytes_per_step": bps,
-                "avg_launch_ms": st["march_ms"] / max(1, st["march_launches"]),
-                "relax_avg_launch_ms": st["relax_ms"] / max(1, st["relax_launches"])}
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": traffic,
+                "kernel": "k_step", "algorithmic_bytes_per_launch": bytes_per_launch,
+                "bytes_per_step": per_step, "bytes_per_cell_cycle": per_cell,
+                "launches_per_cycle": launches_per_cycle, "avg_launch_ms": avg_launch_ms,
+                "timed_launches": st["march_launches"], "timing_stride": args.timing_stride}
     line = {
         "metric": "particle-steps/s",
         "value": value,
@@ -173,9 +211,9 @@ def main():
                    "particles_per_cycle": N, "window": R, "layers": int(p.layers), "track": int(p.track),
                    "periodic": int(p.periodic), "parallelism": "single GPU"},
         "map_cycles_per_s": args.steps / elapsed,
-        "march_ms_total": st["march_ms"],
-        "relax_ms_total": st["relax_ms"],
         "mass_exact": None,
+        "tiles": {"max_particles_per_tile": st["max_tile_particles"],
+                  "crowded_tile_windows": st["crowded_tile_windows"], "far_moves": st["far_particles"]},
         "roofline": roofline,
     }
     line["mass_exact"] = bool(audit["height_sum"] + audit["lost"] == mass0)

@@ -94,6 +94,9 @@ typedef struct swe_stats_t {
   uint64_t march_launches;
   double relax_ms;        /* HIP-event time of the relax kernels (timing on) */
   uint64_t relax_launches;
+  uint64_t max_tile_particles;  /* most particles one step-kernel tile held in a window */
+  uint64_t crowded_tile_windows;/* (tile, window) pairs above the tile's LDS list capacity */
+  uint64_t far_particles;       /* particle moves that skipped more than one tile */
 } swe_stats_t;
 
 /* Reference defaults (wind.hpp:13,23-26; world.hpp:30-32; cellpool.h:184). */
@@ -147,6 +150,18 @@ int swe_synchronize(swe_ctx* ctx);
 int swe_set_path(swe_ctx* ctx, int32_t path);
 
 /*
+ * Launch tuning knobs: they change speed only, never results.  No reference
+ * counterpart.
+ *   SWE_TUNE_TILE_ORDER: order in which the step kernel's workgroups take
+ *     tiles; 0 (default) = each of the 8 XCDs a contiguous range of tiles,
+ *     C in [1, 4096] = runs of C consecutive tiles dealt round-robin over the
+ *     XCDs.
+ */
+enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1 };
+int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);
+int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
+
+/*
  * Batches: many terrains of different sizes in one context, eroded together
  * by one launch per snapshot window (window == 1).  Terrain k's cells, tiles
  * and particles are contiguous ranges of the context's buffers; results of
@@ -187,7 +202,12 @@ int swe_get_size(const swe_ctx* ctx, int32_t* nx, int32_t* ny);
 int swe_save_state(swe_ctx* ctx, const char* path);
 int swe_load_state(swe_ctx* ctx, const char* path);
 
-/* Kernel timing with HIP events on the context stream (off by default). */
+/*
+ * Kernel timing with HIP events on the context stream (off by default).
+ * on = 0: off; on = n >= 1: events around every n-th launch of the march /
+ * relax / step kernels (n > 1 samples the launches, leaving the others
+ * without the events' extra queue packets).
+ */
 int swe_set_timing(swe_ctx* ctx, int32_t on);
 int swe_stats(swe_ctx* ctx, swe_stats_t* out);
 int swe_reset_stats(swe_ctx* ctx);

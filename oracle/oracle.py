@@ -117,7 +117,7 @@ def load():
                                            ctypes.c_float, ctypes.c_float, ctypes.c_int32, _F]),
         "or_erode": (ctypes.c_int, [_PP, ctypes.c_int32, ctypes.c_int32, _I64, _I64, _F, _I64,
                                     ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64, ctypes.c_int32,
-                                    ctypes.c_int32]),
+                                    ctypes.c_int32, _I64]),
         "or_srand": (None, [ctypes.c_uint32]),
         "or_erode_seq": (ctypes.c_int64, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _F, ctypes.c_int64,
                                           ctypes.c_int32]),
@@ -170,12 +170,14 @@ class OracleState:
         self.ema = np.zeros(4 * self.nx * self.ny, np.float32)
         self.lost = ctypes.c_int64(0)
         self.cycle = ctypes.c_uint64(0)
+        self.steps = ctypes.c_int64(0)  # fly() calls that returned true
 
     def erode(self, particles, n_cycles=1, nthreads=0):
         rc = load().or_erode(ctypes.byref(self.p), self.nx, self.ny, _i(self.zm),
                              _i(self.sm) if self.p.layers == 2 else None,
                              _f(self.ema) if self.p.track else None, ctypes.byref(self.lost),
-                             ctypes.byref(self.cycle), int(particles), int(n_cycles), int(nthreads))
+                             ctypes.byref(self.cycle), int(particles), int(n_cycles), int(nthreads),
+                             ctypes.byref(self.steps))
         if rc != 0:
             raise RuntimeError("or_erode failed")
 

@@ -586,12 +586,14 @@ static int64_t or_windows(const or_params* p, int32_t* R)
  *   ema      float[4*nx*ny] massflow, momentum x/y/z EMA fields (track) or NULL
  *   lost     int64          mass carried off by dying particles (in/out)
  *   cycle    uint64         cycle counter feeding the RNG (in/out)
+ *   steps    int64          fly() calls that returned true (in/out) or NULL
  * Returns 0, or -1 on bad arguments / allocation failure.
  */
 int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* sm,
              float* ema, int64_t* lost, uint64_t* cycle, int64_t particles, int32_t n_cycles,
-             int32_t nthreads)
+             int32_t nthreads, int64_t* steps)
 {
+  int64_t nsteps = 0;
   if (nx < 1 || ny < 1 || particles < 0 || n_cycles < 0 || !zm || !lost || !cycle) return -1;
   if (p->layers == 2 && !sm) return -1;
   if (p->track && !ema) return -1;
@@ -640,11 +642,11 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
 
     for (int64_t win = 0; win < nwin; win++) {
 #ifdef _OPENMP
-#pragma omp parallel for schedule(dynamic, 256)
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nsteps)
 #endif
       for (int64_t i = 0; i < particles; i++) {
         or_wind* w = &W[i];
-        for (int32_t s = 0; s < R && w->age >= 0; s++) or_fly(&m, w);
+        for (int32_t s = 0; s < R && w->age >= 0; s++) nsteps += or_fly(&m, w);
       }
       or_relax(p, nx, ny, S, SS, V, D, DS);
       for (int64_t c = 0; c < ncell; c++) {
@@ -670,6 +672,7 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
     }
     *cycle += 1;
   }
+  if (steps) *steps += nsteps;
   free(S); free(SS); free(D); free(DS); free(V); free(T); free(W);
   return 0;
 }

@@ -17,6 +17,8 @@ SWE_ENOMEM = -3
 SWE_EAUDIT = -4
 SWE_ESTATE = -5
 
+SWE_TUNE_TILE_ORDER = 1
+
 
 class SweParams(ctypes.Structure):
     """swe_params (include/swe.h)."""
@@ -87,6 +89,9 @@ class SweStats(ctypes.Structure):
         ("march_launches", ctypes.c_uint64),
         ("relax_ms", ctypes.c_double),
         ("relax_launches", ctypes.c_uint64),
+        ("max_tile_particles", ctypes.c_uint64),
+        ("crowded_tile_windows", ctypes.c_uint64),
+        ("far_particles", ctypes.c_uint64),
     ]
 
 
@@ -117,6 +122,8 @@ SIGNATURES = {
     "swe_save_state": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "swe_load_state": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "swe_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "swe_set_tune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64]),
+    "swe_get_tune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "swe_stats": (ctypes.c_int, [_P, ctypes.POINTER(SweStats)]),
     "swe_reset_stats": (ctypes.c_int, [_P]),
     "swe_normals": (ctypes.c_int, [_P, _FP]),

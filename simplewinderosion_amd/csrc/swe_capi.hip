@@ -32,6 +32,7 @@ struct swe_ctx {
   swe_params p{};
   std::string err;
   int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
+  int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
 
   // map (device); index [g & 1] is current, g counts snapshot windows
   uint64_t g = 0;
@@ -78,11 +79,13 @@ struct swe_ctx {
 
   // counters (device)
   int64_t* lost = nullptr;     // swe::kLostSlots per terrain
+  uint64_t* dstat = nullptr;   // swe::kStatKinds x swe::kStatSlots device statistics
   uint64_t* dcycle = nullptr;  // RNG cycle counter
   uint64_t cycle = 0;          // host mirror
 
   // timing
-  bool timing = false;
+  int32_t timing = 0;     // 0 off, n >= 1: every n-th kernel launch timed
+  uint64_t timing_tick = 0;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   struct Pending {
@@ -200,6 +203,8 @@ int alloc_pbuf(swe_ctx* c, swe::ParticleBuf& b, int64_t n) {
 // buffers of the tile-owned step path for n particles
 int ensure_step(swe_ctx* c, int64_t n) {
   int r;
+  // particle indices and record positions are 32-bit on this path
+  if (n > (int64_t)UINT32_MAX) return fail(c, SWE_EINVAL, "at most 2^32 - 1 particles per cycle on the tile step path");
   if (!c->fD[0]) {
     const int64_t nt = c->ntiles;
     if ((r = dalloc(c, &c->fD[0], c->ncell)) || (r = dalloc(c, &c->fD[1], c->ncell)) ||
@@ -271,6 +276,7 @@ void fill_consts(const swe_ctx* c, float ps[3], float nps[3]) {
 
 int timed_begin(swe_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
   if (!c->timing) return 0;
+  if ((c->timing_tick++ % (uint64_t)c->timing) != 0) return 0;  // not sampled: e0 stays null
   *e0 = take_event(c);
   *e1 = take_event(c);
   if (!*e0 || !*e1) return fail(c, SWE_EHIP, "hipEventCreate failed");
@@ -279,7 +285,7 @@ int timed_begin(swe_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
 }
 
 int timed_end(swe_ctx* c, hipEvent_t e0, hipEvent_t e1, int kind) {
-  if (!c->timing) return 0;
+  if (!c->timing || !e0) return 0;
   HIPCHK(c, hipEventRecord(e1, c->stream));
   c->pending.push_back({e0, e1, kind});
   return 0;
@@ -305,6 +311,7 @@ int erode_global(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin, 
   ma.L = c->L;
   ma.LC = c->LC;
   ma.lost = c->lost;
+  ma.stat = c->dstat;
   ma.cycle = c->dcycle;
   ma.T = c->trk;
   ma.D = c->dz;
@@ -403,6 +410,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   swe::StepArgs sa{};
   sa.trk = c->trk;
   sa.lost = c->lost;
+  sa.stat = c->dstat;
   sa.cycle = c->dcycle;
   sa.terr = c->dterr;
   sa.tile_terrain = c->dtile_terrain;
@@ -416,6 +424,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sa.maxdiff = c->p.maxdiff;
   sa.settling = c->p.settling;
   sa.max_age = c->p.max_age;
+  sa.tile_order = c->tile_order;
 
   const int64_t nt = c->ntiles;
   swe::SpawnArgs sp{};
@@ -487,7 +496,6 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     HIPCHK(c, hipGetLastError());
     c->cycle++;
     c->st.cycles++;
-    c->st.particle_steps += 0;
     if (c->timing && c->pending.size() > 512) {
       if ((r = collect_timing(c))) return r;
     }
@@ -557,6 +565,7 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->sp_tile);
   dfree(c->sp_rank);
   dfree(c->lost);
+  dfree(c->dstat);
   dfree(c->dcycle);
   dfree(c->dterr);
   dfree(c->dtile_terrain);
@@ -619,6 +628,7 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
       (r = dalloc(c, &c->dz, c->ncell)) ||
       (r = dalloc(c, &c->vis[0], c->ncell)) || (r = dalloc(c, &c->vis[1], c->ncell)) ||
       (r = dalloc(c, &c->ema, 4 * c->ncell)) || (r = dalloc(c, &c->lost, (int64_t)swe::kLostSlots * n_terrains)) ||
+      (r = dalloc(c, &c->dstat, (int64_t)swe::kStatKinds * swe::kStatSlots)) ||
       (r = dalloc(c, &c->dcycle, 1)) || (r = dalloc(c, &c->dterr, n_terrains)) ||
       (r = dalloc(c, &c->dtile_terrain, tiles)) || (r = ensure_layers(c))) {
     swe_destroy(c);
@@ -825,6 +835,29 @@ int swe_erode(swe_ctx* c, int64_t particles, int32_t n_cycles) {
   return swe_erode_batch(c, &particles, n_cycles);
 }
 
+int swe_set_tune(swe_ctx* c, int32_t knob, int64_t value) {
+  if (!c) return SWE_EINVAL;
+  switch (knob) {
+    case SWE_TUNE_TILE_ORDER:
+      if (value < 0 || value > 4096) return fail(c, SWE_EINVAL, "tile order must be in [0, 4096]");
+      c->tile_order = (int32_t)value;
+      return 0;
+    default:
+      return fail(c, SWE_EINVAL, "unknown tuning knob");
+  }
+}
+
+int swe_get_tune(swe_ctx* c, int32_t knob, int64_t* value) {
+  if (!c || !value) return SWE_EINVAL;
+  switch (knob) {
+    case SWE_TUNE_TILE_ORDER:
+      *value = c->tile_order;
+      return 0;
+    default:
+      return fail(c, SWE_EINVAL, "unknown tuning knob");
+  }
+}
+
 int swe_set_path(swe_ctx* c, int32_t path) {
   if (!c) return SWE_EINVAL;
   if (path < 0 || path > 2) return fail(c, SWE_EINVAL, "path must be 0 (auto), 1 (global march) or 2 (tile step)");
@@ -911,7 +944,9 @@ int swe_set_timing(swe_ctx* c, int32_t on) {
   if (!c) return SWE_EINVAL;
   int r = collect_timing(c);
   if (r) return r;
-  c->timing = on != 0;
+  if (on < 0) return fail(c, SWE_EINVAL, "timing stride must be >= 0");
+  c->timing = on;
+  c->timing_tick = 0;
   return 0;
 }
 
@@ -920,6 +955,18 @@ int swe_stats(swe_ctx* c, swe_stats_t* out) {
   int r = collect_timing(c);
   if (r) return r;
   *out = c->st;
+  uint64_t h[swe::kStatKinds * swe::kStatSlots];
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(h, c->dstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out->particle_steps = out->max_tile_particles = out->crowded_tile_windows = out->far_particles = 0;
+  for (int q = 0; q < swe::kStatSlots; q++) {
+    out->particle_steps += h[swe::kStatSteps * swe::kStatSlots + q];
+    const uint64_t m = h[swe::kStatMaxTile * swe::kStatSlots + q];
+    if (m > out->max_tile_particles) out->max_tile_particles = m;
+    out->crowded_tile_windows += h[swe::kStatCrowded * swe::kStatSlots + q];
+    out->far_particles += h[swe::kStatFar * swe::kStatSlots + q];
+  }
   return 0;
 }
 
@@ -928,6 +975,9 @@ int swe_reset_stats(swe_ctx* c) {
   int r = collect_timing(c);
   if (r) return r;
   std::memset(&c->st, 0, sizeof(c->st));
+  HIPCHK(c, hipSetDevice(c->device));
+  swe::launch_zero64((int64_t*)c->dstat, (int64_t)swe::kStatKinds * swe::kStatSlots, c->stream);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 

@@ -20,39 +20,49 @@ __device__ __forceinline__ float settle_factor_dev(float settling, uint32_t even
   return 1.0f - r;
 }
 
+// Neighbour offsets of World::cascade (world.hpp:83-92), k = 0..7:
+// (-1,-1) (-1,0) (-1,1) (0,-1) (0,1) (1,-1) (1,0) (1,1), packed two bits per
+// k as offset + 1; bit k of kDiagMask marks the diagonals (distance sqrt 2).
+constexpr uint32_t kOffXPacked = 0xa940u, kOffYPacked = 0x9224u, kDiagMask = 0xa5u;
+__device__ __forceinline__ int32_t off_x(int32_t k) { return (int32_t)((kOffXPacked >> (2 * k)) & 3u) - 1; }
+__device__ __forceinline__ int32_t off_y(int32_t k) { return (int32_t)((kOffYPacked >> (2 * k)) & 3u) - 1; }
+
+// Sort key of a neighbour: descending order of the key is descending height,
+// ties in ascending offset order -- the order the stable insertion sort of
+// world.hpp:118-120 leaves eight elements in.  The height bits are mapped to
+// an unsigned integer of the same order (-0 first folded into +0, which the
+// float comparison treats as equal); map heights are finite.  Out-of-map
+// neighbours get key 0, below every in-map key.
+__device__ __forceinline__ uint64_t cascade_key(float h, int32_t k, bool valid) {
+  const uint32_t u = f2u(h + 0.0f);
+  const uint32_t o = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return valid ? (((uint64_t)o << 32) | (uint32_t)(7 - k)) : 0ull;
+}
+__device__ __forceinline__ float cascade_key_height(uint64_t key) {
+  const uint32_t o = (uint32_t)(key >> 32);
+  return u2f((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
 // One cell's cascade of the window: World::cascade (world.hpp:79-153)
-// against the snapshot with the window's settling factor.  hn[k] / sn[k]:
-// neighbour k's height / sediment in the fixed offset order
-// (world.hpp:83-92); hc / sc: the centre's; bit k of validmask set for
-// in-map neighbours.  The neighbours are sorted by height descending, ties
-// in offset order (what the stable insertion sort std::sort uses on eight
-// elements does), by a sorting network on (height, offset) keys; the walk
-// then evolves the centre.  amt[k] = fixed-point amount neighbour k receives.
-template <int LAYERS>
-__device__ __forceinline__ void cascade_cell_dev(const float hn[8], const float sn[8], float hc, float sc,
-                                             uint32_t validmask, float settling, float maxdiff, int64_t amt[8]) {
-  float key[8], sed[8];
-  int32_t idx[8];
+// against the snapshot with the window's settling factor.  hn[k]: neighbour
+// k's height in the fixed offset order; sed_of(k): its sediment (two-layer
+// maps only); hc / sc: the centre's; bit k of validmask set for in-map
+// neighbours.  The neighbours are sorted by a sorting network on their keys
+// and walked in that order with the centre evolving; emit(k, amount) is
+// called with the fixed-point amount neighbour k receives (the centre loses
+// the same amount) for every transfer.
+template <int LAYERS, class SedOf, class Emit>
+__device__ __forceinline__ void cascade_cell_dev(const float hn[8], const SedOf& sed_of, float hc, float sc,
+                                                 uint32_t validmask, float settling, float maxdiff, Emit&& emit) {
+  uint64_t key[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    key[k] = ((validmask >> k) & 1u) ? hn[k] : -__builtin_inff();
-    sed[k] = LAYERS == 2 ? sn[k] : 0.0f;
-    idx[k] = k;
-  }
-#define SWE_CAS(i, j)                                                                  \
-  {                                                                                    \
-    const bool sw = key[j] > key[i] || (key[j] == key[i] && idx[j] < idx[i]);          \
-    const float tk = sw ? key[j] : key[i];                                             \
-    const int32_t ti = sw ? idx[j] : idx[i];                                           \
-    key[j] = sw ? key[i] : key[j];                                                     \
-    idx[j] = sw ? idx[i] : idx[j];                                                     \
-    key[i] = tk;                                                                       \
-    idx[i] = ti;                                                                       \
-    if (LAYERS == 2) {                                                                 \
-      const float ts = sw ? sed[j] : sed[i];                                           \
-      sed[j] = sw ? sed[i] : sed[j];                                                   \
-      sed[i] = ts;                                                                     \
-    }                                                                                  \
+  for (int k = 0; k < 8; k++) key[k] = cascade_key(hn[k], k, (validmask >> k) & 1u);
+#define SWE_CAS(i, j)                        \
+  {                                          \
+    const uint64_t a = key[i], b = key[j];   \
+    const bool sw = b > a;                   \
+    key[i] = sw ? b : a;                     \
+    key[j] = sw ? a : b;                     \
   }
   SWE_CAS(0, 2) SWE_CAS(1, 3) SWE_CAS(4, 6) SWE_CAS(5, 7)
   SWE_CAS(0, 4) SWE_CAS(1, 5) SWE_CAS(2, 6) SWE_CAS(3, 7)
@@ -62,41 +72,31 @@ __device__ __forceinline__ void cascade_cell_dev(const float hn[8], const float 
   SWE_CAS(1, 2) SWE_CAS(3, 4) SWE_CAS(5, 6)
 #undef SWE_CAS
   const float dd = __builtin_sqrtf(2.0f);
-  int64_t got[8];  // in sorted slot order
 #pragma unroll
   for (int r = 0; r < 8; r++) {
-    got[r] = 0;
-    const int32_t k = idx[r];
-    if (!((validmask >> k) & 1u)) continue;
-    const float diff = hc - key[r];
-    if (diff == 0.0f) continue;
-    const float d = (k == 0 || k == 2 || k == 5 || k == 7) ? dd : 1.0f;
+    if (key[r] == 0ull) break;  // the out-of-map neighbours come last
+    const int32_t k = 7 - (int32_t)(key[r] & 7u);
+    const float diff = hc - cascade_key_height(key[r]);
+    const float d = ((kDiagMask >> k) & 1u) ? dd : 1.0f;
     const float excess = __builtin_fabsf(diff) - (d * maxdiff) * 1.0f;
-    if (excess <= 0.0f) continue;
     float t = settling * excess / 2.0f;
-    if (LAYERS == 2) {
-      const float avail = diff > 0.0f ? sc : sed[r];
+    bool go = diff != 0.0f && !(excess <= 0.0f);
+    if (LAYERS == 2 && go) {
+      const float avail = diff > 0.0f ? sc : sed_of(k);
       if (t > avail) t = avail;
-      if (!(t > 0.0f)) continue;
+      go = t > 0.0f;
     }
+    if (!go) continue;
     const int64_t tf = fx_from_f32(t);
     if (diff > 0.0f) {
       hc -= t;
       sc -= t;
-      got[r] = tf;
+      emit(k, tf);
     } else {
       hc += t;
       sc += t;
-      got[r] = -tf;
+      emit(k, -tf);
     }
-  }
-  // back to offset order
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    int64_t v = 0;
-#pragma unroll
-    for (int r = 0; r < 8; r++) v = idx[r] == k ? got[r] : v;
-    amt[k] = v;
   }
 }
 

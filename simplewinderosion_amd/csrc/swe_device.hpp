@@ -21,8 +21,9 @@ __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 
 // round_half_even(v * 2^32), computed on the IEEE encoding. NaN -> 0,
-// overflow saturates.
-__device__ __forceinline__ int64_t fx_from_f32(float v) {
+// overflow saturates.  The general (slow) form; fx_from_f32 below takes it
+// only for |v| >= 2^31 and NaN.
+__device__ __forceinline__ int64_t fx_from_f32_bits(float v) {
   const uint32_t u = f2u(v);
   const uint32_t e = (u >> 23) & 0xffu;
   const uint64_t sgn = u >> 31;
@@ -49,8 +50,22 @@ __device__ __forceinline__ int64_t fx_from_f32(float v) {
   return sgn ? -(int64_t)r : (int64_t)r;
 }
 
-// round_half_even(m * 2^-32) to float32, by integer operations.
-__device__ __forceinline__ float f32_from_fx(int64_t m) {
+// Same result for every input: for |v| < 2^31, v * 2^32 is exact and below
+// 2^63, rint rounds it half-to-even to an integral float, and the integral
+// float converts to int64 exactly (denormal v give 0 either way).
+__device__ __forceinline__ int64_t fx_from_f32(float v) {
+  if (__builtin_expect(__builtin_fabsf(v) < 2147483648.0f, 1))
+    return (int64_t)__builtin_rintf(v * 4294967296.0f);
+  return fx_from_f32_bits(v);
+}
+
+// round_half_even(m * 2^-32) to float32: the int64 -> float32 conversion
+// rounds to nearest even, and the power-of-two scale is exact (|m| >= 1
+// keeps the result far from the denormal range).
+__device__ __forceinline__ float f32_from_fx(int64_t m) { return (float)m * 2.3283064365386963e-10f; }
+
+// The same by integer operations (reference form of f32_from_fx).
+__device__ __forceinline__ float f32_from_fx_bits(int64_t m) {
   if (m == 0) return 0.0f;
   const bool neg = m < 0;
   const uint64_t a = neg ? (uint64_t)0 - (uint64_t)m : (uint64_t)m;

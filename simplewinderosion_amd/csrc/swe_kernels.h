@@ -10,6 +10,11 @@ namespace swe {
 
 // carried-off mass is spread over this many counters to avoid one hot address
 constexpr int kLostSlots = 256;
+// device statistics: kStatKinds counters of kStatSlots slots each (spread
+// over slots to keep the atomics off one address); summed (max for
+// kStatMaxTile) by the host
+constexpr int kStatSlots = 64;
+enum StatKind { kStatSteps = 0, kStatMaxTile = 1, kStatCrowded = 2, kStatFar = 3, kStatKinds = 4 };
 
 struct MarchArgs {
   const float* S;      // height snapshot of the window
@@ -23,6 +28,7 @@ struct MarchArgs {
   int64_t* L;          // fixed-point load
   int32_t* LC;         // last pre-move cell
   int64_t* lost;       // kLostSlots counters
+  uint64_t* stat;      // kStatKinds x kStatSlots
   const uint64_t* cycle;
   int64_t n;           // particles
   int64_t ncell;

@@ -71,6 +71,7 @@ template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool FIRST>
 __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   int64_t lost_here = 0;
+  uint32_t steps = 0;
   if (i < a.n) {
     FlyConsts k;
     k.nx = a.nx;
@@ -108,6 +109,8 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
         if (fly<PERIODIC, TRACK, LAYERS>(w, pk, cx, cy, k, m, acc) == kDied) {
           die = true;
           dead_cell = (int64_t)cx * a.ny + cy;
+        } else {
+          steps++;
         }
       }
       if (die) {
@@ -135,6 +138,11 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
       const uint32_t slot = (blockIdx.x * 4 + (threadIdx.x >> 6)) & (kLostSlots - 1);
       atomicAdd((unsigned long long*)&a.lost[slot], (unsigned long long)lost_here);
     }
+  }
+  for (int off = 32; off > 0; off >>= 1) steps += __shfl_xor(steps, off, 64);
+  if ((threadIdx.x & 63) == 0 && steps != 0) {
+    const uint32_t slot = (blockIdx.x * 4 + (threadIdx.x >> 6)) & (kStatSlots - 1);
+    atomicAdd((unsigned long long*)&a.stat[kStatSteps * kStatSlots + slot], (unsigned long long)steps);
   }
 }
 
@@ -219,26 +227,25 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
     }
     const int32_t lx = e / kRY, ly = e % kRY;  // ring coords; snapshot coords +1
     const int32_t gx = x0 + lx - 1, gy = y0 + ly - 1;
-    float hn[8], sn[8];
+    float hn[8];
     uint32_t valid = 0;
+    const int32_t ic = (lx + 1) * kHY + ly + 1;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const int32_t hx = lx + 1 + c_offx[k], hy = ly + 1 + c_offy[k];
-      hn[k] = sh[hx * kHY + hy];
-      sn[k] = LAYERS == 2 ? shs[hx * kHY + hy] : 0.0f;
+      hn[k] = sh[ic + c_offx[k] * kHY + c_offy[k]];
       bool ok = true;
       if (!PERIODIC) {
         const int32_t qx = gx + c_offx[k], qy = gy + c_offy[k];
         ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
       }
       valid |= (ok ? 1u : 0u) << k;
+      out[k] = 0;
     }
-    const float hc = sh[(lx + 1) * kHY + ly + 1];
-    const float sc = LAYERS == 2 ? shs[(lx + 1) * kHY + ly + 1] : 0.0f;
-    int64_t amt[8];
-    cascade_cell_dev<LAYERS>(hn, sn, hc, sc, valid, settle_factor_dev(a.settling, v), a.maxdiff, amt);
-#pragma unroll
-    for (int k = 0; k < 8; k++) out[k] = amt[k];
+    const float hc = sh[ic];
+    const float sc = LAYERS == 2 ? shs[ic] : 0.0f;
+    const auto sed_of = [&](int32_t k) { return shs[ic + off_x(k) * kHY + off_y(k)]; };
+    cascade_cell_dev<LAYERS>(hn, sed_of, hc, sc, valid, settle_factor_dev(a.settling, v), a.maxdiff,
+                             [&](int32_t k, int64_t amt) { out[k] = amt; });
   }
   __syncthreads();
 

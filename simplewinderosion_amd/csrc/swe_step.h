@@ -64,6 +64,7 @@ struct StepArgs {
   uint32_t* v_out;
   int64_t* trk;          // 4 x ncell tracks of the cycle
   int64_t* lost;         // kLostSlots counters
+  uint64_t* stat;        // kStatKinds x kStatSlots statistics
   // particles
   ParticleBuf in, out, far_in, far_out;
   uint32_t* far_cnt_in;
@@ -83,6 +84,9 @@ struct StepArgs {
   int32_t ntiles;
   float mapscale, boundarylayer, suspension, gravity, abrasion, maxdiff, settling;
   int32_t max_age;
+  // workgroup -> tile order: 0 = each XCD a contiguous range of tiles;
+  // C >= 1 = runs of C consecutive tiles dealt round-robin over the XCDs
+  int32_t tile_order;
 };
 
 struct SpawnArgs {

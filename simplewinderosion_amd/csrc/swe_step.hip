@@ -74,8 +74,9 @@ __device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
   return v;
 }
 
+// four workgroups per CU fit the LDS; at most 128 VGPRs keep four waves per SIMD
 template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool RELAX_ONLY>
-__global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
+__global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_step(StepArgs a) {
   __shared__ float s_new[kH1X * kH1Y];
   __shared__ float s_snew[LAYERS == 2 ? kH1X * kH1Y : 1];
   // relax-phase and march-phase working sets share one buffer
@@ -95,11 +96,21 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   __shared__ int64_t s_sbase[9];
   __shared__ int64_t s_slen[9];
 
-  // XCD-aware order: XCD k (workgroups b with b % 8 == k) takes a contiguous
-  // range of tiles, so neighbouring tiles share halo lines in one L2.
+  // XCD-aware order (workgroup b runs on XCD b % 8).  tile_order 0: XCD k
+  // takes a contiguous range of tiles, so neighbouring tiles share halo
+  // lines in one L2.  tile_order C >= 1: runs of C consecutive tiles are
+  // dealt round-robin over the XCDs (evens out crowded regions); the last
+  // nb mod 8C workgroups keep their own index.  Both are bijections.
   const uint32_t nb = gridDim.x, b = blockIdx.x;
-  const uint32_t q8 = nb / 8, r8 = nb % 8, xcd = b % 8;
-  const int32_t u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);  // global tile
+  const uint32_t xcd = b % 8;
+  int32_t u;  // global tile
+  if (a.tile_order <= 0) {
+    const uint32_t q8 = nb / 8, r8 = nb % 8;
+    u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);
+  } else {
+    const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C), j = b / 8;
+    u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
+  }
   // the terrain this tile belongs to; all map indices below are terrain-local
   const Terrain T = a.terr[a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
@@ -125,6 +136,22 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   int64_t* ds_out = LAYERS == 2 ? a.ds_out + c0 : nullptr;
   uint32_t* v_out = a.v_out + c0;
   int64_t* trk = TRACK ? a.trk + c0 : nullptr;
+
+  // flow tracks of the tile's cells, loaded now and updated at the end (the
+  // loads complete while the relax and particle phases run)
+  constexpr int kCellPer = (kCell + kStepThreads - 1) / kStepThreads;
+  int64_t tpre[TRACK && !RELAX_ONLY ? kCellPer : 1][4];
+  if (TRACK && !RELAX_ONLY) {
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
+      const bool in = e < kCell && gx < nx && gy < ny;
+      const int64_t g = (int64_t)gx * ny + gy;
+#pragma unroll
+      for (int f = 0; f < 4; f++) tpre[i][f] = in ? trk[f * a.ncell + g] : 0;
+    }
+  }
 
   // distinct source tiles and, for each, the direction codes that lead here
   if (threadIdx.x == 0) {
@@ -157,19 +184,22 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   // here and summed after the relax phase
   int64_t part = 0;
   for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
-  // gather the indices of this tile's particles into LDS (when they fit):
-  // the source segments' bases and lengths are loaded in parallel, then the
-  // concatenated segments are scanned with kScanPer independent code loads
-  // per thread in flight
+  // gather the indices of this tile's particles into LDS: the source
+  // segments' bases and lengths are loaded in parallel, then a flat range
+  // [f0, f1) of the concatenated segments is scanned with kScanPer
+  // independent code loads per thread in flight and the records that belong
+  // here are appended to s_list.  A tile holding more than kList particles
+  // (crowded) takes its sources in flat ranges of kList records, one list
+  // per range.
   const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
-  const bool listed = nin_u <= (uint32_t)kList;
-  if (!RELAX_ONLY && listed) {
+  const bool crowded = nin_u > (uint32_t)kList;
+  auto flat_total = [&]() __attribute__((always_inline)) {
+    int64_t total = 0;
+    for (int q = 0; q < s_nsrc; q++) total += s_slen[q];
+    return total;
+  };
+  auto gather = [&](int64_t f0, int64_t f1) __attribute__((always_inline)) {
     const int32_t nsrc = s_nsrc;
-    if (threadIdx.x < (unsigned)nsrc) {
-      s_sbase[threadIdx.x] = a.base_prev[s_src[threadIdx.x]];
-      s_slen[threadIdx.x] = a.cnt_prev[s_src[threadIdx.x]];
-    }
-    __syncthreads();
     int64_t pre[9];
     int64_t total = 0;
 #pragma unroll
@@ -177,21 +207,22 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       pre[q] = total;
       if (q < nsrc) total += s_slen[q];
     }
+    if (f1 > total) f1 = total;
     constexpr int kScanPer = 16;
-    for (int64_t base = 0; base < total; base += (int64_t)kStepThreads * kScanPer) {
+    for (int64_t base = f0; base < f1; base += (int64_t)kStepThreads * kScanPer) {
       uint32_t code[kScanPer], mask[kScanPer];
-      int64_t idx[kScanPer];
+      uint32_t idx[kScanPer];  // particle buffers hold fewer than 2^32 records
 #pragma unroll
       for (int q = 0; q < kScanPer; q++) {
         const int64_t f = base + (int64_t)q * kStepThreads + threadIdx.x;
         code[q] = 255;
         mask[q] = 0;
         idx[q] = 0;
-        if (f < total) {
+        if (f < f1) {
           int si = 0;
 #pragma unroll
           for (int r = 1; r < 9; r++) si = (r < nsrc && f >= pre[r]) ? r : si;
-          idx[q] = s_sbase[si] + (f - pre[si]);
+          idx[q] = (uint32_t)(s_sbase[si] + (f - pre[si]));
           mask[q] = s_srcmask[si];
           code[q] = a.in.CODE[idx[q]];
         }
@@ -200,10 +231,21 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       for (int q = 0; q < kScanPer; q++) {
         if (code[q] < 9 && ((mask[q] >> code[q]) & 1u)) {
           const uint32_t pos = atomicAdd(&s_nlist, 1u);
-          if (pos < (uint32_t)kList) s_list[pos] = (uint32_t)idx[q];  // pos < nin_u <= kList
+          // pos < kList: a full range holds at most nin_u <= kList records of
+          // this tile, a crowded tile's range at most kList records
+          if (pos < (uint32_t)kList) s_list[pos] = idx[q];
         }
       }
     }
+  };
+  if (!RELAX_ONLY) {
+    const int32_t nsrc = s_nsrc;
+    if (threadIdx.x < (unsigned)nsrc) {
+      s_sbase[threadIdx.x] = a.base_prev[s_src[threadIdx.x]];
+      s_slen[threadIdx.x] = a.cnt_prev[s_src[threadIdx.x]];
+    }
+    __syncthreads();
+    gather(0, crowded ? (int64_t)kList : INT64_MAX);
   }
 
   // ---------------- 1. relax of the previous window ----------------
@@ -212,6 +254,25 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     float* s_sold = s_old + kH3X * kH3Y;  // valid when LAYERS == 2
     uint32_t* s_vold = (uint32_t*)(s_old + kH3X * kH3Y * (LAYERS == 2 ? 2 : 1));
     int64_t* s_casc = (int64_t*)(s_vold + kH2X * kH2Y);
+    // master + pickup delta of the tile and its halo, loaded before the
+    // cascades are computed
+    constexpr int kH1Per = (kH1X * kH1Y + kStepThreads - 1) / kStepThreads;
+    int64_t zpre[kH1Per], spre[LAYERS == 2 ? kH1Per : 1];
+#pragma unroll
+    for (int i = 0; i < kH1Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      int32_t gx = x0 - 1 + e / kH1Y, gy = y0 - 1 + e % kH1Y;
+      bool ok = e < kH1X * kH1Y;
+      if (PERIODIC) {
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
+      } else {
+        ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+      }
+      const int64_t g = (int64_t)gx * ny + gy;
+      zpre[i] = ok ? zm_in[g] + d_in[g] : 0;
+      if (LAYERS == 2) spre[i] = ok ? sm_in[g] + ds_in[g] : 0;
+    }
     for (int e = threadIdx.x; e < kH3X * kH3Y; e += kStepThreads) {
       int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
       float v = 0.0f, vs = 0.0f;
@@ -252,13 +313,12 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       if (v == 0) continue;
       const int32_t lx = e / kH2Y, ly = e % kH2Y;  // H2 coords; H3 = +1, H1 = -1
       const int32_t gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-      float hn[8], sn[8];
+      float hn[8];
       uint32_t valid = 0;
+      const int32_t ic = (lx + 1) * kH3Y + (ly + 1);
 #pragma unroll
       for (int k = 0; k < 8; k++) {
-        const int32_t i3 = (lx + 1 + c_ox[k]) * kH3Y + (ly + 1 + c_oy[k]);
-        hn[k] = s_old[i3];
-        sn[k] = LAYERS == 2 ? s_sold[i3] : 0.0f;
+        hn[k] = s_old[ic + c_ox[k] * kH3Y + c_oy[k]];
         bool ok = true;
         if (!PERIODIC) {
           const int32_t qx = gx + c_ox[k], qy = gy + c_oy[k];
@@ -266,19 +326,15 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
         }
         valid |= (ok ? 1u : 0u) << k;
       }
-      const int32_t ic = (lx + 1) * kH3Y + (ly + 1);
-      int64_t amt[8];
-      cascade_cell_dev<LAYERS>(hn, sn, s_old[ic], LAYERS == 2 ? s_sold[ic] : 0.0f, valid,
-                               settle_factor_dev(a.settling, v), a.maxdiff, amt);
+      const auto sed_of = [&](int32_t k) { return s_sold[ic + off_x(k) * kH3Y + off_y(k)]; };
       int64_t centre = 0;
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        if (amt[k] == 0) continue;
-        centre -= amt[k];
-        const int32_t hx = lx - 1 + c_ox[k], hy = ly - 1 + c_oy[k];
-        if (hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
-          atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)amt[k]);
-      }
+      cascade_cell_dev<LAYERS>(hn, sed_of, s_old[ic], LAYERS == 2 ? s_sold[ic] : 0.0f, valid,
+                               settle_factor_dev(a.settling, v), a.maxdiff, [&](int32_t k, int64_t amt) {
+                                 centre -= amt;
+                                 const int32_t hx = lx - 1 + off_x(k), hy = ly - 1 + off_y(k);
+                                 if (amt != 0 && hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
+                                   atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)amt);
+                               });
       const int32_t hx = lx - 1, hy = ly - 1;
       if (centre != 0 && hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
         atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)centre);
@@ -286,7 +342,10 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     __syncthreads();
 
     // master += pickups + cascade; next snapshot for the tile and its halo
-    for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) {
+#pragma unroll
+    for (int i = 0; i < kH1Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      if (e >= kH1X * kH1Y) continue;
       const int32_t lx = e / kH1Y, ly = e % kH1Y;
       int32_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
       bool ok = true;
@@ -303,7 +362,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       }
       const int64_t g = (int64_t)gx * ny + gy;
       const int64_t cas = s_casc[e];
-      const int64_t z = zm_in[g] + d_in[g] + cas;
+      const int64_t z = zpre[i] + cas;
       const float zf = f32_from_fx(z);
       s_new[e] = zf;
       const bool mine = lx >= 1 && ly >= 1 && lx <= kSX && ly <= kSY && x0 - 1 + lx < nx && y0 - 1 + ly < ny;
@@ -312,7 +371,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
         snap_out[g] = zf;
       }
       if (LAYERS == 2) {
-        int64_t s = sm_in[g] + ds_in[g] + cas;
+        int64_t s = spre[i] + cas;
         if (s < 0) s = 0;
         const float sf = f32_from_fx(s);
         s_snew[e] = sf;
@@ -355,6 +414,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   if (threadIdx.x < 9) s_slot[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_out = 0;
   int64_t lost_here = 0;
+  uint32_t steps_here = 0, far_here = 0;
   __syncthreads();
 
   if (!RELAX_ONLY) {
@@ -374,20 +434,35 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
     const uint64_t ck = cycle_key(T.seed, *a.cycle);
     const int64_t p0 = T.part0;
     const int64_t out_base = s_base;
-    const int32_t nsrc = s_nsrc;
     const uint32_t far_n = *a.far_cnt_in;
 
-    // process one input record; returns nothing, writes outputs
-    auto process = [&](const ParticleBuf& in, int64_t j) {
+    // one input record
+    struct Rec {
+      float4 q0, q1;
+      int64_t ledger;
+      int32_t lastc;
+      uint32_t pid;
+    };
+    auto load = [](const ParticleBuf& in, int64_t j) __attribute__((always_inline)) {
+      Rec r;
+      r.q0 = in.P0[j];
+      r.q1 = in.P1[j];
+      r.ledger = in.L[j];
+      r.lastc = in.LC[j];
+      r.pid = in.PID[j];
+      return r;
+    };
+    // step one record and write it out
+    auto process = [&](const Rec& r) __attribute__((always_inline)) {
       Particle w;
-      const float4 q0 = in.P0[j];
-      const float4 q1 = in.P1[j];
+      const float4 q0 = r.q0;
+      const float4 q1 = r.q1;
       w.px = q0.x; w.py = q0.y; w.pz = q0.z; w.sed = q0.w;
       w.sx = q1.x; w.sy = q1.y; w.sz = q1.z;
       w.age = (int32_t)f2u(q1.w);
-      w.ledger = in.L[j];
-      w.lastc = in.LC[j];
-      const uint32_t pid = in.PID[j];
+      w.ledger = r.ledger;
+      w.lastc = r.lastc;
+      const uint32_t pid = r.pid;
       const uint64_t pk = particle_key(ck, (uint64_t)((int64_t)pid - p0));
       int32_t cx, cy;
       const bool ok = cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);
@@ -428,6 +503,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
         }
         return;
       }
+      steps_here++;
       // route by the cell the particle now occupies; one that leaves a
       // bounded map (or whose position stops being finite) stays with the
       // tile of its last cell and dies there at its next call
@@ -450,6 +526,7 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       const float4 o0 = make_float4(w.px, w.py, w.pz, w.sed);
       const float4 o1 = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
       if (code == 255) {
+        far_here++;
         const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
         a.far_out.P0[fslot] = o0;
         a.far_out.P1[fslot] = o1;
@@ -469,20 +546,29 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
       a.out.CODE[o] = (uint8_t)code;
     };
 
-    if (listed) {
+    // step the listed records; the next record is loaded while the current
+    // one is stepped
+    auto step_list = [&]() __attribute__((always_inline)) {
       const uint32_t nl = s_nlist < (uint32_t)kList ? s_nlist : (uint32_t)kList;
-      for (uint32_t e = threadIdx.x; e < nl; e += kStepThreads) process(a.in, (int64_t)s_list[e]);
-    } else {
-      // a crowded tile: scan the sources again and step in place
-      for (int si = 0; si < nsrc; si++) {
-        const int32_t src = s_src[si];
-        const uint32_t mask = s_srcmask[si];
-        const int64_t b0 = a.base_prev[src];
-        const int64_t n = a.cnt_prev[src];
-        for (int64_t j = b0 + threadIdx.x; j < b0 + n; j += kStepThreads) {
-          const uint32_t code = a.in.CODE[j];
-          if (code < 9 && ((mask >> code) & 1u)) process(a.in, j);
-        }
+      Rec cur{};
+      if (threadIdx.x < nl) cur = load(a.in, (int64_t)s_list[threadIdx.x]);
+      for (uint32_t e = threadIdx.x; e < nl; e += kStepThreads) {
+        Rec nxt{};
+        if (e + kStepThreads < nl) nxt = load(a.in, (int64_t)s_list[e + kStepThreads]);
+        process(cur);
+        cur = nxt;
+      }
+    };
+    step_list();
+    if (crowded) {
+      const int64_t total = flat_total();
+      for (int64_t f0 = kList; f0 < total; f0 += kList) {
+        __syncthreads();  // the previous list is consumed
+        if (threadIdx.x == 0) s_nlist = 0;
+        __syncthreads();
+        gather(f0, f0 + kList);
+        __syncthreads();
+        step_list();
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
@@ -499,24 +585,26 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
         ctx = (lc / ny) / kSX;
         cty = (lc % ny) / kSY;
       }
-      if (ctx == tx && cty == ty) process(a.far_in, j);
+      if (ctx == tx && cty == ty) process(load(a.far_in, j));
     }
   }
   __syncthreads();
 
   // ---------------- 3. write the tile's window results ----------------
-  for (int e = threadIdx.x; e < kCell; e += kStepThreads) {
+#pragma unroll
+  for (int i = 0; i < kCellPer; i++) {
+    const int e = threadIdx.x + i * kStepThreads;
     const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
-    if (gx >= nx || gy >= ny) continue;
+    if (e >= kCell || gx >= nx || gy >= ny) continue;
     const int64_t g = (int64_t)gx * ny + gy;
     d_out[g] = acc_d[e];
     v_out[g] = acc_v[e];
     if (LAYERS == 2) ds_out[g] = acc_ds[e];
-    if (TRACK) {
+    if (TRACK && !RELAX_ONLY) {
 #pragma unroll
       for (int f = 0; f < 4; f++) {
         const int64_t t = acc_t[f * kCell + e];
-        if (t != 0) trk[f * a.ncell + g] += t;
+        if (t != 0) trk[f * a.ncell + g] = tpre[i][f] + t;
       }
     }
   }
@@ -541,6 +629,23 @@ __global__ __launch_bounds__(kStepThreads) void k_step(StepArgs a) {
   if (!DEPOSIT) {
     for (int off = 32; off > 0; off >>= 1) lost_here += __shfl_xor(lost_here, off, 64);
     if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = lost_here;
+  }
+  if (!RELAX_ONLY) {
+    // statistics: steps and far moves (one atomic per wave), tile load
+    for (int off = 32; off > 0; off >>= 1) {
+      steps_here += __shfl_xor(steps_here, off, 64);
+      far_here += __shfl_xor(far_here, off, 64);
+    }
+    const uint32_t slot = (uint32_t)(u * 4 + (threadIdx.x >> 6)) & (kStatSlots - 1);
+    if ((threadIdx.x & 63) == 0 && steps_here)
+      atomicAdd((unsigned long long*)&a.stat[kStatSteps * kStatSlots + slot], (unsigned long long)steps_here);
+    if ((threadIdx.x & 63) == 0 && far_here)
+      atomicAdd((unsigned long long*)&a.stat[kStatFar * kStatSlots + slot], (unsigned long long)far_here);
+    if (threadIdx.x == 0 && nin_u) {
+      atomicMax((unsigned long long*)&a.stat[kStatMaxTile * kStatSlots + (u & (kStatSlots - 1))],
+                (unsigned long long)nin_u);
+      if (crowded) atomicAdd((unsigned long long*)&a.stat[kStatCrowded * kStatSlots + (u & (kStatSlots - 1))], 1ull);
+    }
   }
   __syncthreads();
   if (!DEPOSIT && threadIdx.x == 0) {

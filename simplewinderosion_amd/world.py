@@ -207,7 +207,17 @@ class World:
         self.spawn = p.spawn
 
     def set_timing(self, on):
-        self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")
+        """False/0: off; True/1: time every kernel launch; n > 1: every n-th."""
+        self._check(self.lib.swe_set_timing(self.ctx, int(on)), "swe_set_timing")
+
+    def set_tune(self, knob, value):
+        """Launch tuning knob (capi.SWE_TUNE_*); changes speed, never results."""
+        self._check(self.lib.swe_set_tune(self.ctx, int(knob), int(value)), "swe_set_tune")
+
+    def get_tune(self, knob):
+        v = ctypes.c_int64()
+        self._check(self.lib.swe_get_tune(self.ctx, int(knob), ctypes.byref(v)), "swe_get_tune")
+        return v.value
 
     def stats(self):
         s = capi.SweStats()
@@ -299,7 +309,17 @@ class Batch:
         self._check(self.lib.swe_synchronize(self.ctx), "swe_synchronize")
 
     def set_timing(self, on):
-        self._check(self.lib.swe_set_timing(self.ctx, int(bool(on))), "swe_set_timing")
+        """False/0: off; True/1: time every kernel launch; n > 1: every n-th."""
+        self._check(self.lib.swe_set_timing(self.ctx, int(on)), "swe_set_timing")
+
+    def set_tune(self, knob, value):
+        """Launch tuning knob (capi.SWE_TUNE_*); changes speed, never results."""
+        self._check(self.lib.swe_set_tune(self.ctx, int(knob), int(value)), "swe_set_tune")
+
+    def get_tune(self, knob):
+        v = ctypes.c_int64()
+        self._check(self.lib.swe_get_tune(self.ctx, int(knob), ctypes.byref(v)), "swe_get_tune")
+        return v.value
 
     def stats(self):
         s = capi.SweStats()

@@ -125,6 +125,71 @@ def test_both_paths_agree_at_window_one(statics, periodic, layers, track, deposi
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("order", [1, 3, 64])
+def test_tile_order_does_not_change_results(statics, order):
+    """The workgroup -> tile order is a speed knob: every order gives the
+    oracle's bits (170x230 has 11x8 = 88 tiles, so orders 3 and 64 also
+    exercise the tail that keeps its own index)."""
+    from simplewinderosion_amd import capi
+
+    say("case tile order", order)
+    Wind.suspension = 1e-2
+    World.settling = 0.05
+    rng = np.random.default_rng(5)
+    h = (rng.random((170, 230)) * 0.5).astype(np.float32)
+    w = World(170, 230, periodic=True, layers=2, window=1, abrasion=1e-3)
+    sed = np.full((170, 230), 0.002, np.float32)
+    w.upload(h, sed)
+    w.set_tune(capi.SWE_TUNE_TILE_ORDER, order)
+    assert w.get_tune(capi.SWE_TUNE_TILE_ORDER) == order
+    World.SEED = 3
+    st = O.OracleState(170, 230, h, sediment=sed, params=O.params_from(w.params()))
+    st.erode(30000, 2)
+    w.erode(30000, n_cycles=2)
+    w.synchronize()
+    assert_same(w, st)
+    w.close()
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+@pytest.mark.parametrize("path", [1, 2])
+def test_step_statistics(statics, periodic, path):
+    """swe_stats counts the fly() calls that returned true on the device: the
+    oracle's count on bounded maps (early deaths) and 513 per particle per
+    cycle on periodic ones; the tile statistics are consistent."""
+    say("case stats", periodic, path)
+    Wind.suspension = 1e-2
+    w, st = run_pair(70, 100, 4000, 2, seed=9, periodic=periodic, window=1, settling=0.05, path=path)
+    assert_same(w, st)
+    s = w.stats()
+    assert s["particle_steps"] == st.steps.value
+    if periodic:
+        assert s["particle_steps"] == 4000 * 513 * 2
+    if path == 2:
+        assert 0 < s["max_tile_particles"] <= 4000
+        assert s["crowded_tile_windows"] == 0 or s["max_tile_particles"] > 2048
+    w.reset_stats()
+    s = w.stats()
+    assert s["particle_steps"] == 0 and s["max_tile_particles"] == 0
+    w.close()
+
+
+@pytest.mark.parametrize("nx,ny,particles,settling", [(64, 64, 20000, 0.02), (32, 32, 12000, 0.01)])
+def test_crowded_tiles(statics, nx, ny, particles, settling):
+    """Tiles holding more particles than the step kernel's LDS list (2048)
+    take their sources in several lists; results stay the oracle's bits.
+    32x32 has two tiles of ~6000 particles: six lists per tile and window."""
+    say("case crowded", nx, ny, particles)
+    Wind.suspension = 1e-4
+    w, st = run_pair(nx, ny, particles, 1, seed=4, periodic=True, window=1, settling=settling, path=2,
+                     layers=2, abrasion=1e-3, sed=np.full((nx, ny), 0.002, np.float32))
+    assert_same(w, st)
+    s = w.stats()
+    assert s["crowded_tile_windows"] > 0 and s["max_tile_particles"] > 2048
+    assert s["particle_steps"] == st.steps.value
+    w.close()
+
+
 def test_parity_cpu256_workload(statics):
     """The 256x256 two-layer workload (fBm + sediment 0.05, wind (1,0,1)) for 3 cycles."""
     from simplewinderosion_amd import workloads

@@ -91,6 +91,22 @@ def test_mass_ledger_exact(periodic):
     assert st.lost.value != 0
 
 
+def test_step_count_closed_form_and_bounded_deaths():
+    """Periodic map: every particle makes exactly maxAge + 1 = 513 steps per
+    cycle (wind.hpp:45-46; it dies at its 514th call).  Bounded map: particles
+    that leave die early, so fewer; each one still makes at least one step
+    unless it spawned on the far edge (round(pos) == nx or ny)."""
+    p = O.reference_params(periodic=1, seed=3)
+    st, _ = _run(p, particles=300, cycles=2)
+    assert st.steps.value == 300 * 513 * 2
+    p = O.reference_params(periodic=0, seed=3)
+    st, _ = _run(p, particles=300, cycles=2)
+    assert 0 < st.steps.value < 300 * 513 * 2
+    # one thread gives the same count
+    st1, _ = _run(p, particles=300, cycles=2, nthreads=1)
+    assert st1.steps.value == st.steps.value
+
+
 def test_deposit_on_death_closes_map_mass():
     p = O.reference_params(periodic=1, seed=3, deposit_on_death=1)
     st, z0 = _run(p)
