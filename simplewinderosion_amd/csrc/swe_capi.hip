@@ -71,9 +71,10 @@ struct swe_ctx {
   int64_t fcap = 0;
   swe::ParticleBuf PB[2] = {}, FB[3] = {};
   uint32_t* farcnt = nullptr;  // 3
-  int64_t* tbase[2] = {nullptr, nullptr};
-  int64_t* tcnt[2] = {nullptr, nullptr};
+  int64_t* sbase[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range starts
+  uint32_t* scnt[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range lengths
   uint32_t* nin = nullptr;     // 3 x tiles
+  uint32_t* csum = nullptr;    // 3 x chunks of kStepChunk tiles
   int32_t* sp_tile = nullptr;
   uint32_t* sp_rank = nullptr;
 
@@ -189,13 +190,12 @@ void free_pbuf(swe::ParticleBuf& b) {
   dfree(b.L);
   dfree(b.LC);
   dfree(b.PID);
-  dfree(b.CODE);
 }
 
 int alloc_pbuf(swe_ctx* c, swe::ParticleBuf& b, int64_t n) {
   int r;
   if ((r = dalloc(c, &b.P0, n)) || (r = dalloc(c, &b.P1, n)) || (r = dalloc(c, &b.L, n)) ||
-      (r = dalloc(c, &b.LC, n)) || (r = dalloc(c, &b.PID, n)) || (r = dalloc(c, &b.CODE, n)))
+      (r = dalloc(c, &b.LC, n)) || (r = dalloc(c, &b.PID, n)))
     return r;
   return 0;
 }
@@ -206,11 +206,13 @@ int ensure_step(swe_ctx* c, int64_t n) {
   // particle indices and record positions are 32-bit on this path
   if (n > (int64_t)UINT32_MAX) return fail(c, SWE_EINVAL, "at most 2^32 - 1 particles per cycle on the tile step path");
   if (!c->fD[0]) {
-    const int64_t nt = c->ntiles;
+    const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
     if ((r = dalloc(c, &c->fD[0], c->ncell)) || (r = dalloc(c, &c->fD[1], c->ncell)) ||
         (r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
-        (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->tbase[0], nt)) || (r = dalloc(c, &c->tbase[1], nt)) ||
-        (r = dalloc(c, &c->tcnt[0], nt)) || (r = dalloc(c, &c->tcnt[1], nt)) || (r = dalloc(c, &c->nin, 3 * nt)))
+        (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->sbase[0], 9 * nt)) ||
+        (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
+        (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
+        (r = dalloc(c, &c->csum, 3 * nch)))
       return r;
   }
   if (c->p.layers == 2 && !c->fDS[0]) {
@@ -222,8 +224,10 @@ int ensure_step(swe_ctx* c, int64_t n) {
     dfree(c->sp_tile);
     dfree(c->sp_rank);
     c->fcap = 0;
+    // a tile's output region has room for all its particles in each of its
+    // nine direction sub-ranges
     for (int k = 0; k < 2; k++)
-      if ((r = alloc_pbuf(c, c->PB[k], n))) return r;
+      if ((r = alloc_pbuf(c, c->PB[k], 9 * n))) return r;
     for (int k = 0; k < 3; k++)
       if ((r = alloc_pbuf(c, c->FB[k], n))) return r;
     if ((r = dalloc(c, &c->sp_tile, n)) || (r = dalloc(c, &c->sp_rank, n))) return r;
@@ -426,14 +430,15 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sa.max_age = c->p.max_age;
   sa.tile_order = c->tile_order;
 
-  const int64_t nt = c->ntiles;
+  const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
   swe::SpawnArgs sp{};
   sp.out = c->PB[0];
   sp.tile = c->sp_tile;
   sp.rank = c->sp_rank;
   sp.count = c->nin;  // window 0's input counts
-  sp.base1 = c->tbase[1];
-  sp.cnt = c->tcnt[1];
+  sp.csum = c->csum;
+  sp.sub_base = c->sbase[1];
+  sp.sub_cnt = c->scnt[1];
   sp.cycle = c->dcycle;
   sp.terr = c->dterr;
   sp.nterr = c->nterr;
@@ -446,6 +451,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
     if (total > 0) {
       swe::launch_zero32(c->nin, 3 * nt, c->stream);
+      swe::launch_zero32(c->csum, 3 * nch, c->stream);
       swe::launch_zero32(c->farcnt, 3, c->stream);
       swe::launch_spawn(sp, c->stream);
       HIPCHK(c, hipGetLastError());
@@ -475,13 +481,16 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.far_cnt_in = c->farcnt + w3;
         sa.far_cnt_out = c->farcnt + w3n;
         sa.far_cnt_clear = c->farcnt + w3c;
-        sa.base_prev = c->tbase[wp ^ 1];
-        sa.cnt_prev = c->tcnt[wp ^ 1];
-        sa.base_cur = c->tbase[wp];
-        sa.cnt_cur = c->tcnt[wp];
+        sa.sub_base_prev = c->sbase[wp ^ 1];
+        sa.sub_cnt_prev = c->scnt[wp ^ 1];
+        sa.sub_base_cur = c->sbase[wp];
+        sa.sub_cnt_cur = c->scnt[wp];
         sa.nin_cur = c->nin + w3 * nt;
         sa.nin_next = c->nin + w3n * nt;
         sa.nin_clear = c->nin + w3c * nt;
+        sa.csum_cur = c->csum + w3 * nch;
+        sa.csum_next = c->csum + w3n * nch;
+        sa.csum_clear = c->csum + w3c * nch;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if ((r = timed_begin(c, &e0, &e1))) return r;
         swe::launch_step(sa, c->p.periodic != 0, c->p.track != 0, c->p.layers, c->p.deposit_on_death != 0,
@@ -547,8 +556,8 @@ int swe_destroy(swe_ctx* c) {
     dfree(c->fD[k]);
     dfree(c->fDS[k]);
     dfree(c->fV[k]);
-    dfree(c->tbase[k]);
-    dfree(c->tcnt[k]);
+    dfree(c->sbase[k]);
+    dfree(c->scnt[k]);
     free_pbuf(c->PB[k]);
   }
   for (int k = 0; k < 3; k++) free_pbuf(c->FB[k]);
@@ -562,6 +571,7 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->LC);
   dfree(c->farcnt);
   dfree(c->nin);
+  dfree(c->csum);
   dfree(c->sp_tile);
   dfree(c->sp_rank);
   dfree(c->lost);

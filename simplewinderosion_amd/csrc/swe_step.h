@@ -16,13 +16,14 @@ namespace swe {
 #ifndef SWE_STEP_THREADS
 #define SWE_STEP_THREADS 256
 #endif
-#ifndef SWE_STEP_LIST
-#define SWE_STEP_LIST 2048
+// occupancy target of k_step: waves per SIMD the register budget must allow
+#ifndef SWE_STEP_WAVES_PER_EU
+#define SWE_STEP_WAVES_PER_EU 4
 #endif
 constexpr int kStepTileX = SWE_STEP_TILE_X;    // tile extent along x (slow axis)
 constexpr int kStepTileY = SWE_STEP_TILE_Y;    // tile extent along y (fast axis)
 constexpr int kStepTileThreads = SWE_STEP_THREADS;
-constexpr int kStepListCap = SWE_STEP_LIST;    // particles gathered in LDS per tile
+constexpr int kStepChunk = 32;                 // tiles per chunk sum of the input counts
 
 // One terrain of a context: its cells, tiles and particles are contiguous
 // ranges of the context's buffers
@@ -43,7 +44,6 @@ struct ParticleBuf {
   int64_t* L;     // fixed-point load
   int32_t* LC;    // last pre-move cell
   uint32_t* PID;  // particle index (RNG stream)
-  uint8_t* CODE;  // direction (0..8) of the tile it occupies from the writing tile
 };
 
 struct StepArgs {
@@ -70,13 +70,17 @@ struct StepArgs {
   uint32_t* far_cnt_in;
   uint32_t* far_cnt_out;
   uint32_t* far_cnt_clear;   // the far counter of the window after next
-  const int64_t* base_prev;  // segment base per tile of `in`
-  const int64_t* cnt_prev;   // segment length per tile of `in`
-  int64_t* base_cur;         // segment base per tile of `out` (written here)
-  int64_t* cnt_cur;          // segment length per tile of `out`
+  // routing: 9 sub-ranges (one per direction) of each tile's output region
+  const int64_t* sub_base_prev;  // [tile * 9 + d] start of sub-range d in `in`
+  const uint32_t* sub_cnt_prev;  // [tile * 9 + d] its length
+  int64_t* sub_base_cur;         // the same for `out` (written here)
+  uint32_t* sub_cnt_cur;
   const uint32_t* nin_cur;   // particles each tile receives this window
   uint32_t* nin_next;        // ... next window (accumulated here)
   uint32_t* nin_clear;       // ... the window after next (zeroed here)
+  const uint32_t* csum_cur;  // nin_cur summed over chunks of kStepChunk tiles
+  uint32_t* csum_next;
+  uint32_t* csum_clear;
   const uint64_t* cycle;
   const Terrain* terr;
   const int32_t* tile_terrain;  // per global tile
@@ -94,8 +98,9 @@ struct SpawnArgs {
   int32_t* tile;     // per particle
   uint32_t* rank;    // per particle
   uint32_t* count;   // per tile, zero on entry: window 0's input counts
-  int64_t* base1;    // per tile: segment base of the spawned particles
-  int64_t* cnt;      // per tile: segment length
+  uint32_t* csum;    // per chunk of kStepChunk tiles: window 0's chunk sums
+  int64_t* sub_base; // [tile * 9 + d]: window 0's arrivals, all in d = 4 (own tile)
+  uint32_t* sub_cnt;
   const uint64_t* cycle;
   const Terrain* terr;
   int32_t nterr;

@@ -13,11 +13,14 @@
 //   2. one fly() call (wind.hpp:28-117) for every particle in the tile
 //      against the LDS snapshot; pickups, cascade-event counts and tracks
 //      summed in LDS (int64 / uint32) and stored once per cell;
-//   3. routing: each particle is appended to its tile's segment of the
-//      output particle buffer tagged with the direction of the tile it now
-//      occupies (one of nine); per-direction counts are added to the
-//      destination tiles' input counts for the next window, where every
-//      workgroup turns them into its own segment base with a block-wide sum.
+//   3. routing: the tile's output region of the particle buffer has nine
+//      sub-ranges, one per direction (-1, 0, +1)^2 of the tile a particle now
+//      occupies, each with room for all the tile's particles; a particle is
+//      appended to the sub-range of its direction.  The per-direction counts
+//      are added to the destination tiles' input counts for the next window
+//      (and to the sums of chunks of kChunk tiles), from which every
+//      workgroup derives the start of its own output region.  A tile reads
+//      its arrivals from the nine sub-ranges that point at it: no scan.
 //
 // A particle that jumps more than one tile (never seen in practice) goes to a
 // "far" list that every workgroup scans for particles in its tile.
@@ -34,7 +37,8 @@ constexpr int kH2X = kSX + 4, kH2Y = kSY + 4;  // events halo 2
 constexpr int kH1X = kSX + 2, kH1Y = kSY + 2;  // new values halo 1
 constexpr int kStepThreads = kStepTileThreads;
 constexpr int kCell = kSX * kSY;
-constexpr int kList = kStepListCap;  // particles of a tile gathered into LDS before stepping
+constexpr int kChunk = kStepChunk;   // tiles per input-count chunk sum
+constexpr uint32_t kCrowded = 2048;  // tile load counted as crowded in the statistics
 
 constexpr int32_t c_ox[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
 constexpr int32_t c_oy[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
@@ -74,9 +78,11 @@ __device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
   return v;
 }
 
-// four workgroups per CU fit the LDS; at most 128 VGPRs keep four waves per SIMD
+// the register budget keeps SWE_STEP_WAVES_PER_EU waves per SIMD (the LDS
+// of one workgroup, under 32 KiB, lets five workgroups share a CU)
 template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool RELAX_ONLY>
-__global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_step(StepArgs a) {
+__global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SWE_STEP_WAVES_PER_EU))) void k_step(
+    StepArgs a) {
   __shared__ float s_new[kH1X * kH1Y];
   __shared__ float s_snew[LAYERS == 2 ? kH1X * kH1Y : 1];
   // relax-phase and march-phase working sets share one buffer
@@ -84,17 +90,12 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
   constexpr int kMarchBytes = 8 * kCell * (LAYERS == 2 ? 2 : 1) + 4 * kCell + (TRACK ? 32 * kCell : 0);
   constexpr int kWorkBytes = kRelaxBytes > kMarchBytes ? kRelaxBytes : kMarchBytes;
   __shared__ __attribute__((aligned(16))) unsigned char s_work[kWorkBytes];
-  __shared__ uint32_t s_out;
-  __shared__ uint32_t s_slot[9];
-  __shared__ int32_t s_src[9];
-  __shared__ uint32_t s_srcmask[9];
-  __shared__ int32_t s_nsrc;
+  __shared__ uint32_t s_slot[9];     // particles sent in direction d
+  __shared__ int64_t s_rbase[9];     // arrivals: sub-range d of the source tile
+  __shared__ uint32_t s_rpre[10];    // exclusive prefix of the sub-range lengths
   __shared__ int64_t s_lost[kStepThreads / 64];
+  __shared__ uint64_t s_wstat[kStepThreads / 64];  // per wave: far moves << 32 | steps
   __shared__ int64_t s_base;
-  __shared__ uint32_t s_list[kList];
-  __shared__ uint32_t s_nlist;
-  __shared__ int64_t s_sbase[9];
-  __shared__ int64_t s_slen[9];
 
   // XCD-aware order (workgroup b runs on XCD b % 8).  tile_order 0: XCD k
   // takes a contiguous range of tiles, so neighbouring tiles share halo
@@ -153,100 +154,38 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
     }
   }
 
-  // distinct source tiles and, for each, the direction codes that lead here
-  if (threadIdx.x == 0) {
-    int32_t n = 0;
-    for (int d = 0; d < 9; d++) {
-      const int32_t dx = d / 3 - 1, dy = d % 3 - 1;
-      int32_t sx = tx - dx, sy = ty - dy;
-      if (PERIODIC) {
-        sx = wrapt(sx, tiles_x);
-        sy = wrapt(sy, tiles_y);
-      } else if (sx < 0 || sy < 0 || sx >= tiles_x || sy >= tiles_y) {
-        continue;
-      }
-      const int32_t src = t0 + sx * tiles_y + sy;
-      int j = 0;
-      while (j < n && s_src[j] != src) j++;
-      if (j == n) {
-        s_src[n] = src;
-        s_srcmask[n] = 0;
-        n++;
-      }
-      s_srcmask[j] |= 1u << d;
+  // arrivals of this window: the source tile in direction -d of this one
+  // wrote the particles that moved by d into its sub-range d
+  if (threadIdx.x < 9) {
+    const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
+    int32_t sx = tx - dx, sy = ty - dy;
+    bool ok = !RELAX_ONLY;
+    if (PERIODIC) {
+      sx = wrapt(sx, tiles_x);
+      sy = wrapt(sy, tiles_y);
+    } else {
+      ok = ok && sx >= 0 && sy >= 0 && sx < tiles_x && sy < tiles_y;
     }
-    s_nsrc = n;
-    s_nlist = 0;
+    int64_t rb = 0;
+    uint32_t rc = 0;
+    if (ok) {
+      const int64_t s9 = (int64_t)(t0 + sx * tiles_y + sy) * 9 + d;
+      rb = a.sub_base_prev[s9];
+      rc = a.sub_cnt_prev[s9];
+    }
+    s_rbase[d] = rb;
+    s_slot[d] = rc;  // lengths, turned into s_rpre below
   }
-  __syncthreads();
-  // this window's output segment starts after those of tiles 0 .. u-1, whose
-  // lengths are bounded by their input counts nin[t]; the loads are issued
-  // here and summed after the relax phase
+  // this window's output region starts at 9 x (input counts of tiles 0 ..
+  // u-1): the chunk sums of the earlier chunks plus the earlier tiles of this
+  // chunk; the loads are issued here and summed after the relax phase
   int64_t part = 0;
-  for (int32_t t = threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
-  // gather the indices of this tile's particles into LDS: the source
-  // segments' bases and lengths are loaded in parallel, then a flat range
-  // [f0, f1) of the concatenated segments is scanned with kScanPer
-  // independent code loads per thread in flight and the records that belong
-  // here are appended to s_list.  A tile holding more than kList particles
-  // (crowded) takes its sources in flat ranges of kList records, one list
-  // per range.
-  const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
-  const bool crowded = nin_u > (uint32_t)kList;
-  auto flat_total = [&]() __attribute__((always_inline)) {
-    int64_t total = 0;
-    for (int q = 0; q < s_nsrc; q++) total += s_slen[q];
-    return total;
-  };
-  auto gather = [&](int64_t f0, int64_t f1) __attribute__((always_inline)) {
-    const int32_t nsrc = s_nsrc;
-    int64_t pre[9];
-    int64_t total = 0;
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-      pre[q] = total;
-      if (q < nsrc) total += s_slen[q];
-    }
-    if (f1 > total) f1 = total;
-    constexpr int kScanPer = 16;
-    for (int64_t base = f0; base < f1; base += (int64_t)kStepThreads * kScanPer) {
-      uint32_t code[kScanPer], mask[kScanPer];
-      uint32_t idx[kScanPer];  // particle buffers hold fewer than 2^32 records
-#pragma unroll
-      for (int q = 0; q < kScanPer; q++) {
-        const int64_t f = base + (int64_t)q * kStepThreads + threadIdx.x;
-        code[q] = 255;
-        mask[q] = 0;
-        idx[q] = 0;
-        if (f < f1) {
-          int si = 0;
-#pragma unroll
-          for (int r = 1; r < 9; r++) si = (r < nsrc && f >= pre[r]) ? r : si;
-          idx[q] = (uint32_t)(s_sbase[si] + (f - pre[si]));
-          mask[q] = s_srcmask[si];
-          code[q] = a.in.CODE[idx[q]];
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < kScanPer; q++) {
-        if (code[q] < 9 && ((mask[q] >> code[q]) & 1u)) {
-          const uint32_t pos = atomicAdd(&s_nlist, 1u);
-          // pos < kList: a full range holds at most nin_u <= kList records of
-          // this tile, a crowded tile's range at most kList records
-          if (pos < (uint32_t)kList) s_list[pos] = idx[q];
-        }
-      }
-    }
-  };
-  if (!RELAX_ONLY) {
-    const int32_t nsrc = s_nsrc;
-    if (threadIdx.x < (unsigned)nsrc) {
-      s_sbase[threadIdx.x] = a.base_prev[s_src[threadIdx.x]];
-      s_slen[threadIdx.x] = a.cnt_prev[s_src[threadIdx.x]];
-    }
-    __syncthreads();
-    gather(0, crowded ? (int64_t)kList : INT64_MAX);
+  {
+    const int32_t cu = u / kChunk;
+    for (int32_t c = threadIdx.x; c < cu; c += kStepThreads) part += a.csum_cur[c];
+    for (int32_t t = cu * kChunk + threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
   }
+  const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
 
   // ---------------- 1. relax of the previous window ----------------
   {
@@ -390,8 +329,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
     if (threadIdx.x == 0) {
       int64_t b0 = 0;
       for (int i = 0; i < kStepThreads / 64; i++) b0 += s_lost[i];
-      s_base = b0;
-      a.base_cur[u] = b0;
+      s_base = 9 * b0;
+      uint32_t r = 0;
+      for (int d = 0; d < 9; d++) {
+        s_rpre[d] = r;
+        r += s_slot[d];
+      }
+      s_rpre[9] = r;
     }
   }
 
@@ -411,8 +355,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       acc_t[3 * kCell + e] = 0;
     }
   }
+  __syncthreads();  // s_rpre is read from s_slot before s_slot restarts
   if (threadIdx.x < 9) s_slot[threadIdx.x] = 0;
-  if (threadIdx.x == 0) s_out = 0;
   int64_t lost_here = 0;
   uint32_t steps_here = 0, far_here = 0;
   __syncthreads();
@@ -433,7 +377,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
     k.nps = mk(T.nps[0], T.nps[1], T.nps[2]);
     const uint64_t ck = cycle_key(T.seed, *a.cycle);
     const int64_t p0 = T.part0;
-    const int64_t out_base = s_base;
+    const int64_t out_base = s_base;  // 9 x the input counts of tiles 0 .. u-1
     const uint32_t far_n = *a.far_cnt_in;
 
     // one input record
@@ -451,6 +395,20 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       r.lastc = in.LC[j];
       r.pid = in.PID[j];
       return r;
+    };
+    // a particle more than one tile away goes to the far list; the tile of
+    // cell (fx, fy) counts it among its next window's arrivals
+    auto send_far = [&](float4 o0, float4 o1, int64_t ledger, int32_t lastc, uint32_t pid, int32_t fx,
+                        int32_t fy) __attribute__((always_inline)) {
+      const int32_t dest = t0 + (fx / kSX) * tiles_y + fy / kSY;
+      atomicAdd(&a.nin_next[dest], 1u);
+      atomicAdd(&a.csum_next[dest / kChunk], 1u);
+      const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
+      a.far_out.P0[fslot] = o0;
+      a.far_out.P1[fslot] = o1;
+      a.far_out.L[fslot] = ledger;
+      a.far_out.LC[fslot] = lastc;
+      a.far_out.PID[fslot] = pid;
     };
     // step one record and write it out
     auto process = [&](const Rec& r) __attribute__((always_inline)) {
@@ -471,13 +429,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       if (ok && (cx < x0 || cy < y0 || cx >= x0 + kSX || cy >= y0 + kSY)) {
         // routing always places a particle in the tile of its cell; keep
         // LDS indices in range whatever happens by forwarding a stray one
-        const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
-        a.far_out.P0[fslot] = q0;
-        a.far_out.P1[fslot] = q1;
-        a.far_out.L[fslot] = w.ledger;
-        a.far_out.LC[fslot] = w.lastc;
-        a.far_out.PID[fslot] = pid;
-        a.far_out.CODE[fslot] = 255;
+        send_far(q0, q1, w.ledger, w.lastc, pid, cx, cy);
         return;
       }
       if (ok) {
@@ -507,7 +459,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       // route by the cell the particle now occupies; one that leaves a
       // bounded map (or whose position stops being finite) stays with the
       // tile of its last cell and dies there at its next call
-      int32_t ncx, ncy;
+      int32_t ncx = 0, ncy = 0;
       int32_t code = 4;
       if (cell_of<PERIODIC>(w.px, w.pz, k, ncx, ncy)) {
         int32_t dtx = ncx / kSX - tx, dty = ncy / kSY - ty;
@@ -527,48 +479,35 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       const float4 o1 = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
       if (code == 255) {
         far_here++;
-        const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
-        a.far_out.P0[fslot] = o0;
-        a.far_out.P1[fslot] = o1;
-        a.far_out.L[fslot] = w.ledger;
-        a.far_out.LC[fslot] = w.lastc;
-        a.far_out.PID[fslot] = pid;
-        a.far_out.CODE[fslot] = 255;
+        send_far(o0, o1, w.ledger, w.lastc, pid, ncx, ncy);
         return;
       }
-      const int64_t o = out_base + atomicAdd(&s_out, 1u);
-      atomicAdd(&s_slot[code], 1u);
+      // sub-range `code` of this tile's output region holds up to nin_u records
+      const int64_t o = out_base + (int64_t)code * nin_u + atomicAdd(&s_slot[code], 1u);
       a.out.P0[o] = o0;
       a.out.P1[o] = o1;
       a.out.L[o] = w.ledger;
       a.out.LC[o] = w.lastc;
       a.out.PID[o] = pid;
-      a.out.CODE[o] = (uint8_t)code;
     };
 
-    // step the listed records; the next record is loaded while the current
-    // one is stepped
-    auto step_list = [&]() __attribute__((always_inline)) {
-      const uint32_t nl = s_nlist < (uint32_t)kList ? s_nlist : (uint32_t)kList;
+    // the arrivals, e = 0 .. n_in - 1 over the nine sub-ranges in order; each
+    // thread's e increase, so its sub-range cursor only moves forward.  The
+    // next record is loaded while the current one is stepped.
+    const uint32_t n_in = s_rpre[9];
+    uint32_t sr = 0;
+    auto index_of = [&](uint32_t e) __attribute__((always_inline)) {
+      while (sr < 8 && e >= s_rpre[sr + 1]) sr++;
+      return s_rbase[sr] + (int64_t)(e - s_rpre[sr]);
+    };
+    {
       Rec cur{};
-      if (threadIdx.x < nl) cur = load(a.in, (int64_t)s_list[threadIdx.x]);
-      for (uint32_t e = threadIdx.x; e < nl; e += kStepThreads) {
+      if (threadIdx.x < n_in) cur = load(a.in, index_of(threadIdx.x));
+      for (uint32_t e = threadIdx.x; e < n_in; e += kStepThreads) {
         Rec nxt{};
-        if (e + kStepThreads < nl) nxt = load(a.in, (int64_t)s_list[e + kStepThreads]);
+        if (e + kStepThreads < n_in) nxt = load(a.in, index_of(e + kStepThreads));
         process(cur);
         cur = nxt;
-      }
-    };
-    step_list();
-    if (crowded) {
-      const int64_t total = flat_total();
-      for (int64_t f0 = kList; f0 < total; f0 += kList) {
-        __syncthreads();  // the previous list is consumed
-        if (threadIdx.x == 0) s_nlist = 0;
-        __syncthreads();
-        gather(f0, f0 + kList);
-        __syncthreads();
-        step_list();
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
@@ -608,22 +547,33 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
       }
     }
   }
-  if (threadIdx.x == 0) a.cnt_cur[u] = (int64_t)s_out;
+  // s_rbase is free again (the arrivals are consumed): it carries the chunk
+  // of each direction's destination tile to the aggregation below
+  int32_t* s_dchunk = (int32_t*)s_rbase;
   if (threadIdx.x < 9) {
-    // particles sent towards direction d: the next window's input of that tile
-    const uint32_t n = s_slot[threadIdx.x];
+    // particles sent towards direction d: sub-range d of this tile's output,
+    // and the next window's arrivals of the tile in direction d
+    const int32_t d = threadIdx.x;
+    const uint32_t n = s_slot[d];
+    a.sub_base_cur[(int64_t)u * 9 + d] = s_base + (int64_t)d * nin_u;
+    a.sub_cnt_cur[(int64_t)u * 9 + d] = n;
+    int32_t chunk = -1;
     if (n) {
-      const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
+      const int32_t dx = d / 3 - 1, dy = d % 3 - 1;
       int32_t dtx = tx + dx, dty = ty + dy;
       if (PERIODIC) {
         dtx = wrapt(dtx, tiles_x);
         dty = wrapt(dty, tiles_y);
       }
-      atomicAdd(&a.nin_next[t0 + dtx * tiles_y + dty], n);
+      const int32_t dest = t0 + dtx * tiles_y + dty;
+      atomicAdd(&a.nin_next[dest], n);
+      chunk = dest / kChunk;
     }
+    s_dchunk[d] = chunk;
   }
   if (threadIdx.x == 0) {
     a.nin_clear[u] = 0;  // the window after next accumulates here
+    if (u % kChunk == 0) a.csum_clear[u / kChunk] = 0;
     if (u == 0) *a.far_cnt_clear = 0;
   }
   if (!DEPOSIT) {
@@ -631,21 +581,37 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(4)
     if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = lost_here;
   }
   if (!RELAX_ONLY) {
-    // statistics: steps and far moves (one atomic per wave), tile load
     for (int off = 32; off > 0; off >>= 1) {
       steps_here += __shfl_xor(steps_here, off, 64);
       far_here += __shfl_xor(far_here, off, 64);
     }
-    const uint32_t slot = (uint32_t)(u * 4 + (threadIdx.x >> 6)) & (kStatSlots - 1);
-    if ((threadIdx.x & 63) == 0 && steps_here)
-      atomicAdd((unsigned long long*)&a.stat[kStatSteps * kStatSlots + slot], (unsigned long long)steps_here);
-    if ((threadIdx.x & 63) == 0 && far_here)
-      atomicAdd((unsigned long long*)&a.stat[kStatFar * kStatSlots + slot], (unsigned long long)far_here);
-    if (threadIdx.x == 0 && nin_u) {
-      atomicMax((unsigned long long*)&a.stat[kStatMaxTile * kStatSlots + (u & (kStatSlots - 1))],
-                (unsigned long long)nin_u);
-      if (crowded) atomicAdd((unsigned long long*)&a.stat[kStatCrowded * kStatSlots + (u & (kStatSlots - 1))], 1ull);
+    if ((threadIdx.x & 63) == 0) s_wstat[threadIdx.x >> 6] = ((uint64_t)far_here << 32) | steps_here;
+  }
+  __syncthreads();
+  if (!RELAX_ONLY && threadIdx.x == 0) {
+    // chunk sums of the next window's arrivals: one atomic per distinct chunk
+    // (the nine destinations mostly share one or two)
+    for (int d = 0; d < 9; d++) {
+      const int32_t ch = s_dchunk[d];
+      if (ch < 0) continue;
+      bool first = true;
+      for (int e = 0; e < d; e++) first = first && s_dchunk[e] != ch;
+      if (!first) continue;
+      uint32_t n = 0;
+      for (int e = d; e < 9; e++) n += s_dchunk[e] == ch ? s_slot[e] : 0u;
+      atomicAdd(&a.csum_next[ch], n);
     }
+    // statistics: one atomic per workgroup and kind
+    uint64_t steps = 0, far = 0;
+    for (int i = 0; i < kStepThreads / 64; i++) {
+      steps += (uint32_t)s_wstat[i];
+      far += s_wstat[i] >> 32;
+    }
+    const uint32_t slot = (uint32_t)u & (kStatSlots - 1);
+    if (steps) atomicAdd((unsigned long long*)&a.stat[kStatSteps * kStatSlots + slot], (unsigned long long)steps);
+    if (far) atomicAdd((unsigned long long*)&a.stat[kStatFar * kStatSlots + slot], (unsigned long long)far);
+    if (nin_u) atomicMax((unsigned long long*)&a.stat[kStatMaxTile * kStatSlots + slot], (unsigned long long)nin_u);
+    if (nin_u > kCrowded) atomicAdd((unsigned long long*)&a.stat[kStatCrowded * kStatSlots + slot], 1ull);
   }
   __syncthreads();
   if (!DEPOSIT && threadIdx.x == 0) {
@@ -695,16 +661,15 @@ __global__ __launch_bounds__(256) void k_spawn_count(SpawnArgs a) {
 }
 
 __global__ __launch_bounds__(256) void k_spawn_scan(SpawnArgs a) {
-  // one workgroup: exclusive prefix of the per-tile counts
+  // one workgroup: exclusive prefix of the per-tile counts -> each tile's
+  // spawned particles form sub-range 4 (its own tile) of window 0's input;
+  // chunk sums of the counts
   __shared__ int64_t s_part[256];
   const int32_t ntiles = a.ntiles;
   const int32_t per = (ntiles + 255) / 256;
   const int32_t t0 = threadIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
   int64_t run = 0;
-  for (int32_t t = t0; t < t1; t++) {
-    a.base1[t] = run;
-    run += a.count[t];
-  }
+  for (int32_t t = t0; t < t1; t++) run += a.count[t];
   s_part[threadIdx.x] = run;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -716,10 +681,21 @@ __global__ __launch_bounds__(256) void k_spawn_scan(SpawnArgs a) {
     }
   }
   __syncthreads();
-  const int64_t off = s_part[threadIdx.x];
+  int64_t b = s_part[threadIdx.x];
   for (int32_t t = t0; t < t1; t++) {
-    a.base1[t] += off;
-    a.cnt[t] = a.count[t];
+    const uint32_t n = a.count[t];
+    for (int d = 0; d < 9; d++) {
+      a.sub_base[(int64_t)t * 9 + d] = d == 4 ? b : 0;
+      a.sub_cnt[(int64_t)t * 9 + d] = d == 4 ? n : 0u;
+    }
+    b += n;
+  }
+  const int32_t nchunks = (ntiles + kStepChunk - 1) / kStepChunk;
+  for (int32_t c = threadIdx.x; c < nchunks; c += 256) {
+    uint32_t sum = 0;
+    const int32_t e = (c + 1) * kStepChunk < ntiles ? (c + 1) * kStepChunk : ntiles;
+    for (int32_t t = c * kStepChunk; t < e; t++) sum += a.count[t];
+    a.csum[c] = sum;
   }
 }
 
@@ -729,13 +705,12 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
   const Terrain T = a.terr[terrain_of(a.terr, a.nterr, i)];
   Particle w;
   spawn(particle_key(cycle_key(T.seed, *a.cycle), (uint64_t)(i - T.part0)), T.nx, T.ny, a.spawn_mode, w);
-  const int64_t o = a.base1[a.tile[i]] + a.rank[i];
+  const int64_t o = a.sub_base[(int64_t)a.tile[i] * 9 + 4] + a.rank[i];
   a.out.P0[o] = make_float4(w.px, w.py, w.pz, w.sed);
   a.out.P1[o] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
   a.out.L[o] = 0;
   a.out.LC[o] = 0;
   a.out.PID[o] = (uint32_t)i;
-  a.out.CODE[o] = 4;  // in its own tile
 }
 
 // ---------------------------------------------------------------------------
