@@ -86,7 +86,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   __shared__ float s_new[kH1X * kH1Y];
   __shared__ float s_snew[LAYERS == 2 ? kH1X * kH1Y : 1];
   // relax-phase and march-phase working sets share one buffer
-  constexpr int kRelaxBytes = 4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y + 8 * kH1X * kH1Y;
+  constexpr int kRelaxBytes =
+      4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y + 8 * kH1X * kH1Y + 2 * kH2X * kH2Y;
   constexpr int kMarchBytes = 8 * kCell * (LAYERS == 2 ? 2 : 1) + 4 * kCell + (TRACK ? 32 * kCell : 0);
   constexpr int kWorkBytes = kRelaxBytes > kMarchBytes ? kRelaxBytes : kMarchBytes;
   __shared__ __attribute__((aligned(16))) unsigned char s_work[kWorkBytes];
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   __shared__ int64_t s_lost[kStepThreads / 64];
   __shared__ uint64_t s_wstat[kStepThreads / 64];  // per wave: far moves << 32 | steps
   __shared__ int64_t s_base;
+  __shared__ uint32_t s_ncas;        // cells whose cascade moves material
 
   // XCD-aware order (workgroup b runs on XCD b % 8).  tile_order 0: XCD k
   // takes a contiguous range of tiles, so neighbouring tiles share halo
@@ -243,13 +245,52 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       if (ok) v = v_in[(int64_t)gx * ny + gy];
       s_vold[e] = v;
     }
+    uint16_t* s_cas = (uint16_t*)(s_casc + kH1X * kH1Y);  // H2 cells with a transfer
     for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
+    if (threadIdx.x == 0) s_ncas = 0;
     __syncthreads();
 
-    // cascades of every visited cell within two cells of the tile
-    for (int e = threadIdx.x; e < kH2X * kH2Y; e += kStepThreads) {
+    // visited cells within two cells of the tile whose cascade moves
+    // material: one neighbour beyond the threshold at the window's start
+    // height.  The walk changes the centre only after a transfer, so a cell
+    // with none such moves nothing and is skipped.  Listed in LDS so that
+    // the cascades below run on full waves.
+    for (int e0 = 0; e0 < kH2X * kH2Y; e0 += kStepThreads) {
+      const int e = e0 + (int)threadIdx.x;
+      bool act = false;
+      if (e < kH2X * kH2Y && s_vold[e] != 0) {
+        const int32_t lx = e / kH2Y, ly = e % kH2Y;
+        const int32_t gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+        const int32_t ic = (lx + 1) * kH3Y + (ly + 1);
+        const float hc = s_old[ic];
+        const float dd = __builtin_sqrtf(2.0f);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          bool ok = true;
+          if (!PERIODIC) {
+            const int32_t qx = gx + c_ox[k], qy = gy + c_oy[k];
+            ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
+          }
+          const float diff = hc - s_old[ic + c_ox[k] * kH3Y + c_oy[k]];
+          const float d = (c_ox[k] != 0 && c_oy[k] != 0) ? dd : 1.0f;
+          const float excess = __builtin_fabsf(diff) - (d * a.maxdiff) * 1.0f;
+          act = act || (ok && diff != 0.0f && !(excess <= 0.0f));
+        }
+      }
+      const uint64_t m = __ballot(act);
+      const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+      uint32_t base = 0;
+      if ((threadIdx.x & 63) == 0 && m) base = atomicAdd(&s_ncas, (uint32_t)__popcll(m));
+      base = __shfl(base, 0, 64);
+      if (act) s_cas[base + below] = (uint16_t)e;
+    }
+    __syncthreads();
+
+    // their cascades
+    const uint32_t ncas = s_ncas;
+    for (uint32_t i = threadIdx.x; i < ncas; i += kStepThreads) {
+      const int32_t e = s_cas[i];
       const uint32_t v = s_vold[e];
-      if (v == 0) continue;
       const int32_t lx = e / kH2Y, ly = e % kH2Y;  // H2 coords; H3 = +1, H1 = -1
       const int32_t gx = x0 - 2 + lx, gy = y0 - 2 + ly;
       float hn[8];
