@@ -117,7 +117,7 @@ def main():
     ap.add_argument("--workload", default="gpu1024")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
     ap.add_argument("--window", type=int, default=None)
     ap.add_argument("--track", type=int, default=None)
     ap.add_argument("--tile-order", type=int, default=None,
