@@ -65,8 +65,6 @@ struct swe_ctx {
   int32_t* dtile_terrain = nullptr;
 
   // tile-owned step path
-  int64_t* fD[2] = {nullptr, nullptr};
-  int64_t* fDS[2] = {nullptr, nullptr};
   uint32_t* fV[2] = {nullptr, nullptr};
   int64_t fcap = 0;
   swe::ParticleBuf PB[2] = {}, FB[3] = {};
@@ -205,18 +203,14 @@ int ensure_step(swe_ctx* c, int64_t n) {
   int r;
   // particle indices and record positions are 32-bit on this path
   if (n > (int64_t)UINT32_MAX) return fail(c, SWE_EINVAL, "at most 2^32 - 1 particles per cycle on the tile step path");
-  if (!c->fD[0]) {
+  if (!c->fV[0]) {
     const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
-    if ((r = dalloc(c, &c->fD[0], c->ncell)) || (r = dalloc(c, &c->fD[1], c->ncell)) ||
-        (r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
+    if ((r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
         (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->sbase[0], 9 * nt)) ||
         (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
         (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
         (r = dalloc(c, &c->csum, 3 * nch)))
       return r;
-  }
-  if (c->p.layers == 2 && !c->fDS[0]) {
-    if ((r = dalloc(c, &c->fDS[0], c->ncell)) || (r = dalloc(c, &c->fDS[1], c->ncell))) return r;
   }
   if (n > c->fcap) {
     for (int k = 0; k < 2; k++) free_pbuf(c->PB[k]);
@@ -467,11 +461,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.sm_out = layers2 ? c->smv[nxt] : nullptr;
         sa.ssnap_in = layers2 ? c->ss[cur] : nullptr;
         sa.ssnap_out = layers2 ? c->ss[nxt] : nullptr;
-        sa.d_in = c->fD[cur];
-        sa.ds_in = layers2 ? c->fDS[cur] : nullptr;
         sa.v_in = c->fV[cur];
-        sa.d_out = c->fD[nxt];
-        sa.ds_out = layers2 ? c->fDS[nxt] : nullptr;
         sa.v_out = c->fV[nxt];
         const int w3 = (int)(w % 3), w3n = (int)((w + 1) % 3), w3c = (int)((w + 2) % 3);
         sa.in = c->PB[wp];
@@ -553,8 +543,6 @@ int swe_destroy(swe_ctx* c) {
     dfree(c->smv[k]);
     dfree(c->ss[k]);
     dfree(c->vis[k]);
-    dfree(c->fD[k]);
-    dfree(c->fDS[k]);
     dfree(c->fV[k]);
     dfree(c->sbase[k]);
     dfree(c->scnt[k]);

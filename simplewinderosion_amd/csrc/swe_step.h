@@ -47,20 +47,18 @@ struct ParticleBuf {
 };
 
 struct StepArgs {
-  // map, window parity in -> out
+  // map, window parity in -> out.  A master holds the previous window's
+  // pickups already; its cascades are added here (relax), then this
+  // window's pickups before it is written out.
   const int64_t* zm_in;
   int64_t* zm_out;
-  const float* snap_in;
-  float* snap_out;
+  const float* snap_in;  // snapshot the previous window's particles read
+  float* snap_out;       // snapshot of this window (master after the relax)
   const int64_t* sm_in;
   int64_t* sm_out;
   const float* ssnap_in;
   float* ssnap_out;
-  const int64_t* d_in;   // previous window's pickups
-  const int64_t* ds_in;
   const uint32_t* v_in;  // previous window's cascade events
-  int64_t* d_out;        // this window's pickups
-  int64_t* ds_out;
   uint32_t* v_out;
   int64_t* trk;          // 4 x ncell tracks of the cycle
   int64_t* lost;         // kLostSlots counters

@@ -132,11 +132,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   int64_t* sm_out = LAYERS == 2 ? a.sm_out + c0 : nullptr;
   const float* ssnap_in = LAYERS == 2 ? a.ssnap_in + c0 : nullptr;
   float* ssnap_out = LAYERS == 2 ? a.ssnap_out + c0 : nullptr;
-  const int64_t* d_in = a.d_in + c0;
-  const int64_t* ds_in = LAYERS == 2 ? a.ds_in + c0 : nullptr;
   const uint32_t* v_in = a.v_in + c0;
-  int64_t* d_out = a.d_out + c0;
-  int64_t* ds_out = LAYERS == 2 ? a.ds_out + c0 : nullptr;
   uint32_t* v_out = a.v_out + c0;
   int64_t* trk = TRACK ? a.trk + c0 : nullptr;
 
@@ -195,8 +191,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     float* s_sold = s_old + kH3X * kH3Y;  // valid when LAYERS == 2
     uint32_t* s_vold = (uint32_t*)(s_old + kH3X * kH3Y * (LAYERS == 2 ? 2 : 1));
     int64_t* s_casc = (int64_t*)(s_vold + kH2X * kH2Y);
-    // master + pickup delta of the tile and its halo, loaded before the
-    // cascades are computed
+    // master (with the previous window's pickups) of the tile and its halo,
+    // loaded before the cascades are computed
     constexpr int kH1Per = (kH1X * kH1Y + kStepThreads - 1) / kStepThreads;
     int64_t zpre[kH1Per], spre[LAYERS == 2 ? kH1Per : 1];
 #pragma unroll
@@ -211,8 +207,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
       const int64_t g = (int64_t)gx * ny + gy;
-      zpre[i] = ok ? zm_in[g] + d_in[g] : 0;
-      if (LAYERS == 2) spre[i] = ok ? sm_in[g] + ds_in[g] : 0;
+      zpre[i] = ok ? zm_in[g] : 0;
+      if (LAYERS == 2) spre[i] = ok ? sm_in[g] : 0;
     }
     for (int e = threadIdx.x; e < kH3X * kH3Y; e += kStepThreads) {
       int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
@@ -321,7 +317,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     }
     __syncthreads();
 
-    // master += pickups + cascade; next snapshot for the tile and its halo
+    // master += cascade; next snapshot for the tile and its halo.  The
+    // tile's own masters seed the particle phase's pickup accumulators
+    // (acc_d / acc_ds below: the front of s_work, whose relax-phase arrays
+    // are dead once the cascades are summed, and clear of s_casc)
+    static_assert(8 * kCell * (LAYERS == 2 ? 2 : 1) <= 4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y,
+                  "the pickup accumulators must not overlap s_casc");
+    int64_t* acc_z = (int64_t*)s_work;
 #pragma unroll
     for (int i = 0; i < kH1Per; i++) {
       const int e = threadIdx.x + i * kStepThreads;
@@ -346,8 +348,9 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       const float zf = f32_from_fx(z);
       s_new[e] = zf;
       const bool mine = lx >= 1 && ly >= 1 && lx <= kSX && ly <= kSY && x0 - 1 + lx < nx && y0 - 1 + ly < ny;
+      const int32_t li = (lx - 1) * kSY + (ly - 1);
       if (mine) {
-        zm_out[g] = z;
+        acc_z[li] = z;
         snap_out[g] = zf;
       }
       if (LAYERS == 2) {
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
         const float sf = f32_from_fx(s);
         s_snew[e] = sf;
         if (mine) {
-          sm_out[g] = s;
+          acc_z[kCell + li] = s;
           ssnap_out[g] = sf;
         }
       }
@@ -386,9 +389,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   uint32_t* acc_v = (uint32_t*)(acc_d + kCell * (LAYERS == 2 ? 2 : 1));
   int64_t* acc_t = (int64_t*)(acc_v + kCell);  // TRACK (kCell*4), 8-byte aligned: kCell*4 bytes of acc_v
   for (int e = threadIdx.x; e < kCell; e += kStepThreads) {
-    acc_d[e] = 0;
-    if (LAYERS == 2) acc_ds[e] = 0;
-    acc_v[e] = 0;
+    acc_v[e] = 0;  // acc_d / acc_ds hold the masters, the pickups add to them
     if (TRACK) {
       acc_t[e] = 0;
       acc_t[kCell + e] = 0;
@@ -577,9 +578,11 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
     if (e >= kCell || gx >= nx || gy >= ny) continue;
     const int64_t g = (int64_t)gx * ny + gy;
-    d_out[g] = acc_d[e];
+    // master with this window's pickups; the clamp of the sediment layer at
+    // zero follows the next window's cascades
+    zm_out[g] = acc_d[e];
     v_out[g] = acc_v[e];
-    if (LAYERS == 2) ds_out[g] = acc_ds[e];
+    if (LAYERS == 2) sm_out[g] = acc_ds[e];
     if (TRACK && !RELAX_ONLY) {
 #pragma unroll
       for (int f = 0; f < 4; f++) {
