@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--track", type=int, default=None)
     ap.add_argument("--tile-order", type=int, default=None,
                     help="SWE_TUNE_TILE_ORDER for the step kernel (default: the library's)")
+    ap.add_argument("--graph", type=int, default=None,
+                    help="SWE_TUNE_GRAPH: 1 = a cycle is one HIP graph launch (used when timing is off)")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -147,9 +149,11 @@ def main():
         over["track"] = bool(args.track)
     world = workloads.make_world(args.workload, seed=args.seed, **over)
     p = world.params()
+    from simplewinderosion_amd import capi
     if args.tile_order is not None:
-        from simplewinderosion_amd import capi
         world.set_tune(capi.SWE_TUNE_TILE_ORDER, args.tile_order)
+    if args.graph is not None:
+        world.set_tune(capi.SWE_TUNE_GRAPH, args.graph)
     if not p.periodic:
         raise SystemExit("bench counts particle-steps analytically: periodic workloads only")
     N = wl["particles"]

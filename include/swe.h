@@ -156,8 +156,12 @@ int swe_set_path(swe_ctx* ctx, int32_t path);
  *     tiles; 0 (default) = each of the 8 XCDs a contiguous range of tiles,
  *     C in [1, 4096] = runs of C consecutive tiles dealt round-robin over the
  *     XCDs.
+ *   SWE_TUNE_GRAPH: 1 (default) = the tile path captures a map-cycle's
+ *     kernel launches into a HIP graph once (per starting window parity and
+ *     launch configuration) and launches the graph each cycle; 0 = launches
+ *     each kernel.  Timing (swe_set_timing) uses individual launches.
  */
-enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1 };
+enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2 };
 int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);
 int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
 

@@ -18,6 +18,7 @@ SWE_EAUDIT = -4
 SWE_ESTATE = -5
 
 SWE_TUNE_TILE_ORDER = 1
+SWE_TUNE_GRAPH = 2
 
 
 class SweParams(ctypes.Structure):

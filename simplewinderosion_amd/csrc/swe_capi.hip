@@ -33,6 +33,11 @@ struct swe_ctx {
   std::string err;
   int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
   int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
+  int32_t use_graph = 1;   // SWE_TUNE_GRAPH
+  // one captured map-cycle of the tile path per starting window parity, and
+  // the launch configuration it was captured for
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::string gkey[2];
 
   // map (device); index [g & 1] is current, g counts snapshot windows
   uint64_t g = 0;
@@ -441,7 +446,9 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sp.periodic = c->p.periodic;
   sp.spawn_mode = c->p.spawn;
 
-  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+  // the launches of one map-cycle (kernels only: no copies, fills or host
+  // synchronisation, so that the sequence can be captured into a graph)
+  auto enqueue_cycle = [&]() -> int {
     if (c->p.track) swe::launch_zero64(c->trk, 4 * c->ncell, c->stream);
     if (total > 0) {
       swe::launch_zero32(c->nin, 3 * nt, c->stream);
@@ -482,17 +489,64 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.csum_next = c->csum + w3n * nch;
         sa.csum_clear = c->csum + w3c * nch;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if ((r = timed_begin(c, &e0, &e1))) return r;
+        int rr;
+        if ((rr = timed_begin(c, &e0, &e1))) return rr;
         swe::launch_step(sa, c->p.periodic != 0, c->p.track != 0, c->p.layers, c->p.deposit_on_death != 0,
                          w == nwin, c->stream);
         HIPCHK(c, hipGetLastError());
-        if ((r = timed_end(c, e0, e1, 2))) return r;
+        if ((rr = timed_end(c, e0, e1, 2))) return rr;
         c->g++;
       }
     }
     if (c->p.track) swe::launch_ema(c->ema, c->trk, 4 * c->ncell, c->p.lrate, c->stream);
     swe::launch_bump(c->dcycle, c->stream);
     HIPCHK(c, hipGetLastError());
+    return 0;
+  };
+
+  // graph mode: the cycle is captured once per starting parity and launched
+  // as one graph; the capture is redone when anything it depends on changes
+  const bool graph = c->use_graph != 0 && c->timing == 0;
+  std::string key;
+  if (graph) {
+    key.append((const char*)&c->p, sizeof(c->p));
+    key.append((const char*)particles, sizeof(int64_t) * (size_t)c->nterr);
+    const void* ptrs[2] = {(const void*)c->PB[0].P0, (const void*)c->trk};
+    key.append((const char*)ptrs, sizeof(ptrs));
+    key.append((const char*)&c->tile_order, sizeof(c->tile_order));
+    key.append((const char*)&nwin, sizeof(nwin));
+  }
+  const uint64_t windows_per_cycle = total > 0 ? (uint64_t)nwin + 1 : 0;
+
+  for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
+    if (graph) {
+      const int par = (int)(c->g & 1);
+      if (!c->gexec[par] || c->gkey[par] != key) {
+        if (c->gexec[par]) (void)hipGraphExecDestroy(c->gexec[par]);
+        c->gexec[par] = nullptr;
+        const uint64_t g0 = c->g;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_cycle();
+        hipGraph_t gr = nullptr;
+        const hipError_t ee = hipStreamEndCapture(c->stream, &gr);
+        c->g = g0;  // the capture only recorded the launches
+        if (rc != 0 || ee != hipSuccess) {
+          if (gr) (void)hipGraphDestroy(gr);
+          return rc != 0 ? rc : fail(c, SWE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[par], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (ei != hipSuccess) {
+          c->gexec[par] = nullptr;
+          return fail(c, SWE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+        c->gkey[par] = key;
+      }
+      HIPCHK(c, hipGraphLaunch(c->gexec[par], c->stream));
+      c->g += windows_per_cycle;
+    } else {
+      if ((r = enqueue_cycle())) return r;
+    }
     c->cycle++;
     c->st.cycles++;
     if (c->timing && c->pending.size() > 512) {
@@ -567,6 +621,8 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->dcycle);
   dfree(c->dterr);
   dfree(c->dtile_terrain);
+  for (int k = 0; k < 2; k++)
+    if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -840,6 +896,10 @@ int swe_set_tune(swe_ctx* c, int32_t knob, int64_t value) {
       if (value < 0 || value > 4096) return fail(c, SWE_EINVAL, "tile order must be in [0, 4096]");
       c->tile_order = (int32_t)value;
       return 0;
+    case SWE_TUNE_GRAPH:
+      if (value != 0 && value != 1) return fail(c, SWE_EINVAL, "graph mode must be 0 or 1");
+      c->use_graph = (int32_t)value;
+      return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");
   }
@@ -850,6 +910,9 @@ int swe_get_tune(swe_ctx* c, int32_t knob, int64_t* value) {
   switch (knob) {
     case SWE_TUNE_TILE_ORDER:
       *value = c->tile_order;
+      return 0;
+    case SWE_TUNE_GRAPH:
+      *value = c->use_graph;
       return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");
