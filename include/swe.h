@@ -166,6 +166,25 @@ int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);
 int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
 
 /*
+ * Autotuner (no reference counterpart): times `trial_cycles` map-cycles of
+ * the context's own workload (particles per terrain, as for
+ * swe_erode_batch) for every tile order in {0, 1, 4, 16} with and without
+ * graphs, each from the same saved state, then restores the state (map,
+ * fields, counters and statistics are as before the call) and keeps the
+ * fastest setting.  Tile step path (window == 1) only.
+ */
+typedef struct swe_tune_result {
+  int32_t tile_order;
+  int32_t graph;
+  int32_t candidates;    /* settings timed */
+  int32_t reserved;
+  double ms_per_cycle;   /* of the setting kept */
+} swe_tune_result;
+int swe_autotune(swe_ctx* ctx, const int64_t* particles, int32_t trial_cycles, swe_tune_result* out);
+/* The GPU architecture of the context's device (e.g. "gfx950:sramecc+:xnack-"), the key of tuning caches. */
+int swe_device_arch(swe_ctx* ctx, char* buf, int32_t len);
+
+/*
  * Batches: many terrains of different sizes in one context, eroded together
  * by one launch per snapshot window (window == 1).  Terrain k's cells, tiles
  * and particles are contiguous ranges of the context's buffers; results of

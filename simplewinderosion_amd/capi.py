@@ -96,6 +96,16 @@ class SweStats(ctypes.Structure):
     ]
 
 
+class SweTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("tile_order", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("candidates", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("ms_per_cycle", ctypes.c_double),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/swe.h declares
 _P = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -124,6 +134,8 @@ SIGNATURES = {
     "swe_load_state": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "swe_set_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
     "swe_set_tune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64]),
+    "swe_autotune": (ctypes.c_int, [_P, _I64P, ctypes.c_int32, ctypes.POINTER(SweTuneResult)]),
+    "swe_device_arch": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
     "swe_get_tune": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64)]),
     "swe_stats": (ctypes.c_int, [_P, ctypes.POINTER(SweStats)]),
     "swe_reset_stats": (ctypes.c_int, [_P]),

@@ -882,6 +882,127 @@ int swe_erode_batch(swe_ctx* c, const int64_t* particles, int32_t n_cycles) {
   return erode_global(c, particles[0], n_cycles, nwin, R);
 }
 
+int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe_tune_result* out) {
+  if (!c || !particles || !out) return SWE_EINVAL;
+  if (trial_cycles < 1) return fail(c, SWE_EINVAL, "trial_cycles must be >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  int r;
+  if ((r = ensure_layers(c))) return r;
+  int32_t R;
+  (void)windows_of(c->p, &R);
+  if (!(c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1))
+    return fail(c, SWE_EINVAL, "autotune covers the tile step path (window == 1)");
+  if ((r = collect_timing(c))) return r;
+  // the state a trial changes, kept on the device and put back afterwards
+  const int cur0 = c->cur();
+  const uint64_t g0 = c->g, cycle0 = c->cycle;
+  const swe_stats_t st0 = c->st;
+  const int64_t n = c->ncell;
+  const int64_t nlost = (int64_t)swe::kLostSlots * c->nterr, nstat = (int64_t)swe::kStatKinds * swe::kStatSlots;
+  int64_t *zm = nullptr, *sm = nullptr, *lost = nullptr;
+  float *zs = nullptr, *ss = nullptr, *ema = nullptr;
+  uint64_t *cyc = nullptr, *stat = nullptr;
+  const bool l2 = c->smv[0] != nullptr;
+  auto release = [&]() {
+    dfree(zm); dfree(sm); dfree(lost); dfree(zs); dfree(ss); dfree(ema); dfree(cyc); dfree(stat);
+  };
+  if ((r = dalloc(c, &zm, n)) || (r = dalloc(c, &zs, n)) || (l2 && ((r = dalloc(c, &sm, n)) || (r = dalloc(c, &ss, n)))) ||
+      (r = dalloc(c, &ema, 4 * n)) || (r = dalloc(c, &lost, nlost)) || (r = dalloc(c, &cyc, 1)) ||
+      (r = dalloc(c, &stat, nstat))) {
+    release();
+    return r;
+  }
+  auto copy = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream);
+  };
+  auto save_or_restore = [&](bool save) -> hipError_t {
+    hipError_t e = hipSuccess;
+    auto cp = [&](void* state, void* keep, size_t bytes) {
+      if (e == hipSuccess) e = save ? copy(keep, state, bytes) : copy(state, keep, bytes);
+    };
+    cp(c->zmv[cur0], zm, sizeof(int64_t) * n);
+    cp(c->zs[cur0], zs, sizeof(float) * n);
+    if (l2) {
+      cp(c->smv[cur0], sm, sizeof(int64_t) * n);
+      cp(c->ss[cur0], ss, sizeof(float) * n);
+    }
+    cp(c->ema, ema, sizeof(float) * 4 * n);
+    cp(c->lost, lost, sizeof(int64_t) * nlost);
+    cp(c->dcycle, cyc, sizeof(uint64_t));
+    cp(c->dstat, stat, sizeof(uint64_t) * nstat);
+    if (!save && e == hipSuccess && c->fV[cur0]) {
+      swe::launch_zero32(c->fV[cur0], n, c->stream);  // no events pending at a cycle start
+      e = hipGetLastError();
+    }
+    c->g = g0;
+    c->cycle = cycle0;
+    c->st = st0;
+    return e;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = save_or_restore(true);
+  if (he == hipSuccess) he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    release();
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(c, SWE_EHIP, std::string("autotune set-up: ") + hipGetErrorString(he));
+  }
+  const int32_t order0 = c->tile_order, graph0 = c->use_graph, timing0 = c->timing;
+  c->timing = 0;
+  static const int32_t kOrders[] = {0, 1, 4, 16};
+  double best = 1e300;
+  int32_t best_order = order0, best_graph = graph0, ncand = 0;
+  for (int32_t graph = 0; graph <= 1 && r == 0; graph++) {
+    for (int32_t order : kOrders) {
+      c->tile_order = order;
+      c->use_graph = graph;
+      // one untimed cycle (graph capture, caches), then the timed ones, each
+      // from the saved state
+      if ((he = save_or_restore(false)) != hipSuccess) break;
+      if ((r = swe_erode_batch(c, particles, 1))) break;
+      if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+      if ((r = swe_erode_batch(c, particles, trial_cycles))) break;
+      if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) break;
+      if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
+      float ms = 0.0f;
+      if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+      const double per = ms / trial_cycles;
+      ncand++;
+      if (per < best) {
+        best = per;
+        best_order = order;
+        best_graph = graph;
+      }
+    }
+    if (he != hipSuccess) break;
+  }
+  const hipError_t hr = save_or_restore(false);
+  if (he == hipSuccess) he = hr;
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  release();
+  c->timing = timing0;
+  c->tile_order = r == 0 && he == hipSuccess ? best_order : order0;
+  c->use_graph = r == 0 && he == hipSuccess ? best_graph : graph0;
+  if (r) return r;
+  if (he != hipSuccess) return fail(c, SWE_EHIP, std::string("autotune: ") + hipGetErrorString(he));
+  out->tile_order = c->tile_order;
+  out->graph = c->use_graph;
+  out->candidates = ncand;
+  out->ms_per_cycle = best;
+  return 0;
+}
+
+int swe_device_arch(swe_ctx* c, char* buf, int32_t len) {
+  if (!c || !buf || len < 1) return SWE_EINVAL;
+  hipDeviceProp_t prop;
+  HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+  std::snprintf(buf, (size_t)len, "%s", prop.gcnArchName);
+  return 0;
+}
+
 int swe_erode(swe_ctx* c, int64_t particles, int32_t n_cycles) {
   if (!c) return SWE_EINVAL;
   if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context erodes with swe_erode_batch");

@@ -219,6 +219,14 @@ class World:
         self._check(self.lib.swe_get_tune(self.ctx, int(knob), ctypes.byref(v)), "swe_get_tune")
         return v.value
 
+    def autotune(self, particles, trial_cycles=2, cache=None, retune=False):
+        """Pick the fastest launch setting for this workload (map state is
+        kept); see simplewinderosion_amd.tuning."""
+        from . import tuning
+
+        return tuning.autotune(self, particles, trial_cycles=trial_cycles,
+                               cache=tuning.DEFAULT_CACHE if cache is None else cache, retune=retune)
+
     def stats(self):
         s = capi.SweStats()
         self._check(self.lib.swe_stats(self.ctx, ctypes.byref(s)), "swe_stats")
@@ -320,6 +328,14 @@ class Batch:
         v = ctypes.c_int64()
         self._check(self.lib.swe_get_tune(self.ctx, int(knob), ctypes.byref(v)), "swe_get_tune")
         return v.value
+
+    def autotune(self, particles, trial_cycles=2, cache=None, retune=False):
+        """Pick the fastest launch setting for this workload (map state is
+        kept); see simplewinderosion_amd.tuning."""
+        from . import tuning
+
+        return tuning.autotune(self, particles, trial_cycles=trial_cycles,
+                               cache=tuning.DEFAULT_CACHE if cache is None else cache, retune=retune)
 
     def stats(self):
         s = capi.SweStats()
