@@ -1,0 +1,59 @@
+/*
+ * swe_nn.h -- C-ABI of libswe_nn.so: the kernels of the learned surrogate
+ * of the erosion (no counterpart in the reference, which has no surrogate;
+ * the surrogate predicts K map-cycles of World::erode, world.hpp:36-77, at
+ * once).  All pointers are device pointers; `stream` is a hipStream_t (NULL
+ * = the default stream).  Activations are NHWC bf16, weights [Cout][9][Cin]
+ * bf16 (tap t = 3*(dy+1) + (dx+1)), padding periodic.  Every call returns 0,
+ * -1 for bad arguments or -2 for a launch error.
+ */
+#ifndef SWE_NN_H
+#define SWE_NN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* pixels per weight-gradient partial sum */
+#define kSweNnChunk 8192
+
+/* y = conv3x3(x, w) + bias, optional ReLU; y16 (bf16) and/or y32 (float).
+ * Cin % 8 == 0; BF16 MFMA with FP32 accumulation. */
+int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
+                   int32_t Cout, int32_t relu, void* y16, float* y32, void* stream);
+/* wt[ci][t][co] = w[co][8-t][ci]: the weights whose forward convolution of
+ * dy is the data gradient dx (Cin and Cout swap). */
+int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream);
+/* floats of work memory swe_nn_wgrad needs */
+int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
+/* dw = sum_p dy[p] x[p + t]^T (float [Cout][9][Cin]); db = sum_p dy[p]
+ * (float [Cout], optional); deterministic. */
+int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                 float* dw, float* db, float* work, int64_t work_len, void* stream);
+/* g16 = bf16(g32 * (y > 0)); y may be NULL (conversion only) */
+int swe_nn_relu_grad(const float* g32, const void* y, void* g16, int64_t n, void* stream);
+int swe_nn_add(const void* a, const void* b, void* out, int64_t n, void* stream);
+/* 2x2 average pooling and its gradient; nearest 2x upsampling and its gradient */
+int swe_nn_pool2(const void* x, void* y, int32_t N, int32_t H, int32_t W, int32_t C, void* stream);
+int swe_nn_pool2_grad(const void* dy, void* dx, int32_t N, int32_t H, int32_t W, int32_t C, void* stream);
+int swe_nn_up2(const void* x, void* y, int32_t N, int32_t h, int32_t w, int32_t C, void* stream);
+int swe_nn_up2_grad(const void* dy, void* dx, int32_t N, int32_t h, int32_t w, int32_t C, void* stream);
+/* mean squared error over the first `used` of C channels (target [npix][used]);
+ * grad [npix][C]; part[ceil(npix*C/256)] partial loss sums */
+int swe_nn_mse(const float* pred, const float* target, float* grad, float* part, int64_t npix, int32_t C,
+               int32_t used, void* stream);
+/* Adam on float master weights w (+ bf16 copy w16, optional); step >= 1 */
+int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
+                float eps, int32_t step, void* stream);
+/* master[i] += q_i with q = round(d * scale * 2^32) corrected so that
+ * sum(q) == target exactly (fixed-point mass projection); work: 256 int64 */
+int swe_nn_project_mass(const float* d, int64_t n, float scale, int64_t target, int64_t* master, int64_t* work,
+                        void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_NN_H */

@@ -49,8 +49,27 @@ def build(verbose=False):
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building libswe.so")
     os.replace(tmp, LIB)
+    build_nn(verbose)
     build_demo(verbose)
     return LIB
+
+
+NN_LIB = os.path.join(HERE, "libswe_nn.so")
+NN_SOURCES = ["swe_nn.hip"]
+
+
+def build_nn(verbose=False):
+    """The surrogate's kernels (include/swe_nn.h) as libswe_nn.so."""
+    tmp = NN_LIB + ".tmp"
+    cmd = [hipcc()] + HIP_FLAGS + ["-shared", "-o", tmp] + [os.path.join(CSRC, s) for s in NN_SOURCES]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building libswe_nn.so")
+    os.replace(tmp, NN_LIB)
+    return NN_LIB
 
 
 DEMO_SRC = os.path.join(HERE, "examples", "erode_demo.cpp")

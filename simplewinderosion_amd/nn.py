@@ -1,0 +1,168 @@
+"""ctypes binding of libswe_nn.so (include/swe_nn.h), the surrogate's HIP
+kernels, over torch device tensors (torch supplies device memory and the
+stream; the arithmetic is the library's).  Every function launches on
+torch's current stream and raises on a bad argument or launch error.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswe_nn.so")
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_F = ctypes.c_float
+SIGNATURES = {
+    "swe_nn_conv3x3": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _I32, _P, _P, _P]),
+    "swe_nn_flip": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
+    "swe_nn_wgrad_work": (_I64, [_I32, _I32, _I32, _I32, _I32]),
+    "swe_nn_wgrad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
+    "swe_nn_relu_grad": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "swe_nn_add": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "swe_nn_pool2": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_pool2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_up2": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_up2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
+    "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
+    "swe_nn_project_mass": (ctypes.c_int, [_P, _I64, _F, _I64, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libswe_nn.so (built in-tree by simplewinderosion_amd/build.py)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libswe_nn.so is not built (%s); run `python -c 'import __graft_entry__ as g; g.build()'`"
+                               % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ck(rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed (%d)" % (what, rc))
+
+
+def _dense(t, dtype, what):
+    import torch
+
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError("%s must be a contiguous %s device tensor" % (what, dtype))
+    if t.data_ptr() % 16:
+        raise ValueError("%s must be 16-byte aligned" % what)
+    return t
+
+
+def conv3x3(x, w, bias=None, relu=False, out16=None, out32=None):
+    """x [N,H,W,Cin] bf16, w [Cout,9,Cin] bf16, bias [Cout] float -> out16
+    [N,H,W,Cout] bf16 and/or out32 float (allocated when neither is given)."""
+    import torch
+
+    _dense(x, torch.bfloat16, "x")
+    _dense(w, torch.bfloat16, "w")
+    N, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    if w.shape[1] != 9 or w.shape[2] != Cin:
+        raise ValueError("weights must be [Cout, 9, Cin]")
+    if out16 is None and out32 is None:
+        out16 = torch.empty((N, H, W, Cout), device=x.device, dtype=torch.bfloat16)
+    _ck(load().swe_nn_conv3x3(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, int(bool(relu)), _p(out16), _p(out32),
+                              _stream()), "swe_nn_conv3x3")
+    return out16 if out16 is not None else out32
+
+
+def flip(w):
+    """[Cout,9,Cin] -> [Cin,9,Cout] weights of the data gradient."""
+    import torch
+
+    Cout, _, Cin = w.shape
+    wt = torch.empty((Cin, 9, Cout), device=w.device, dtype=torch.bfloat16)
+    _ck(load().swe_nn_flip(_p(w), Cin, Cout, _p(wt), _stream()), "swe_nn_flip")
+    return wt
+
+
+def wgrad(x, dy, dw, db=None, work=None):
+    """dw [Cout,9,Cin] float = sum_p dy x^T; db [Cout] float (optional)."""
+    import torch
+
+    N, H, W, Cin = x.shape
+    Cout = dy.shape[3]
+    need = load().swe_nn_wgrad_work(N, H, W, Cin, Cout)
+    if work is None or work.numel() < need:
+        work = torch.empty(need, device=x.device, dtype=torch.float32)
+    _ck(load().swe_nn_wgrad(_p(x), _p(dy), N, H, W, Cin, Cout, _p(dw), _p(db), _p(work), work.numel(), _stream()),
+        "swe_nn_wgrad")
+    return work
+
+
+def relu_grad(g32, y, g16):
+    _ck(load().swe_nn_relu_grad(_p(g32), _p(y), _p(g16), g32.numel(), _stream()), "swe_nn_relu_grad")
+    return g16
+
+
+def add(a, b, out):
+    _ck(load().swe_nn_add(_p(a), _p(b), _p(out), a.numel(), _stream()), "swe_nn_add")
+    return out
+
+
+def pool2(x, y):
+    N, H, W, C = x.shape
+    _ck(load().swe_nn_pool2(_p(x), _p(y), N, H, W, C, _stream()), "swe_nn_pool2")
+    return y
+
+
+def pool2_grad(dy, dx):
+    N, H, W, C = dx.shape
+    _ck(load().swe_nn_pool2_grad(_p(dy), _p(dx), N, H, W, C, _stream()), "swe_nn_pool2_grad")
+    return dx
+
+
+def up2(x, y):
+    N, h, w, C = x.shape
+    _ck(load().swe_nn_up2(_p(x), _p(y), N, h, w, C, _stream()), "swe_nn_up2")
+    return y
+
+
+def up2_grad(dy, dx):
+    N, h, w, C = dx.shape
+    _ck(load().swe_nn_up2_grad(_p(dy), _p(dx), N, h, w, C, _stream()), "swe_nn_up2_grad")
+    return dx
+
+
+def mse(pred, target, grad, part):
+    """pred [npix, C] float, target [npix, used] float -> loss partials; grad [npix, C]."""
+    npix, C = pred.shape[0], pred.shape[-1]
+    used = target.shape[-1]
+    _ck(load().swe_nn_mse(_p(pred), _p(target), _p(grad), _p(part), npix, C, used, _stream()), "swe_nn_mse")
+    return part
+
+
+def adam(w, w16, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    _ck(load().swe_nn_adam(_p(w), _p(w16), _p(g), _p(m), _p(v), w.numel(), lr, b1, b2, eps, int(step), _stream()),
+        "swe_nn_adam")
+
+
+def project_mass(d, scale, target, master, work):
+    """master (int64, n) += round(d * scale * 2^32) corrected to sum exactly `target`."""
+    _ck(load().swe_nn_project_mass(_p(d), d.numel(), float(scale), int(target), _p(master), _p(work), _stream()),
+        "swe_nn_project_mass")

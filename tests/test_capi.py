@@ -100,3 +100,19 @@ def test_cpp_header_mirror_compiles_and_fails_cleanly_without_gpu(tmp_path):
     if out.returncode == 0:
         pytest.skip("a GPU is present: the GPU test covers the run")
     assert "swe_create failed" in out.stderr
+
+
+def test_nn_library_exports_every_declared_symbol():
+    """libswe_nn.so exports, and simplewinderosion_amd/nn.py binds, every
+    function include/swe_nn.h declares."""
+    from simplewinderosion_amd import nn
+
+    text = open(os.path.join(ROOT, "include", "swe_nn.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = set(re.findall(r"\b(swe_nn_[a-z_0-9]+)\s*\(", text))
+    assert names == set(nn.SIGNATURES), names ^ set(nn.SIGNATURES)
+    lib = nn.load()
+    for name in names:
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", nn.LIB_PATH], capture_output=True, text=True).stdout
+    assert names <= set(re.findall(r"\sT\s(swe_nn_[a-z_0-9]+)$", out, flags=re.M))
