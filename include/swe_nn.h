@@ -34,6 +34,8 @@ int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W,
                  float* dw, float* db, float* work, int64_t work_len, void* stream);
 /* g16 = bf16(g32 * (y > 0)); y may be NULL (conversion only) */
 int swe_nn_relu_grad(const float* g32, const void* y, void* g16, int64_t n, void* stream);
+/* g16 *= (y > 0) in place (bf16) */
+int swe_nn_relu_mask(void* g16, const void* y, int64_t n, void* stream);
 int swe_nn_add(const void* a, const void* b, void* out, int64_t n, void* stream);
 /* 2x2 average pooling and its gradient; nearest 2x upsampling and its gradient */
 int swe_nn_pool2(const void* x, void* y, int32_t N, int32_t H, int32_t W, int32_t C, void* stream);

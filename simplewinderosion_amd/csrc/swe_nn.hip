@@ -198,6 +198,13 @@ __global__ void k_relu_grad(const float* __restrict__ g32, const bf16* __restric
   g16[i] = f2bf(g);
 }
 
+// g16 *= (y > 0), in place
+__global__ void k_relu_mask(bf16* __restrict__ g16, const bf16* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  if (!(bf2f(y[i]) > 0.0f)) g16[i] = f2bf(0.0f);
+}
+
 __global__ void k_add(const bf16* __restrict__ a, const bf16* __restrict__ b, bf16* __restrict__ out, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -414,6 +421,12 @@ int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W,
 int swe_nn_relu_grad(const float* g32, const void* y, void* g16, int64_t n, void* stream) {
   if (!g32 || !g16) return -1;
   hipLaunchKernelGGL(k_relu_grad, dim3(G(n)), dim3(256), 0, S(stream), g32, (const bf16*)y, (bf16*)g16, n);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_relu_mask(void* g16, const void* y, int64_t n, void* stream) {
+  if (!g16 || !y) return -1;
+  hipLaunchKernelGGL(k_relu_mask, dim3(G(n)), dim3(256), 0, S(stream), (bf16*)g16, (const bf16*)y, n);
   SWENN_LAUNCH_CHECK();
 }
 

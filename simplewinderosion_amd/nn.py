@@ -19,6 +19,7 @@ SIGNATURES = {
     "swe_nn_wgrad_work": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "swe_nn_wgrad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
     "swe_nn_relu_grad": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
+    "swe_nn_relu_mask": (ctypes.c_int, [_P, _P, _I64, _P]),
     "swe_nn_add": (ctypes.c_int, [_P, _P, _P, _I64, _P]),
     "swe_nn_pool2": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_pool2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
@@ -117,6 +118,11 @@ def wgrad(x, dy, dw, db=None, work=None):
 
 def relu_grad(g32, y, g16):
     _ck(load().swe_nn_relu_grad(_p(g32), _p(y), _p(g16), g32.numel(), _stream()), "swe_nn_relu_grad")
+    return g16
+
+
+def relu_mask(g16, y):
+    _ck(load().swe_nn_relu_mask(_p(g16), _p(y), g16.numel(), _stream()), "swe_nn_relu_mask")
     return g16
 
 

@@ -107,6 +107,10 @@ def test_pool_upsample_add_relu_grad():
     g16 = torch.empty_like(x)
     nn.relu_grad(g32, x, g16)
     assert torch.equal(g16, torch.where(x.float() > 0, g32, torch.zeros_like(g32)).to(torch.bfloat16))
+    m16 = g32.to(torch.bfloat16)
+    ref = torch.where(x.float() > 0, m16.float(), torch.zeros_like(g32)).to(torch.bfloat16)
+    nn.relu_mask(m16, x)
+    assert torch.equal(m16, ref)
 
 
 def test_mse_and_adam():
