@@ -105,63 +105,67 @@ __global__ void k_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int32_
 // Weight gradient: dW[co][t][ci] = sum_p dY[p][co] * X[p shifted by t][ci].
 // Rows = output channels, columns = input channels, reduction over pixels.
 // Workgroup (4 waves) = one tap, a 32 (co) x 32 (ci) block, and a chunk of
-// `chunk` pixels of the flattened batch; wave w takes the 16x16 sub-block
-// (w / 2, w % 2).  Lanes gather their 8 consecutive pixels channel by
-// channel (NHWC is channel-contiguous).  Partial sums per chunk go to
-// `part`, reduced by k_reduce_parts (deterministic order).
+// `chunk` pixels of the flattened batch.  Per step of 32 pixels, half the
+// threads load dY and half the shifted X as 16-byte runs of 8 channels
+// (coalesced NHWC rows) and store them channel-major into LDS; each wave
+// then reads its MFMA operands -- 8 consecutive pixels of one channel -- as
+// 16-byte LDS reads and accumulates its 16x16 sub-block (w / 2, w % 2).
+// Partial sums per chunk go to `part`, reduced by k_reduce_parts
+// (deterministic order).  Cin % 8 == 0 and Cout % 8 == 0.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dy,
                                                float* __restrict__ part, int32_t N, int32_t H, int32_t W,
                                                int32_t Cin, int32_t Cout, int32_t chunk) {
-  const int32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[32][40];  // [co][pixel], rows padded against bank conflicts
+  __shared__ __attribute__((aligned(16))) bf16 s_x[32][40];   // [ci][pixel]
+  const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int32_t cib = (Cin + 31) / 32;
   int32_t b = blockIdx.y;
   const int32_t t = b % 9;
   b /= 9;
-  const int32_t ci0 = (b % cib) * 32 + (wave & 1) * 16;
-  const int32_t co0 = (b / cib) * 32 + (wave >> 1) * 16;
+  const int32_t ciB = (b % cib) * 32, coB = (b / cib) * 32;
   const int32_t npix = N * H * W;  // < 2^31 (checked by the launcher)
   const int32_t p0 = (int32_t)blockIdx.x * chunk;
   const int32_t pend = p0 + chunk < npix ? p0 + chunk : npix;
   const int32_t dyo = t / 3 - 1, dxo = t % 3 - 1;
-  const int32_t co = co0 + (lane & 15), ci = ci0 + (lane & 15);
-  const bool co_ok = co < Cout, ci_ok = ci < Cin;
+  // loader role: threads 0..127 load dY, 128..255 load X; (pixel q, group g)
+  const bool ldy = tid < 128;
+  const int32_t q = (tid & 127) >> 2, g = tid & 3;
+  const int32_t cl = (ldy ? coB : ciB) + 8 * g;  // first channel of this thread's run
+  const bool cl_ok = cl < (ldy ? Cout : Cin);
+  // consumer role
+  const int32_t co_l = (wave >> 1) * 16 + (lane & 15), ci_l = (wave & 1) * 16 + (lane & 15);
+  const int32_t kq = (lane >> 4) * 8;
   f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
   for (int32_t q0 = p0; q0 < pend; q0 += 32) {
-    bf16x8 a, bb;
-    // this lane's 8 consecutive pixels start at p; walk their (x, y, image)
-    int32_t p = q0 + (lane >> 4) * 8;
-    int32_t xx = p % W, r = p / W;
-    int32_t yy = r % H, im = r / H;
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-      bf16 va = (bf16)0.0f, vb = (bf16)0.0f;
-      if (p < pend) {
-        if (co_ok) va = dy[(int64_t)p * Cout + co];
-        if (ci_ok) {
-          const int32_t sy = wrapi(yy + dyo, H), sx = wrapi(xx + dxo, W);
-          vb = x[(((int64_t)im * H + sy) * W + sx) * Cin + ci];
-        }
-      }
-      a[e] = va;
-      bb[e] = vb;
-      p++;
-      if (++xx == W) {
-        xx = 0;
-        if (++yy == H) {
-          yy = 0;
-          im++;
-        }
+    const int32_t p = q0 + q;
+    bf16x8 v = {};
+    if (p < pend && cl_ok) {
+      if (ldy) {
+        v = *(const bf16x8*)(dy + (int64_t)p * Cout + cl);
+      } else {
+        const int32_t xx = p % W, r = p / W;
+        const int32_t yy = r % H, im = r / H;
+        const int32_t sy = wrapi(yy + dyo, H), sx = wrapi(xx + dxo, W);
+        v = *(const bf16x8*)(x + (((int64_t)im * H + sy) * W + sx) * Cin + cl);
       }
     }
+    bf16(*dst)[40] = ldy ? s_dy : s_x;
+#pragma unroll
+    for (int e = 0; e < 8; e++) dst[8 * g + e][q] = v[e];
+    __syncthreads();
+    const bf16x8 a = *(const bf16x8*)&s_dy[co_l][kq];
+    const bf16x8 bb = *(const bf16x8*)&s_x[ci_l][kq];
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
+    __syncthreads();
   }
-  // D rows = co0 + 4*(lane/16) + i, column = ci
-  if (!ci_ok) return;
+  // D rows = coB + 16*(wave/2) + 4*(lane/16) + i, column = ciB + 16*(wave%2) + lane%16
+  const int32_t ci = ciB + ci_l;
+  if (ci >= Cin) return;
   const int64_t nw = (int64_t)Cout * 9 * Cin;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
-    const int32_t c = co0 + (lane >> 4) * 4 + i;
+    const int32_t c = coB + (wave >> 1) * 16 + (lane >> 4) * 4 + i;
     if (c < Cout) part[(int64_t)blockIdx.x * nw + ((int64_t)c * 9 + t) * Cin + ci] = acc[i];
   }
 }
@@ -174,14 +178,26 @@ __global__ void k_reduce_parts(const float* __restrict__ part, float* __restrict
   out[i] = s;
 }
 
-// bias gradient partials: part[chunk][c] = sum over the chunk's pixels
-__global__ void k_bgrad(const bf16* __restrict__ dy, float* __restrict__ part, int64_t npix, int32_t C, int32_t chunk) {
-  const int32_t c = threadIdx.x;
-  if (c >= C) return;
+// bias gradient partials: part[chunk][c] = sum over the chunk's pixels;
+// the 256 threads form 256 / C rows of C channels, each row summing every
+// (256 / C)-th pixel, then the rows are added in LDS (fixed order)
+__global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, float* __restrict__ part, int64_t npix,
+                                               int32_t C, int32_t chunk) {
+  __shared__ float s_sum[256];
+  const int32_t rows = 256 / C;
+  const int32_t c = threadIdx.x % C, r = threadIdx.x / C;
   const int64_t p0 = (int64_t)blockIdx.x * chunk;
+  const int64_t pend = p0 + chunk < npix ? p0 + chunk : npix;
   float s = 0.0f;
-  for (int64_t p = p0; p < p0 + chunk && p < npix; p++) s += bf2f(dy[p * C + c]);
-  part[(int64_t)blockIdx.x * C + c] = s;
+  if (r < rows)
+    for (int64_t p = p0 + r; p < pend; p += rows) s += bf2f(dy[p * C + c]);
+  s_sum[threadIdx.x] = s;
+  __syncthreads();
+  if (r == 0) {
+    float tot = 0.0f;
+    for (int32_t k = 0; k < rows; k++) tot += s_sum[k * C + c];
+    part[(int64_t)blockIdx.x * C + c] = tot;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -400,6 +416,7 @@ int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t 
 int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                  float* dw, float* db, float* work, int64_t work_len, void* stream) {
   if (!x || !dy || !dw || !work) return -1;
+  if (Cin < 8 || Cout < 8 || Cin % 8 || Cout % 8) return -1;
   if ((int64_t)N * H * W >= ((int64_t)1 << 31) - kSweNnChunk) return -1;
   if (work_len < swe_nn_wgrad_work(N, H, W, Cin, Cout)) return -1;
   const int64_t npix = (int64_t)N * H * W;
@@ -410,10 +427,14 @@ int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W,
                      Cout, kSweNnChunk);
   hipLaunchKernelGGL(k_reduce_parts, dim3(G(nw)), dim3(256), 0, S(stream), work, dw, nw, chunks);
   if (db) {
+    // smaller pixel chunks for the bias: chunks * (kSweNnChunk / kBiasChunk)
+    // partials of Cout floats fit the work size (9 * Cin >= 16)
     if (Cout > 256) return -1;
-    hipLaunchKernelGGL(k_bgrad, dim3((unsigned)chunks), dim3(256), 0, S(stream), (const bf16*)dy, work, npix, Cout,
-                       kSweNnChunk);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(G(Cout)), dim3(256), 0, S(stream), work, db, (int64_t)Cout, chunks);
+    constexpr int32_t kBiasChunk = kSweNnChunk / 16;
+    const int32_t bchunks = (int32_t)((npix + kBiasChunk - 1) / kBiasChunk);
+    hipLaunchKernelGGL(k_bgrad, dim3((unsigned)bchunks), dim3(256), 0, S(stream), (const bf16*)dy, work, npix, Cout,
+                       kBiasChunk);
+    hipLaunchKernelGGL(k_reduce_parts, dim3(G(Cout)), dim3(256), 0, S(stream), work, db, (int64_t)Cout, bchunks);
   }
   SWENN_LAUNCH_CHECK();
 }
