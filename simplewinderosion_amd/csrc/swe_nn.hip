@@ -184,10 +184,18 @@ __global__ void k_reduce_parts(const float* __restrict__ part, float* __restrict
 __global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, float* __restrict__ part, int64_t npix,
                                                int32_t C, int32_t chunk) {
   __shared__ float s_sum[256];
-  const int32_t rows = 256 / C;
-  const int32_t c = threadIdx.x % C, r = threadIdx.x / C;
   const int64_t p0 = (int64_t)blockIdx.x * chunk;
   const int64_t pend = p0 + chunk < npix ? p0 + chunk : npix;
+  if (C > 256) {  // one thread per channel, several channels per thread
+    for (int32_t c = threadIdx.x; c < C; c += 256) {
+      float s = 0.0f;
+      for (int64_t p = p0; p < pend; p++) s += bf2f(dy[p * C + c]);
+      part[(int64_t)blockIdx.x * C + c] = s;
+    }
+    return;
+  }
+  const int32_t rows = 256 / C;
+  const int32_t c = threadIdx.x % C, r = threadIdx.x / C;
   float s = 0.0f;
   if (r < rows)
     for (int64_t p = p0 + r; p < pend; p += rows) s += bf2f(dy[p * C + c]);
@@ -429,7 +437,6 @@ int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W,
   if (db) {
     // smaller pixel chunks for the bias: chunks * (kSweNnChunk / kBiasChunk)
     // partials of Cout floats fit the work size (9 * Cin >= 16)
-    if (Cout > 256) return -1;
     constexpr int32_t kBiasChunk = kSweNnChunk / 16;
     const int32_t bchunks = (int32_t)((npix + kBiasChunk - 1) / kBiasChunk);
     hipLaunchKernelGGL(k_bgrad, dim3((unsigned)bchunks), dim3(256), 0, S(stream), (const bf16*)dy, work, npix, Cout,

@@ -35,21 +35,25 @@ WORKLOADS = {
                     params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
                                 pspeed=glm_normalize((1.0, 0.0, 1.0))),
                     terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=8), sediment=0.05),
-    # the surrogate's training maps (one particle per cell) and its 1024^2
-    # rollout map: one layer, mass kept on the map (deposit on death)
-    "surr256": dict(nx=256, ny=256, particles=65536, cycles=50, periodic=True, layers=1, window=1,
-                    params=dict(suspension=1e-4, maxdiff=0.005, settling=0.01,
+    # the surrogate's training maps and its 1024^2 rollout map (the fBm
+    # recipe with the 8 obstacles); two layers, one particle per cell, mass
+    # kept on the map (deposit on death).  The 256^2 training maps have the
+    # rollout map's features per cell: one fBm period per 256 cells and, on
+    # every other map, one obstacle (0.5 per 256^2 of the rollout map).
+    "surr256": dict(nx=256, ny=256, particles=65536, cycles=50, periodic=True, layers=2, window=1,
+                    params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
                                 pspeed=glm_normalize((1.0, 0.0, 1.0))),
-                    terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=2), sediment=None),
-    "surr1024": dict(nx=1024, ny=1024, particles=1_048_576, cycles=50, periodic=True, layers=1, window=1,
-                     params=dict(suspension=1e-4, maxdiff=0.005, settling=0.01,
+                    terrain=dict(octaves=6, base_cycles=1.0, n_pyramids=1), sediment=0.05),
+    "surr1024": dict(nx=1024, ny=1024, particles=1_000_000, cycles=50, periodic=True, layers=2, window=1,
+                     params=dict(suspension=1e-4, abrasion=1e-4, maxdiff=0.005, settling=0.01,
                                  pspeed=glm_normalize((1.0, 0.0, 1.0))),
-                     terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=8), sediment=None),
+                     terrain=dict(octaves=6, base_cycles=4.0, n_pyramids=8), sediment=0.05),
 }
 
 
-def make_inputs(name, seed=1, nx=None, ny=None):
-    """Surface height and sediment arrays (float32, [nx, ny]) of a workload."""
+def make_inputs(name, seed=1, nx=None, ny=None, n_pyramids=None):
+    """Surface height and sediment arrays (float32, [nx, ny]) of a workload
+    (n_pyramids overrides the workload's obstacle count)."""
     import numpy as np
 
     w = WORKLOADS[name]
@@ -66,7 +70,7 @@ def make_inputs(name, seed=1, nx=None, ny=None):
         return h, None
     t = w["terrain"]
     base = generate_terrain(nx, ny, seed, octaves=t["octaves"], base_cycles=t["base_cycles"],
-                            n_pyramids=t["n_pyramids"])
+                            n_pyramids=t["n_pyramids"] if n_pyramids is None else n_pyramids)
     if w["sediment"] is None:
         return base.astype(np.float32), None
     sed = np.full((nx, ny), np.float32(w["sediment"]), np.float32)

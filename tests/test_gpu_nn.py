@@ -1,9 +1,10 @@
 """The surrogate's HIP kernels (libswe_nn.so, through its C-ABI) against
-plain PyTorch float32 references of the same ops: 3x3 periodic convolution
-forward on BF16 MFMA, its data and weight gradients, pooling, upsampling,
-the loss, Adam and the fixed-point mass projection.  Inputs are bf16 values;
-the references run in float32 on those values, so the tolerances only cover
-float32 summation order (and bf16 rounding of bf16 outputs)."""
+references of the same ops computed on the CPU in float64 (convolutions and
+their gradients by PyTorch autograd, the rest in closed form): 3x3 periodic
+convolution forward on BF16 MFMA, its data and weight gradients, pooling,
+upsampling, the loss, Adam and the fixed-point mass projection.  Inputs are
+bf16 values, exact in float64, so the tolerances only cover the kernels'
+float32 accumulation (and bf16 rounding of bf16 outputs)."""
 import numpy as np
 import pytest
 
@@ -18,12 +19,14 @@ DEV = "cuda"
 
 
 def ref_conv(x16, w16, bias=None):
-    """float32 periodic 3x3 convolution of NHWC x with [Cout,9,Cin] w."""
-    x = x16.float().permute(0, 3, 1, 2)
+    """float64 CPU periodic 3x3 convolution of NHWC x with [Cout,9,Cin] w,
+    returned on the device as float32 for comparison."""
+    x = x16.cpu().double().permute(0, 3, 1, 2)
     Cout, _, Cin = w16.shape
-    w = w16.float().view(Cout, 3, 3, Cin).permute(0, 3, 1, 2).contiguous()
-    y = F.conv2d(F.pad(x, (1, 1, 1, 1), mode="circular"), w, bias)
-    return y.permute(0, 2, 3, 1).contiguous()
+    w = w16.cpu().double().view(Cout, 3, 3, Cin).permute(0, 3, 1, 2).contiguous()
+    b = None if bias is None else bias.cpu().double()
+    y = F.conv2d(F.pad(x, (1, 1, 1, 1), mode="circular"), w, b)
+    return y.permute(0, 2, 3, 1).contiguous().float().to(DEV)
 
 
 def rnd(shape, dtype=torch.bfloat16, seed=0, scale=1.0):
@@ -53,9 +56,10 @@ def test_conv3x3_forward(N, H, W, Cin, Cout, relu):
     assert torch.equal(y16, y32.to(torch.bfloat16))
 
 
-def test_conv3x3_gradients():
-    print("case conv grads", flush=True)
-    N, H, W, Cin, Cout = 2, 10, 70, 16, 32
+@pytest.mark.parametrize("Cin,Cout", [(16, 32), (24, 384)])
+def test_conv3x3_gradients(Cin, Cout):
+    print("case conv grads", Cin, Cout, flush=True)
+    N, H, W = 2, 10, 70
     x = rnd((N, H, W, Cin), seed=4)
     w = rnd((Cout, 9, Cin), seed=5, scale=0.2)
     dy = rnd((N, H, W, Cout), seed=6)
@@ -66,14 +70,14 @@ def test_conv3x3_gradients():
     db = torch.empty((Cout,), device=DEV)
     nn.wgrad(x, dy, dw, db)
     torch.cuda.synchronize()
-    xr = x.float().requires_grad_(True)
-    wr = w.float().requires_grad_(True)
-    br = torch.zeros(Cout, device=DEV, requires_grad=True)
+    xr = x.cpu().double().requires_grad_(True)
+    wr = w.cpu().double().requires_grad_(True)
+    br = torch.zeros(Cout, dtype=torch.float64, requires_grad=True)
     y = ref_conv_grad(xr, wr, br)
-    y.backward(dy.float())
-    assert torch.allclose(dx, xr.grad, atol=1e-3, rtol=1e-4)
-    assert torch.allclose(dw, wr.grad, atol=2e-3, rtol=1e-4)
-    assert torch.allclose(db, br.grad, atol=2e-3, rtol=1e-4)
+    y.backward(dy.cpu().double())
+    assert torch.allclose(dx.cpu().double(), xr.grad, atol=1e-3, rtol=1e-4)
+    assert torch.allclose(dw.cpu().double(), wr.grad, atol=2e-3, rtol=1e-4)
+    assert torch.allclose(db.cpu().double(), br.grad, atol=2e-3, rtol=1e-4)
 
 
 def ref_conv_grad(x, w, b):
