@@ -114,14 +114,14 @@ def test_rollout_mass_exact_and_judged(statics):
 def test_trained_network_beats_no_change(statics):
     """What the network gains over the method without it: trained briefly on
     pairs the plain method generates (64x64 maps of the surrogate workload's
-    dynamics, K = 10), it predicts a held-out map's K-cycle height change
+    dynamics, K = 20), it predicts a held-out map's K-cycle height change
     with a smaller RMS error than the no-change prediction."""
     print("case surrogate gain", flush=True)
     import math
 
     from simplewinderosion_amd.world import Batch, glm_normalize
 
-    K, T, n = 10, 60, 5
+    K, T, n = 20, 100, 5
     workloads.apply("surr256", 11)
     b = Batch([(64, 64)] * n, periodic=True, layers=2, abrasion=1e-4, deposit_on_death=True)
     az = [0.3 + 1.1 * k for k in range(n)]
@@ -141,8 +141,8 @@ def test_trained_network_beats_no_change(statics):
     train_states, test_states = states[:, : n - 1], states[:, n - 1:]
     m = S.Surrogate(seed=2)
     S.fit_scales(m, train_states, K)
-    S.train(m, train_states, az[: n - 1], K, steps=400, batch=4, lr=1e-3)
-    h0, h1 = test_states[: T - K, 0], test_states[K:, 0]
+    S.train(m, train_states, az[: n - 1], K, steps=1500, batch=8, lr=2e-3)
+    h0, h1 = test_states[: T - K, 0], test_states[K:T, 0]
     dh, _ = m.predict(h0[:, 0].contiguous(), h0[:, 1].contiguous(), [az[-1]] * (T - K))
     true = h1[:, 0] - h0[:, 0]
     err = float(((dh - true) ** 2).mean().sqrt())
