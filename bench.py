@@ -109,6 +109,37 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 3}), flush=True)
 
 
+def bench_batch(args):
+    """The multi-terrain workload: 64 terrains (sides log-uniform in
+    128..1024, own seed, wind azimuth uniform at 45 degrees elevation,
+    particles per cycle = area) eroded by one launch per window."""
+    from simplewinderosion_amd import workloads
+
+    terrains = workloads.batch64(seed=args.seed)
+    b = workloads.make_batch(terrains)
+    parts = [t["particles"] for t in terrains]
+    print("bench: batch64, %d terrains, %d cells, warmup %d, steps %d"
+          % (len(terrains), sum(parts), args.warmup, args.steps), file=sys.stderr, flush=True)
+    b.erode(parts, n_cycles=args.warmup)
+    b.synchronize()
+    b.reset_stats()
+    t0 = time.perf_counter()
+    b.erode(parts, n_cycles=args.steps)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    st = b.stats()
+    line = {"metric": "particle-steps/s", "value": st["particle_steps"] / el, "unit": "particle-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (int64 fixed-point accumulation)", "data": "synthetic",
+            "config": {"workload": "batch64", "terrains": len(terrains), "cells": sum(parts),
+                       "particles_per_cycle": sum(parts), "window": 1, "layers": 2, "periodic": 1,
+                       "parallelism": "single GPU"},
+            "tiles": {"max_particles_per_tile": st["max_tile_particles"]}}
+    b.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +169,10 @@ def main():
               "terrain or by particles with an int64 all-reduce of the delta map; see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch64":
+        bench_batch(args)
+        return
 
     from simplewinderosion_amd import workloads
 

@@ -342,6 +342,9 @@ class Batch:
         self._check(self.lib.swe_stats(self.ctx, ctypes.byref(s)), "swe_stats")
         return {k: getattr(s, k) for k, _ in s._fields_}
 
+    def reset_stats(self):
+        self._check(self.lib.swe_reset_stats(self.ctx), "swe_reset_stats")
+
     def download(self, k):
         nx, ny = self.sizes[k]
         n = nx * ny
