@@ -49,6 +49,12 @@ int swe_nn_mse(const float* pred, const float* target, float* grad, float* part,
 /* Adam on float master weights w (+ bf16 copy w16, optional); step >= 1 */
 int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                 float eps, int32_t step, void* stream);
+/* Surrogate input features x [N][H][W][8] bf16 from float maps h, s [N][H][W]
+ * (s may be NULL), per-map mean heights and wind azimuths [N]: (h - mean) /
+ * h_scale, s / s_scale, cos az, sin az, periodic central differences along
+ * H and W over g_scale, periodic 5-point Laplacian over l_scale, 0. */
+int swe_nn_features(const float* h, const float* s, const float* mean, const float* az, int32_t N, int32_t H,
+                    int32_t W, float h_scale, float s_scale, float g_scale, float l_scale, void* x, void* stream);
 /* master[i] += q_i with q = round(d * scale * 2^32) corrected so that
  * sum(q) == target exactly (fixed-point mass projection); work: 256 int64 */
 int swe_nn_project_mass(const float* d, int64_t n, float scale, int64_t target, int64_t* master, int64_t* work,

@@ -331,6 +331,36 @@ __global__ void k_adam(float* __restrict__ w, bf16* __restrict__ w16, const floa
 }
 
 // ---------------------------------------------------------------------------
+// Input features of the surrogate, computed in float from the float maps and
+// rounded to bf16 only at the end (the map's slopes are far below the bf16
+// resolution of the heights themselves): x[n][y][x][0..7] = height anomaly,
+// sediment, cos and sin of the wind azimuth, periodic central differences
+// along the two map axes, the periodic 5-point Laplacian, 0.
+// ---------------------------------------------------------------------------
+__global__ void k_features(const float* __restrict__ h, const float* __restrict__ s, const float* __restrict__ mean,
+                           const float* __restrict__ az, int32_t N, int32_t H, int32_t W, float h_scale,
+                           float s_scale, float g_scale, float l_scale, bf16* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)N * H * W) return;
+  const int32_t xx = (int32_t)(i % W), yy = (int32_t)((i / W) % H);
+  const int64_t n = i / ((int64_t)W * H);
+  const float* hn = h + n * H * W;
+  const float c = hn[(int64_t)yy * W + xx];
+  const float up = hn[(int64_t)wrapi(yy + 1, H) * W + xx], dn = hn[(int64_t)wrapi(yy - 1, H) * W + xx];
+  const float rt = hn[(int64_t)yy * W + wrapi(xx + 1, W)], lt = hn[(int64_t)yy * W + wrapi(xx - 1, W)];
+  bf16x8 v;
+  v[0] = f2bf((c - mean[n]) / h_scale);
+  v[1] = f2bf(s ? s[i] / s_scale : 0.0f);
+  v[2] = f2bf(cosf(az[n]));
+  v[3] = f2bf(sinf(az[n]));
+  v[4] = f2bf(0.5f * (up - dn) / g_scale);
+  v[5] = f2bf(0.5f * (rt - lt) / g_scale);
+  v[6] = f2bf(((up + dn) + (rt + lt) - 4.0f * c) / l_scale);
+  v[7] = f2bf(0.0f);
+  *(bf16x8*)(x + i * 8) = v;
+}
+
+// ---------------------------------------------------------------------------
 // Fixed-point mass projection: the predicted change d (float, per cell) is
 // turned into int64 units of 2^-32 whose sum is exactly `target`: q_i =
 // round(d_i * 2^32 * scale) and the residual target - sum(q) spread as evenly
@@ -502,6 +532,14 @@ int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t
   if (step < 1) return -1;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(G(n)), dim3(256), 0, S(stream), w, (bf16*)w16, g, m, v, n, lr, b1, b2, eps, c1, c2);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_features(const float* h, const float* s, const float* mean, const float* az, int32_t N, int32_t H,
+                    int32_t W, float h_scale, float s_scale, float g_scale, float l_scale, void* x, void* stream) {
+  if (!h || !mean || !az || !x || N < 1 || H < 1 || W < 1) return -1;
+  hipLaunchKernelGGL(k_features, dim3(G((int64_t)N * H * W)), dim3(256), 0, S(stream), h, s, mean, az, N, H, W,
+                     h_scale, s_scale, g_scale, l_scale, (bf16*)x);
   SWENN_LAUNCH_CHECK();
 }
 

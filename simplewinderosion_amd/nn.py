@@ -27,6 +27,7 @@ SIGNATURES = {
     "swe_nn_up2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
     "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
+    "swe_nn_features": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _F, _F, _P, _P]),
     "swe_nn_project_mass": (ctypes.c_int, [_P, _I64, _F, _I64, _P, _P, _P]),
 }
 
@@ -172,3 +173,14 @@ def project_mass(d, scale, target, master, work):
     """master (int64, n) += round(d * scale * 2^32) corrected to sum exactly `target`."""
     _ck(load().swe_nn_project_mass(_p(d), d.numel(), float(scale), int(target), _p(master), _p(work), _stream()),
         "swe_nn_project_mass")
+
+
+def features(h, s, mean, az, h_scale, s_scale, g_scale, l_scale):
+    """h, s [N,H,W] float (s may be None), mean, az [N] float -> x [N,H,W,8] bf16."""
+    import torch
+
+    N, H, W = h.shape
+    x = torch.empty((N, H, W, 8), device=h.device, dtype=torch.bfloat16)
+    _ck(load().swe_nn_features(_p(h), _p(s), _p(mean), _p(az), N, H, W, float(h_scale), float(s_scale),
+                               float(g_scale), float(l_scale), _p(x), _stream()), "swe_nn_features")
+    return x

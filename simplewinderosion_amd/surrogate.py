@@ -32,9 +32,9 @@ from . import nn
 # channels per resolution level (level l = 1/2^l of the map); the encoder
 # pools after levels 0 .. L-2, the bottom runs at level L-1
 CH = (32, 64, 128, 192, 256)
-C_IN = 8    # height anomaly, sediment, cos(azimuth), sin(azimuth), zeros
+C_IN = 8    # height anomaly, sediment, cos and sin of the azimuth, two slopes, Laplacian, 0
 C_OUT = 16  # height change, sediment change, unused
-N_IN, N_OUT = 4, 2
+N_IN, N_OUT = 7, 2
 
 
 def _layers():
@@ -85,6 +85,8 @@ class Surrogate:
                 self.v[key] = torch.zeros(shape, device=self.dev)
         self.h_scale = 1.0   # height anomaly scale of the inputs
         self.s_scale = 1.0   # sediment scale of the inputs
+        self.g_scale = 1.0   # scale of the height's central differences
+        self.l_scale = 1.0   # scale of the height's Laplacian
         self.d_scale = 1.0   # scale of the K-cycle height change
         self.ds_scale = 1.0  # scale of the K-cycle sediment change
         self.K = 50
@@ -103,7 +105,8 @@ class Surrogate:
     def state_dict(self):
         d = {"w." + n: self.w32[n].cpu().numpy() for n in self.w32}
         d.update({"b." + n: self.b32[n].cpu().numpy() for n in self.b32})
-        d["scales"] = np.array([self.h_scale, self.s_scale, self.d_scale, self.ds_scale, self.K], np.float64)
+        d["scales"] = np.array([self.h_scale, self.s_scale, self.g_scale, self.l_scale, self.d_scale, self.ds_scale,
+                                self.K], np.float64)
         return d
 
     def load_state_dict(self, d):
@@ -112,7 +115,8 @@ class Surrogate:
             self.w32[n] = torch.from_numpy(np.asarray(d["w." + n], np.float32)).to(self.dev).contiguous()
             self.w16[n] = self.w32[n].to(torch.bfloat16).contiguous()
             self.b32[n] = torch.from_numpy(np.asarray(d["b." + n], np.float32)).to(self.dev).contiguous()
-        self.h_scale, self.s_scale, self.d_scale, self.ds_scale, k = (float(x) for x in d["scales"])
+        (self.h_scale, self.s_scale, self.g_scale, self.l_scale, self.d_scale, self.ds_scale,
+         k) = (float(x) for x in d["scales"])
         self.K = int(k)
 
     def save(self, path):
@@ -128,17 +132,15 @@ class Surrogate:
     # -- network ----------------------------------------------------------------
     def features(self, h, s, az):
         """h, s [N,H,W] float (device), az [N] wind azimuths -> x0 [N,H,W,8]
-        bf16: height anomaly (per map) / h_scale, sediment / s_scale,
-        cos az, sin az."""
+        bf16 (libswe_nn's feature kernel: anomaly, sediment, cos, sin,
+        slopes and Laplacian computed in float)."""
         torch = _torch()
-        N, H, W = h.shape
-        az = torch.as_tensor(az, dtype=torch.float32).to(self.dev).reshape(N, 1, 1)
-        x = torch.zeros((N, H, W, C_IN), device=self.dev, dtype=torch.bfloat16)
-        x[..., 0] = ((h - h.mean(dim=(1, 2), keepdim=True)) / self.h_scale).to(torch.bfloat16)
-        x[..., 1] = (s / self.s_scale).to(torch.bfloat16)
-        x[..., 2] = torch.cos(az).expand(N, H, W).to(torch.bfloat16)
-        x[..., 3] = torch.sin(az).expand(N, H, W).to(torch.bfloat16)
-        return x
+        N = h.shape[0]
+        h = h.contiguous().float()
+        s = s.contiguous().float()
+        mean = h.mean(dim=(1, 2)).contiguous()
+        azt = torch.as_tensor(np.asarray(az, np.float32).reshape(N)).to(self.dev).contiguous()
+        return nn.features(h, s, mean, azt, self.h_scale, self.s_scale, self.g_scale, self.l_scale)
 
     def _buf(self, shape, dtype=None):
         torch = _torch()
@@ -294,12 +296,19 @@ def generate_states(n_terrains, cycles, seed=1, workload="surr256", device=0, lo
 
 
 def fit_scales(model, states, K):
+    """Normalisation scales from (a subsample of at most ~100 times of) the data."""
     T = states.shape[0] - K
-    h, s = states[:T, :, 0], states[:T, :, 1]
+    step = max(1, T // 100)
+    h, s = states[:T:step, :, 0], states[:T:step, :, 1]
+    torch = _torch()
     model.h_scale = float((h - h.mean(dim=(2, 3), keepdim=True)).std())
     model.s_scale = float(s.std()) or 1.0
-    model.d_scale = float((states[K:, :, 0] - h).std())
-    model.ds_scale = float((states[K:, :, 1] - s).std()) or 1.0
+    gx = 0.5 * (torch.roll(h, -1, dims=2) - torch.roll(h, 1, dims=2))
+    lap = torch.roll(h, -1, dims=2) + torch.roll(h, 1, dims=2) + torch.roll(h, -1, dims=3) + torch.roll(h, 1, dims=3) - 4 * h
+    model.g_scale = float(gx.std()) or 1.0
+    model.l_scale = float(lap.std()) or 1.0
+    model.d_scale = float((states[K:K + T:step, :, 0] - h).std())
+    model.ds_scale = float((states[K:K + T:step, :, 1] - s).std()) or 1.0
     model.K = K
 
 
@@ -447,6 +456,7 @@ def main():
     ap.add_argument("--tol-rms", type=float, default=1e-3)
     ap.add_argument("--tol-noise", type=float, default=None,
                     help="accept blocks within this multiple of the plain method's own run-to-run RMS")
+    ap.add_argument("--rollout", default="surr1024", help="comma-separated workloads to roll out on")
     ap.add_argument("--save", default=None)
     ap.add_argument("--load", default=None, help="skip data and training: use a saved model")
     ap.add_argument("--out", default=None)
@@ -487,20 +497,22 @@ def main():
     torch.cuda.synchronize()
     ti = (time.perf_counter() - t0) / 10
     res["inference_1024"] = {"seconds": ti, "tflops": Surrogate.flops_forward(1, 1024, 1024) / ti / 1e12}
-    # rollout on the 1024^2 map with obstacles and its plain twin
-    world = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
-    plain = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
-    ps = workloads.WORKLOADS["surr1024"]["params"]["pspeed"]
-    azimuth = math.atan2(ps[2], ps[0])
-    n = workloads.WORKLOADS["surr1024"]["particles"]
-    rec = rollout(model, world, plain, n, args.blocks, azimuth, judge_every=args.judge_every, tol_rms=args.tol_rms,
-                  tol_noise=args.tol_noise, log=log)
-    rec["map_cycles_per_s"] = args.blocks * model.K / rec["t_total_s"]
-    rec["plain_map_cycles_per_s"] = model.K / rec["plain_seconds_per_block"]
-    rec["speedup_vs_plain"] = rec["map_cycles_per_s"] / rec["plain_map_cycles_per_s"]
-    res["rollout"] = rec
-    world.close()
-    plain.close()
+    # rollouts: the 1024^2 map with obstacles (and optionally a 256^2 map of
+    # the training kind with an unseen seed), each with its plain twin
+    for wl in args.rollout.split(","):
+        world = workloads.make_world(wl, seed=7, deposit_on_death=True)
+        plain = workloads.make_world(wl, seed=7, deposit_on_death=True)
+        ps = workloads.WORKLOADS[wl]["params"]["pspeed"]
+        azimuth = math.atan2(ps[2], ps[0])
+        n = workloads.WORKLOADS[wl]["particles"]
+        rec = rollout(model, world, plain, n, args.blocks, azimuth, judge_every=args.judge_every,
+                      tol_rms=args.tol_rms, tol_noise=args.tol_noise, log=log)
+        rec["map_cycles_per_s"] = args.blocks * model.K / rec["t_total_s"]
+        rec["plain_map_cycles_per_s"] = model.K / rec["plain_seconds_per_block"]
+        rec["speedup_vs_plain"] = rec["map_cycles_per_s"] / rec["plain_map_cycles_per_s"]
+        res["rollout" if wl == "surr1024" else "rollout_" + wl] = rec
+        world.close()
+        plain.close()
     line = json.dumps(res)
     if args.out:
         with open(args.out, "w") as f:

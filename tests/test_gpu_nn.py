@@ -158,3 +158,25 @@ def test_mass_projection_exact(target):
     assert sum(int(v) for v in got) == target
     q = np.rint(d.cpu().numpy().astype(np.float32) * np.float32(2.0 ** 32)).astype(np.int64)
     assert np.abs(got - q).max() <= abs(target - int(q.sum())) // n + 1
+
+
+def test_features_match_float64():
+    print("case features", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(14)
+    h = torch.rand((2, 16, 24), generator=g, dtype=torch.float64)
+    s = torch.rand((2, 16, 24), generator=g, dtype=torch.float64) * 0.1
+    az = torch.tensor([0.3, 2.5], dtype=torch.float64)
+    mean = h.mean(dim=(1, 2))
+    x = nn.features(h.float().to(DEV), s.float().to(DEV), mean.float().to(DEV), az.float().to(DEV), 0.2, 0.05, 0.01,
+                    0.02).cpu().double()
+    r = lambda a, k, d: torch.roll(a, k, dims=d)
+    ref = torch.zeros((2, 16, 24, 8), dtype=torch.float64)
+    ref[..., 0] = (h - mean[:, None, None]) / 0.2
+    ref[..., 1] = s / 0.05
+    ref[..., 2] = torch.cos(az)[:, None, None]
+    ref[..., 3] = torch.sin(az)[:, None, None]
+    ref[..., 4] = 0.5 * (r(h, -1, 1) - r(h, 1, 1)) / 0.01
+    ref[..., 5] = 0.5 * (r(h, -1, 2) - r(h, 1, 2)) / 0.01
+    ref[..., 6] = (r(h, -1, 1) + r(h, 1, 1) + r(h, -1, 2) + r(h, 1, 2) - 4 * h) / 0.02
+    # bf16 keeps 8 significant bits; inputs were rounded to float32 first
+    assert torch.allclose(x, ref, rtol=1e-2, atol=1e-2)
