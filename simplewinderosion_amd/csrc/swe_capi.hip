@@ -72,7 +72,8 @@ struct swe_ctx {
   // tile-owned step path
   uint32_t* fV[2] = {nullptr, nullptr};
   int64_t fcap = 0;
-  swe::ParticleBuf PB[2] = {}, FB[3] = {};
+  swe::PRec* PB[2] = {};  // particle records, ping-pong by window parity
+  swe::PRec* FB[3] = {};  // far lists
   uint32_t* farcnt = nullptr;  // 3
   int64_t* sbase[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range starts
   uint32_t* scnt[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range lengths
@@ -187,22 +188,6 @@ int ensure_particles(swe_ctx* c, int64_t n) {
   return 0;
 }
 
-void free_pbuf(swe::ParticleBuf& b) {
-  dfree(b.P0);
-  dfree(b.P1);
-  dfree(b.L);
-  dfree(b.LC);
-  dfree(b.PID);
-}
-
-int alloc_pbuf(swe_ctx* c, swe::ParticleBuf& b, int64_t n) {
-  int r;
-  if ((r = dalloc(c, &b.P0, n)) || (r = dalloc(c, &b.P1, n)) || (r = dalloc(c, &b.L, n)) ||
-      (r = dalloc(c, &b.LC, n)) || (r = dalloc(c, &b.PID, n)))
-    return r;
-  return 0;
-}
-
 // buffers of the tile-owned step path for n particles
 int ensure_step(swe_ctx* c, int64_t n) {
   int r;
@@ -218,17 +203,17 @@ int ensure_step(swe_ctx* c, int64_t n) {
       return r;
   }
   if (n > c->fcap) {
-    for (int k = 0; k < 2; k++) free_pbuf(c->PB[k]);
-    for (int k = 0; k < 3; k++) free_pbuf(c->FB[k]);
+    for (int k = 0; k < 2; k++) dfree(c->PB[k]);
+    for (int k = 0; k < 3; k++) dfree(c->FB[k]);
     dfree(c->sp_tile);
     dfree(c->sp_rank);
     c->fcap = 0;
     // a tile's output region has room for all its particles in each of its
     // nine direction sub-ranges
     for (int k = 0; k < 2; k++)
-      if ((r = alloc_pbuf(c, c->PB[k], 9 * n))) return r;
+      if ((r = dalloc(c, &c->PB[k], 9 * n))) return r;
     for (int k = 0; k < 3; k++)
-      if ((r = alloc_pbuf(c, c->FB[k], n))) return r;
+      if ((r = dalloc(c, &c->FB[k], n))) return r;
     if ((r = dalloc(c, &c->sp_tile, n)) || (r = dalloc(c, &c->sp_rank, n))) return r;
     c->fcap = n;
   }
@@ -511,7 +496,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   if (graph) {
     key.append((const char*)&c->p, sizeof(c->p));
     key.append((const char*)particles, sizeof(int64_t) * (size_t)c->nterr);
-    const void* ptrs[2] = {(const void*)c->PB[0].P0, (const void*)c->trk};
+    const void* ptrs[2] = {(const void*)c->PB[0], (const void*)c->trk};
     key.append((const char*)ptrs, sizeof(ptrs));
     key.append((const char*)&c->tile_order, sizeof(c->tile_order));
     key.append((const char*)&nwin, sizeof(nwin));
@@ -600,9 +585,9 @@ int swe_destroy(swe_ctx* c) {
     dfree(c->fV[k]);
     dfree(c->sbase[k]);
     dfree(c->scnt[k]);
-    free_pbuf(c->PB[k]);
+    dfree(c->PB[k]);
   }
-  for (int k = 0; k < 3; k++) free_pbuf(c->FB[k]);
+  for (int k = 0; k < 3; k++) dfree(c->FB[k]);
   dfree(c->trk);
   dfree(c->ema);
   dfree(c->dz);

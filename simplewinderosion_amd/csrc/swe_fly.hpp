@@ -7,6 +7,7 @@
 #pragma once
 
 #include "swe_device.hpp"
+#include "swe_fdiv.h"
 
 namespace swe {
 
@@ -25,7 +26,19 @@ struct FlyConsts {
   int32_t max_age;
   float boundarylayer, suspension, gravity, abrasion;
   f3 ps, nps;  // pspeed and normalize(pspeed)
+  // boundarylayer a power of two with a normal reciprocal: dividing by it
+  // and multiplying by rcp_bl round the same real number, so give one result
+  bool bl_pow2;
+  float rcp_bl;
 };
+
+// Fills bl_pow2 / rcp_bl from boundarylayer.
+__device__ __forceinline__ void set_boundarylayer(FlyConsts& k, float bl) {
+  k.boundarylayer = bl;
+  const uint32_t u = f2u(bl), e = (u >> 23) & 0xffu;
+  k.bl_pow2 = (u & 0x807fffffu) == 0u && e >= 1u && e <= 253u;
+  k.rcp_bl = k.bl_pow2 ? u2f((254u - e) << 23) : 0.0f;
+}
 
 // Outcome of one call.
 enum FlyOutcome { kStepped = 0, kDied = 1 };
@@ -111,7 +124,8 @@ __device__ __forceinline__ FlyOutcome fly(Particle& w, uint64_t pk, int32_t cx, 
   const int32_t call = w.age;
   w.age = w.age + 1;
 
-  float hfac = exp_f32(-(w.py - h) / k.boundarylayer);  // :50-52
+  const float hrel = -(w.py - h);
+  float hfac = exp_f32(k.bl_pow2 ? hrel * k.rcp_bl : hrel / k.boundarylayer);  // :50-52
   if (hfac < 0.0f) hfac = 0.0f;
 
   float shadow = dot(k.nps, n);  // :56-59
@@ -136,7 +150,8 @@ __device__ __forceinline__ FlyOutcome fly(Particle& w, uint64_t pk, int32_t cx, 
     const float r1 = (float)(int32_t)(((t >> 21) & 0x1fffffull) % 1001ull);
     const float r2 = (float)(int32_t)(((t >> 42) & 0x1fffffull) % 1001ull);
     const float kk = 0.1f * hfac * collision;
-    sp = add(sp, divs(smul(kk, subs(mk(r0, r1, r2), 500.0f)), 500.0f));
+    const f3 tb = smul(kk, subs(mk(r0, r1, r2), 500.0f));
+    sp = add(sp, mk(swe_div500(tb.x), swe_div500(tb.y), swe_div500(tb.z)));
   }
 
   sp = muls(sp, (float)(1.0 - 0.3 * (double)w.sed));  // :87 (double, then float)

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_march(MarchArgs a) {
     k.rcp_ny = a.rcp_ny;
     k.mapscale = a.mapscale;
     k.max_age = a.p.max_age;
-    k.boundarylayer = a.p.boundarylayer;
+    set_boundarylayer(k, a.p.boundarylayer);
     k.suspension = a.p.suspension;
     k.gravity = a.p.gravity;
     k.abrasion = a.p.abrasion;

@@ -37,14 +37,16 @@ struct Terrain {
   float ps[3], nps[3];  // pspeed and normalize(pspeed)
 };
 
-// One particle buffer (structure of arrays)
-struct ParticleBuf {
-  float4* P0;     // pos.xyz, sediment
-  float4* P1;     // speed.xyz, age bits
-  int64_t* L;     // fixed-point load
-  int32_t* LC;    // last pre-move cell
-  uint32_t* PID;  // particle index (RNG stream)
+// One particle record: three 16-byte words, so a record moves with three
+// vector loads or stores at immediate offsets from one address
+struct alignas(16) PRec {
+  float4 q0;       // pos.xyz, sediment
+  float4 q1;       // speed.xyz, age bits
+  int64_t ledger;  // fixed-point load
+  int32_t lastc;   // last pre-move cell
+  uint32_t pid;    // particle index (RNG stream)
 };
+static_assert(sizeof(PRec) == 48, "particle record is three 16-byte words");
 
 struct StepArgs {
   // map, window parity in -> out.  A master holds the previous window's
@@ -64,7 +66,10 @@ struct StepArgs {
   int64_t* lost;         // kLostSlots counters
   uint64_t* stat;        // kStatKinds x kStatSlots statistics
   // particles
-  ParticleBuf in, out, far_in, far_out;
+  const PRec* in;
+  PRec* out;
+  const PRec* far_in;
+  PRec* far_out;
   uint32_t* far_cnt_in;
   uint32_t* far_cnt_out;
   uint32_t* far_cnt_clear;   // the far counter of the window after next
@@ -92,7 +97,7 @@ struct StepArgs {
 };
 
 struct SpawnArgs {
-  ParticleBuf out;
+  PRec* out;
   int32_t* tile;     // per particle
   uint32_t* rank;    // per particle
   uint32_t* count;   // per tile, zero on entry: window 0's input counts

@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     k.rcp_ny = rcp_ny;
     k.mapscale = a.mapscale;
     k.max_age = a.max_age;
-    k.boundarylayer = a.boundarylayer;
+    set_boundarylayer(k, a.boundarylayer);
     k.suspension = a.suspension;
     k.gravity = a.gravity;
     k.abrasion = a.abrasion;
@@ -423,21 +423,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     const uint32_t far_n = *a.far_cnt_in;
 
     // one input record
-    struct Rec {
-      float4 q0, q1;
-      int64_t ledger;
-      int32_t lastc;
-      uint32_t pid;
-    };
-    auto load = [](const ParticleBuf& in, int64_t j) __attribute__((always_inline)) {
-      Rec r;
-      r.q0 = in.P0[j];
-      r.q1 = in.P1[j];
-      r.ledger = in.L[j];
-      r.lastc = in.LC[j];
-      r.pid = in.PID[j];
-      return r;
-    };
+    using Rec = PRec;
+    auto load = [](const PRec* in, int64_t j) __attribute__((always_inline)) { return in[j]; };
     // a particle more than one tile away goes to the far list; the tile of
     // cell (fx, fy) counts it among its next window's arrivals
     auto send_far = [&](float4 o0, float4 o1, int64_t ledger, int32_t lastc, uint32_t pid, int32_t fx,
@@ -446,11 +433,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       atomicAdd(&a.nin_next[dest], 1u);
       atomicAdd(&a.csum_next[dest / kChunk], 1u);
       const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
-      a.far_out.P0[fslot] = o0;
-      a.far_out.P1[fslot] = o1;
-      a.far_out.L[fslot] = ledger;
-      a.far_out.LC[fslot] = lastc;
-      a.far_out.PID[fslot] = pid;
+      a.far_out[fslot] = PRec{o0, o1, ledger, lastc, pid};
     };
     // step one record and write it out
     auto process = [&](const Rec& r) __attribute__((always_inline)) {
@@ -526,11 +509,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       }
       // sub-range `code` of this tile's output region holds up to nin_u records
       const int64_t o = out_base + (int64_t)code * nin_u + atomicAdd(&s_slot[code], 1u);
-      a.out.P0[o] = o0;
-      a.out.P1[o] = o1;
-      a.out.L[o] = w.ledger;
-      a.out.LC[o] = w.lastc;
-      a.out.PID[o] = pid;
+      a.out[o] = PRec{o0, o1, w.ledger, w.lastc, pid};
     };
 
     // the arrivals, e = 0 .. n_in - 1 over the nine sub-ranges in order; each
@@ -553,16 +532,16 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
-      const int64_t fp = (int64_t)a.far_in.PID[j];
+      const int64_t fp = (int64_t)a.far_in[j].pid;
       if (fp < p0 || fp >= p0 + T.npart) continue;  // another terrain's particle
       int32_t cx, cy;
-      const float4 q0 = a.far_in.P0[j];
+      const float4 q0 = a.far_in[j].q0;
       int32_t ctx, cty;
       if (cell_of<PERIODIC>(q0.x, q0.z, k, cx, cy)) {
         ctx = cx / kSX;
         cty = cy / kSY;
       } else {
-        const int32_t lc = a.far_in.LC[j];
+        const int32_t lc = a.far_in[j].lastc;
         ctx = (lc / ny) / kSX;
         cty = (lc % ny) / kSY;
       }
@@ -750,11 +729,8 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
   Particle w;
   spawn(particle_key(cycle_key(T.seed, *a.cycle), (uint64_t)(i - T.part0)), T.nx, T.ny, a.spawn_mode, w);
   const int64_t o = a.sub_base[(int64_t)a.tile[i] * 9 + 4] + a.rank[i];
-  a.out.P0[o] = make_float4(w.px, w.py, w.pz, w.sed);
-  a.out.P1[o] = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
-  a.out.L[o] = 0;
-  a.out.LC[o] = 0;
-  a.out.PID[o] = (uint32_t)i;
+  a.out[o] = PRec{make_float4(w.px, w.py, w.pz, w.sed), make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age)), 0, 0,
+                  (uint32_t)i};
 }
 
 // ---------------------------------------------------------------------------
