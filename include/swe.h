@@ -97,6 +97,7 @@ typedef struct swe_stats_t {
   uint64_t max_tile_particles;  /* most particles one step-kernel tile held in a window */
   uint64_t crowded_tile_windows;/* (tile, window) pairs above the tile's LDS list capacity */
   uint64_t far_particles;       /* particle moves that skipped more than one tile */
+  uint64_t heavy_tile_windows;  /* (tile, window) pairs run first as crowded (SWE_TUNE_HEAVY_FIRST) */
 } swe_stats_t;
 
 /* Reference defaults (wind.hpp:13,23-26; world.hpp:30-32; cellpool.h:184). */
@@ -160,8 +161,12 @@ int swe_set_path(swe_ctx* ctx, int32_t path);
  *     kernel launches into a HIP graph once (per starting window parity and
  *     launch configuration) and launches the graph each cycle; 0 = launches
  *     each kernel.  Timing (swe_set_timing) uses individual launches.
+ *   SWE_TUNE_HEAVY_FIRST: 1 (default) = a tile that received more than 1024
+ *     particles in one window is handed to one of the first workgroups of
+ *     the next window's launch, so crowded tiles start first instead of
+ *     finishing last; 0 = every tile in the SWE_TUNE_TILE_ORDER order.
  */
-enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2 };
+enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2, SWE_TUNE_HEAVY_FIRST = 3 };
 int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);
 int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
 
@@ -169,7 +174,8 @@ int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
  * Autotuner (no reference counterpart): times `trial_cycles` map-cycles of
  * the context's own workload (particles per terrain, as for
  * swe_erode_batch) for every tile order in {0, 1, 4, 16} with and without
- * graphs, each from the same saved state, then restores the state (map,
+ * graphs (and, on maps of 64 tiles or more, with and without
+ * SWE_TUNE_HEAVY_FIRST), each from the same saved state, then restores the state (map,
  * fields, counters and statistics are as before the call) and keeps the
  * fastest setting.  Tile step path (window == 1) only.
  */
@@ -177,7 +183,7 @@ typedef struct swe_tune_result {
   int32_t tile_order;
   int32_t graph;
   int32_t candidates;    /* settings timed */
-  int32_t reserved;
+  int32_t heavy_first;
   double ms_per_cycle;   /* of the setting kept */
 } swe_tune_result;
 int swe_autotune(swe_ctx* ctx, const int64_t* particles, int32_t trial_cycles, swe_tune_result* out);

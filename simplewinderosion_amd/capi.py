@@ -19,6 +19,7 @@ SWE_ESTATE = -5
 
 SWE_TUNE_TILE_ORDER = 1
 SWE_TUNE_GRAPH = 2
+SWE_TUNE_HEAVY_FIRST = 3
 
 
 class SweParams(ctypes.Structure):
@@ -93,6 +94,7 @@ class SweStats(ctypes.Structure):
         ("max_tile_particles", ctypes.c_uint64),
         ("crowded_tile_windows", ctypes.c_uint64),
         ("far_particles", ctypes.c_uint64),
+        ("heavy_tile_windows", ctypes.c_uint64),
     ]
 
 
@@ -101,7 +103,7 @@ class SweTuneResult(ctypes.Structure):
         ("tile_order", ctypes.c_int32),
         ("graph", ctypes.c_int32),
         ("candidates", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("heavy_first", ctypes.c_int32),
         ("ms_per_cycle", ctypes.c_double),
     ]
 

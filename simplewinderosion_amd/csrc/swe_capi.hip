@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <array>
 #include <cstring>
 #include <string>
@@ -34,6 +35,7 @@ struct swe_ctx {
   int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
   int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
   int32_t use_graph = 1;   // SWE_TUNE_GRAPH
+  int32_t heavy_first = 1; // SWE_TUNE_HEAVY_FIRST
   // one captured map-cycle of the tile path per starting window parity, and
   // the launch configuration it was captured for
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -79,6 +81,9 @@ struct swe_ctx {
   uint32_t* scnt[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range lengths
   uint32_t* nin = nullptr;     // 3 x tiles
   uint32_t* csum = nullptr;    // 3 x chunks of kStepChunk tiles
+  uint32_t* heavy[2] = {nullptr, nullptr};  // kHeavySlotsMax: crowded tiles listed for a window
+  uint32_t* hcnt = nullptr;    // 3: their counts
+  uint32_t* hstamp = nullptr;  // per tile: window + 1 when listed
   int32_t* sp_tile = nullptr;
   uint32_t* sp_rank = nullptr;
 
@@ -199,7 +204,9 @@ int ensure_step(swe_ctx* c, int64_t n) {
         (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->sbase[0], 9 * nt)) ||
         (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
         (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
-        (r = dalloc(c, &c->csum, 3 * nch)))
+        (r = dalloc(c, &c->csum, 3 * nch)) || (r = dalloc(c, &c->heavy[0], swe::kHeavySlotsMax)) ||
+        (r = dalloc(c, &c->heavy[1], swe::kHeavySlotsMax)) || (r = dalloc(c, &c->hcnt, 3)) ||
+        (r = dalloc(c, &c->hstamp, nt)))
       return r;
   }
   if (n > c->fcap) {
@@ -413,6 +420,9 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sa.settling = c->p.settling;
   sa.max_age = c->p.max_age;
   sa.tile_order = c->tile_order;
+  // listed crowded tiles get the first workgroups of a launch; at most one
+  // slot per 8 tiles, a multiple of 8 (none on maps under 64 tiles)
+  sa.heavy_slots = c->heavy_first ? (int32_t)std::min<int64_t>(swe::kHeavySlotsMax, (c->ntiles / 8) & ~7LL) : 0;
 
   const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
   swe::SpawnArgs sp{};
@@ -439,6 +449,10 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
       swe::launch_zero32(c->nin, 3 * nt, c->stream);
       swe::launch_zero32(c->csum, 3 * nch, c->stream);
       swe::launch_zero32(c->farcnt, 3, c->stream);
+      if (sa.heavy_slots > 0) {
+        swe::launch_zero32(c->hcnt, 3, c->stream);
+        swe::launch_zero32(c->hstamp, nt, c->stream);
+      }
       swe::launch_spawn(sp, c->stream);
       HIPCHK(c, hipGetLastError());
       // windows 0 .. nwin-1 step the particles; window nwin only relaxes
@@ -473,6 +487,13 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.csum_cur = c->csum + w3 * nch;
         sa.csum_next = c->csum + w3n * nch;
         sa.csum_clear = c->csum + w3c * nch;
+        sa.window = (int32_t)w;
+        sa.heavy_cur = c->heavy[wp];
+        sa.heavy_next = c->heavy[wp ^ 1];
+        sa.hcnt_cur = c->hcnt + w3;
+        sa.hcnt_next = c->hcnt + w3n;
+        sa.hcnt_clear = c->hcnt + w3c;
+        sa.hstamp = c->hstamp;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int rr;
         if ((rr = timed_begin(c, &e0, &e1))) return rr;
@@ -499,6 +520,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     const void* ptrs[2] = {(const void*)c->PB[0], (const void*)c->trk};
     key.append((const char*)ptrs, sizeof(ptrs));
     key.append((const char*)&c->tile_order, sizeof(c->tile_order));
+    key.append((const char*)&c->heavy_first, sizeof(c->heavy_first));
     key.append((const char*)&nwin, sizeof(nwin));
   }
   const uint64_t windows_per_cycle = total > 0 ? (uint64_t)nwin + 1 : 0;
@@ -599,6 +621,10 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->farcnt);
   dfree(c->nin);
   dfree(c->csum);
+  dfree(c->heavy[0]);
+  dfree(c->heavy[1]);
+  dfree(c->hcnt);
+  dfree(c->hstamp);
   dfree(c->sp_tile);
   dfree(c->sp_rank);
   dfree(c->lost);
@@ -933,15 +959,19 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
     if (e0) (void)hipEventDestroy(e0);
     return fail(c, SWE_EHIP, std::string("autotune set-up: ") + hipGetErrorString(he));
   }
-  const int32_t order0 = c->tile_order, graph0 = c->use_graph, timing0 = c->timing;
+  const int32_t order0 = c->tile_order, graph0 = c->use_graph, heavy0 = c->heavy_first, timing0 = c->timing;
   c->timing = 0;
   static const int32_t kOrders[] = {0, 1, 4, 16};
   double best = 1e300;
-  int32_t best_order = order0, best_graph = graph0, ncand = 0;
+  int32_t best_order = order0, best_graph = graph0, best_heavy = heavy0, ncand = 0;
+  // heavy-first only changes anything where a launch has heavy slots
+  const int32_t heavy_lo = c->ntiles / 8 >= 8 ? 0 : heavy0, heavy_hi = c->ntiles / 8 >= 8 ? 1 : heavy0;
+  for (int32_t hv = heavy_hi; hv >= heavy_lo && r == 0 && he == hipSuccess; hv--)
   for (int32_t graph = 0; graph <= 1 && r == 0; graph++) {
     for (int32_t order : kOrders) {
       c->tile_order = order;
       c->use_graph = graph;
+      c->heavy_first = hv;
       // one untimed cycle (graph capture, caches), then the timed ones, each
       // from the saved state
       if ((he = save_or_restore(false)) != hipSuccess) break;
@@ -958,6 +988,7 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
         best = per;
         best_order = order;
         best_graph = graph;
+        best_heavy = hv;
       }
     }
     if (he != hipSuccess) break;
@@ -971,10 +1002,12 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
   c->timing = timing0;
   c->tile_order = r == 0 && he == hipSuccess ? best_order : order0;
   c->use_graph = r == 0 && he == hipSuccess ? best_graph : graph0;
+  c->heavy_first = r == 0 && he == hipSuccess ? best_heavy : heavy0;
   if (r) return r;
   if (he != hipSuccess) return fail(c, SWE_EHIP, std::string("autotune: ") + hipGetErrorString(he));
   out->tile_order = c->tile_order;
   out->graph = c->use_graph;
+  out->heavy_first = c->heavy_first;
   out->candidates = ncand;
   out->ms_per_cycle = best;
   return 0;
@@ -1006,6 +1039,10 @@ int swe_set_tune(swe_ctx* c, int32_t knob, int64_t value) {
       if (value != 0 && value != 1) return fail(c, SWE_EINVAL, "graph mode must be 0 or 1");
       c->use_graph = (int32_t)value;
       return 0;
+    case SWE_TUNE_HEAVY_FIRST:
+      if (value != 0 && value != 1) return fail(c, SWE_EINVAL, "heavy-first must be 0 or 1");
+      c->heavy_first = (int32_t)value;
+      return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");
   }
@@ -1019,6 +1056,9 @@ int swe_get_tune(swe_ctx* c, int32_t knob, int64_t* value) {
       return 0;
     case SWE_TUNE_GRAPH:
       *value = c->use_graph;
+      return 0;
+    case SWE_TUNE_HEAVY_FIRST:
+      *value = c->heavy_first;
       return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");
@@ -1127,11 +1167,13 @@ int swe_stats(swe_ctx* c, swe_stats_t* out) {
   HIPCHK(c, hipMemcpyAsync(h, c->dstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   out->particle_steps = out->max_tile_particles = out->crowded_tile_windows = out->far_particles = 0;
+  out->heavy_tile_windows = 0;
   for (int q = 0; q < swe::kStatSlots; q++) {
     out->particle_steps += h[swe::kStatSteps * swe::kStatSlots + q];
     const uint64_t m = h[swe::kStatMaxTile * swe::kStatSlots + q];
     if (m > out->max_tile_particles) out->max_tile_particles = m;
     out->crowded_tile_windows += h[swe::kStatCrowded * swe::kStatSlots + q];
+    out->heavy_tile_windows += h[swe::kStatHeavy * swe::kStatSlots + q];
     out->far_particles += h[swe::kStatFar * swe::kStatSlots + q];
   }
   return 0;

@@ -94,7 +94,28 @@ struct StepArgs {
   // workgroup -> tile order: 0 = each XCD a contiguous range of tiles;
   // C >= 1 = runs of C consecutive tiles dealt round-robin over the XCDs
   int32_t tile_order;
+  // crowded tiles first: a tile with more than kHeavyMin arrivals in one
+  // window is listed for the next launch, whose first heavy_slots
+  // workgroups take the listed tiles (so they start at once instead of
+  // finishing last); the workgroup of the usual order then skips the tile
+  int32_t heavy_slots;       // 0 = off, else a multiple of 8 (XCD mapping)
+  int32_t window;            // window index within the cycle
+  const uint32_t* heavy_cur; // this launch's listed tiles
+  uint32_t* heavy_next;      // the next launch's (appended here)
+  const uint32_t* hcnt_cur;  // list lengths (may exceed heavy_slots)
+  uint32_t* hcnt_next;
+  uint32_t* hcnt_clear;      // the length of the list after next
+  uint32_t* hstamp;          // per tile: w + 1 when listed for window w (0 at cycle start)
 };
+
+#ifndef SWE_HEAVY_MIN
+#define SWE_HEAVY_MIN 1024
+#endif
+#ifndef SWE_HEAVY_SLOTS
+#define SWE_HEAVY_SLOTS 256
+#endif
+constexpr uint32_t kHeavyMin = SWE_HEAVY_MIN;        // arrivals that make a tile heavy
+constexpr int32_t kHeavySlotsMax = SWE_HEAVY_SLOTS;  // listed tiles per launch (a multiple of 8)
 
 struct SpawnArgs {
   PRec* out;

@@ -104,16 +104,29 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   // lines in one L2.  tile_order C >= 1: runs of C consecutive tiles are
   // dealt round-robin over the XCDs (evens out crowded regions); the last
   // nb mod 8C workgroups keep their own index.  Both are bijections.
-  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  // The first heavy_slots workgroups take the tiles listed as crowded by the
+  // previous launch (empty slots return at once); the rest follow the
+  // usual order and skip a listed tile.  heavy_slots is a multiple of 8, so
+  // b and b - heavy_slots fall on the same XCD.
+  const uint32_t hs = RELAX_ONLY ? 0u : (uint32_t)a.heavy_slots;
+  const uint32_t nb = gridDim.x - (uint32_t)a.heavy_slots, b = blockIdx.x - (uint32_t)a.heavy_slots;
   const uint32_t xcd = b % 8;
   int32_t u;  // global tile
-  if (a.tile_order <= 0) {
+  bool listed = false;
+  if (blockIdx.x < (uint32_t)a.heavy_slots) {
+    if (hs == 0 || blockIdx.x >= *a.hcnt_cur) return;
+    u = (int32_t)a.heavy_cur[blockIdx.x];
+    listed = true;
+  } else if (a.tile_order <= 0) {
     const uint32_t q8 = nb / 8, r8 = nb % 8;
     u = (int32_t)(xcd * q8 + (xcd < r8 ? xcd : r8) + b / 8);
   } else {
     const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C), j = b / 8;
     u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
   }
+  // a listed tile's stamp is window + 1 (or window + 2 once its heavy
+  // workgroup has listed it again): either way the usual one skips it
+  if (hs > 0 && !listed && a.hstamp[u] > (uint32_t)a.window) return;
   // the terrain this tile belongs to; all map indices below are terrain-local
   const Terrain T = a.terr[a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
@@ -597,7 +610,19 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   if (threadIdx.x == 0) {
     a.nin_clear[u] = 0;  // the window after next accumulates here
     if (u % kChunk == 0) a.csum_clear[u / kChunk] = 0;
-    if (u == 0) *a.far_cnt_clear = 0;
+    if (u == 0) {
+      *a.far_cnt_clear = 0;
+      if (hs > 0) *a.hcnt_clear = 0;
+    }
+    // crowded now: listed for the next launch (the count moves little from
+    // one window to the next)
+    if (hs > 0 && nin_u > kHeavyMin) {
+      const uint32_t slot = atomicAdd(a.hcnt_next, 1u);
+      if (slot < hs) {
+        a.heavy_next[slot] = (uint32_t)u;
+        a.hstamp[u] = (uint32_t)a.window + 2u;
+      }
+    }
   }
   if (!DEPOSIT) {
     for (int off = 32; off > 0; off >>= 1) lost_here += __shfl_xor(lost_here, off, 64);
@@ -635,6 +660,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if (far) atomicAdd((unsigned long long*)&a.stat[kStatFar * kStatSlots + slot], (unsigned long long)far);
     if (nin_u) atomicMax((unsigned long long*)&a.stat[kStatMaxTile * kStatSlots + slot], (unsigned long long)nin_u);
     if (nin_u > kCrowded) atomicAdd((unsigned long long*)&a.stat[kStatCrowded * kStatSlots + slot], 1ull);
+    if (listed) atomicAdd((unsigned long long*)&a.stat[kStatHeavy * kStatSlots + slot], 1ull);
   }
   __syncthreads();
   if (!DEPOSIT && threadIdx.x == 0) {
@@ -738,7 +764,7 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
 // ---------------------------------------------------------------------------
 template <bool P, bool T, int L, bool D>
 static void step4(const StepArgs& a, bool relax_only, hipStream_t s) {
-  const unsigned g = (unsigned)a.ntiles;
+  const unsigned g = (unsigned)(a.ntiles + a.heavy_slots);
   if (relax_only)
     hipLaunchKernelGGL((k_step<P, T, L, D, true>), dim3(g), dim3(kStepThreads), 0, s, a);
   else

@@ -43,7 +43,7 @@ def _load(path):
 
 def autotune(target, particles, trial_cycles=2, cache=DEFAULT_CACHE, retune=False):
     """Tune a World or Batch for `particles` (an int, or one per terrain);
-    returns {"tile_order", "graph", "ms_per_cycle", "candidates", "cached"}."""
+    returns {"tile_order", "graph", "heavy_first", "ms_per_cycle", "candidates", "cached"}."""
     if isinstance(particles, int):
         particles = [particles]
     key = _key(target, particles)
@@ -52,13 +52,14 @@ def autotune(target, particles, trial_cycles=2, cache=DEFAULT_CACHE, retune=Fals
         hit = table[key]
         target.set_tune(capi.SWE_TUNE_TILE_ORDER, hit["tile_order"])
         target.set_tune(capi.SWE_TUNE_GRAPH, hit["graph"])
+        target.set_tune(capi.SWE_TUNE_HEAVY_FIRST, hit.get("heavy_first", 1))
         return dict(hit, cached=True)
     target._push()
     arr = (ctypes.c_int64 * len(particles))(*particles)
     res = capi.SweTuneResult()
     target._check(target.lib.swe_autotune(target.ctx, arr, int(trial_cycles), ctypes.byref(res)), "swe_autotune")
-    out = {"tile_order": res.tile_order, "graph": res.graph, "ms_per_cycle": res.ms_per_cycle,
-           "candidates": res.candidates}
+    out = {"tile_order": res.tile_order, "graph": res.graph, "heavy_first": res.heavy_first,
+           "ms_per_cycle": res.ms_per_cycle, "candidates": res.candidates}
     if cache:
         table = _load(cache)
         table[key] = out

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from simplewinderosion_amd import World, Wind
+from simplewinderosion_amd import World, Wind, capi
 
 pytestmark = pytest.mark.gpu
 
@@ -188,6 +188,36 @@ def test_crowded_tiles(statics, nx, ny, particles, settling):
     assert s["crowded_tile_windows"] > 0 and s["max_tile_particles"] > 2048
     assert s["particle_steps"] == st.steps.value
     w.close()
+
+
+def test_heavy_first_tiles(statics):
+    """Tiles that receive more than 1024 particles in a window run on the
+    first workgroups of the next launch (128x256: 64 tiles, 8 heavy slots;
+    ~1400 particles per tile, so tiles are listed, skipped and left over);
+    results are the oracle's bits with the knob on and off."""
+    say("case heavy-first")
+    Wind.suspension = 1e-4
+    nx, ny, particles = 128, 256, 90000
+    sed = np.full((nx, ny), 0.002, np.float32)
+    w, st = run_pair(nx, ny, particles, 1, seed=6, periodic=True, window=1, settling=0.02, path=2, layers=2,
+                     abrasion=1e-3, sed=sed)
+    assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 1
+    assert_same(w, st)
+    s = w.stats()
+    assert s["heavy_tile_windows"] > 0 and s["max_tile_particles"] > 1024
+    assert s["particle_steps"] == st.steps.value
+    w.close()
+    rng = np.random.default_rng(0)
+    h = (rng.random((nx, ny)) * 0.5).astype(np.float32)
+    w2 = World(nx, ny, periodic=True, window=1, layers=2, abrasion=1e-3)
+    w2.upload(h, sed)
+    w2.set_path(2)
+    w2.set_tune(capi.SWE_TUNE_HEAVY_FIRST, 0)
+    w2.erode(particles, n_cycles=1)
+    w2.synchronize()
+    assert_same(w2, st)
+    assert w2.stats()["heavy_tile_windows"] == 0
+    w2.close()
 
 
 def test_parity_cpu256_workload(statics):
