@@ -18,7 +18,7 @@ namespace swe {
 #endif
 // occupancy target of k_step: waves per SIMD the register budget must allow
 #ifndef SWE_STEP_WAVES_PER_EU
-#define SWE_STEP_WAVES_PER_EU 4
+#define SWE_STEP_WAVES_PER_EU 5
 #endif
 constexpr int kStepTileX = SWE_STEP_TILE_X;    // tile extent along x (slow axis)
 constexpr int kStepTileY = SWE_STEP_TILE_Y;    // tile extent along y (fast axis)

@@ -71,6 +71,13 @@ struct TileAcc {
   }
 };
 
+// a copy of a uniform float in a vector register
+__device__ __forceinline__ float in_vgpr(float x) {
+  float y;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "s"(x));
+  return y;
+}
+
 __device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
   // small-range wrap for tile coordinates (|v| < 2n)
   if (v < 0) v += n;
@@ -78,8 +85,8 @@ __device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
   return v;
 }
 
-// the register budget keeps SWE_STEP_WAVES_PER_EU waves per SIMD (the LDS
-// of one workgroup, under 32 KiB, lets five workgroups share a CU)
+// the register budget keeps SWE_STEP_WAVES_PER_EU (5) waves per SIMD: with
+// the LDS of one workgroup under 32 KiB, five workgroups share a CU
 template <bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool RELAX_ONLY>
 __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SWE_STEP_WAVES_PER_EU))) void k_step(
     StepArgs a) {
@@ -149,21 +156,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   uint32_t* v_out = a.v_out + c0;
   int64_t* trk = TRACK ? a.trk + c0 : nullptr;
 
-  // flow tracks of the tile's cells, loaded now and updated at the end (the
-  // loads complete while the relax and particle phases run)
   constexpr int kCellPer = (kCell + kStepThreads - 1) / kStepThreads;
-  int64_t tpre[TRACK && !RELAX_ONLY ? kCellPer : 1][4];
-  if (TRACK && !RELAX_ONLY) {
-#pragma unroll
-    for (int i = 0; i < kCellPer; i++) {
-      const int e = threadIdx.x + i * kStepThreads;
-      const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
-      const bool in = e < kCell && gx < nx && gy < ny;
-      const int64_t g = (int64_t)gx * ny + gy;
-#pragma unroll
-      for (int f = 0; f < 4; f++) tpre[i][f] = in ? trk[f * a.ncell + g] : 0;
-    }
-  }
 
   // arrivals of this window: the source tile in direction -d of this one
   // wrote the particles that moved by d into its sub-range d
@@ -430,6 +423,16 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     k.abrasion = a.abrasion;
     k.ps = mk(T.ps[0], T.ps[1], T.ps[2]);
     k.nps = mk(T.nps[0], T.nps[1], T.nps[2]);
+    // uniform constants of fly() held in vector registers (one each) rather
+    // than scalar ones, which this kernel has too few of
+    k.mapscale = in_vgpr(k.mapscale);
+    k.boundarylayer = in_vgpr(k.boundarylayer);
+    k.rcp_bl = in_vgpr(k.rcp_bl);
+    k.suspension = in_vgpr(k.suspension);
+    k.gravity = in_vgpr(k.gravity);
+    k.abrasion = in_vgpr(k.abrasion);
+    k.ps = mk(in_vgpr(k.ps.x), in_vgpr(k.ps.y), in_vgpr(k.ps.z));
+    k.nps = mk(in_vgpr(k.nps.x), in_vgpr(k.nps.y), in_vgpr(k.nps.z));
     const uint64_t ck = cycle_key(T.seed, *a.cycle);
     const int64_t p0 = T.part0;
     const int64_t out_base = s_base;  // 9 x the input counts of tiles 0 .. u-1
@@ -575,13 +578,31 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     zm_out[g] = acc_d[e];
     v_out[g] = acc_v[e];
     if (LAYERS == 2) sm_out[g] = acc_ds[e];
-    if (TRACK && !RELAX_ONLY) {
+  }
+  if (TRACK && !RELAX_ONLY) {
+    // flow tracks of the cells this window's particles visited, read here
+    // rather than prefetched (16 fewer vector registers through the particle
+    // loop lets five workgroups share a CU): all loads are issued first,
+    // then the sums stored
+    int64_t tv[kCellPer][4], old[kCellPer][4], gi[kCellPer];
 #pragma unroll
-      for (int f = 0; f < 4; f++) {
-        const int64_t t = acc_t[f * kCell + e];
-        if (t != 0) trk[f * a.ncell + g] = tpre[i][f] + t;
-      }
+    for (int i = 0; i < kCellPer; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
+      const bool in = e < kCell && gx < nx && gy < ny;
+      gi[i] = (int64_t)gx * ny + gy;
+#pragma unroll
+      for (int f = 0; f < 4; f++) tv[i][f] = in ? acc_t[f * kCell + (in ? e : 0)] : 0;
     }
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++)
+#pragma unroll
+      for (int f = 0; f < 4; f++) old[i][f] = tv[i][f] != 0 ? trk[f * a.ncell + gi[i]] : 0;
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++)
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+        if (tv[i][f] != 0) trk[f * a.ncell + gi[i]] = old[i][f] + tv[i][f];
   }
   // s_rbase is free again (the arrivals are consumed): it carries the chunk
   // of each direction's destination tile to the aggregation below

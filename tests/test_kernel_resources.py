@@ -1,6 +1,6 @@
 """Resource budget of the step kernel, read from the gfx950 code-object
-metadata that hipcc emits: no private (scratch) memory, at most 128 VGPRs
-(four waves per SIMD) and LDS for four workgroups per CU.  A kernel that
+metadata that hipcc emits: no private (scratch) memory, at most 96 VGPRs
+(five waves per SIMD) and LDS for five workgroups per CU.  A kernel that
 breaks one of these still computes the right bits but several times slower,
 so the budget is checked at build time, without a GPU."""
 import os
@@ -39,6 +39,6 @@ def test_step_kernel_budget(metadata):
     assert len(steps) == 32  # periodic x track x layers x deposit x relax-only
     for name, f in steps.items():
         assert f["private_segment_fixed_size"] == 0, name
-        assert f["vgpr_count"] <= 128, (name, f["vgpr_count"])
+        assert f["vgpr_count"] <= 96, (name, f["vgpr_count"])
         assert f.get("vgpr_spill_count", 0) == 0, name
-        assert 4 * f["group_segment_fixed_size"] <= LDS_PER_CU, (name, f["group_segment_fixed_size"])
+        assert 5 * f["group_segment_fixed_size"] <= LDS_PER_CU, (name, f["group_segment_fixed_size"])
