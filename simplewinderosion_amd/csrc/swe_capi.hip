@@ -409,6 +409,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sa.cycle = c->dcycle;
   sa.terr = c->dterr;
   sa.tile_terrain = c->dtile_terrain;
+  sa.nterr = c->nterr;
   sa.ncell = c->ncell;
   sa.ntiles = c->ntiles;
   sa.mapscale = c->p.mapscale;

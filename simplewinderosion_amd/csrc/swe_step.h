@@ -87,6 +87,7 @@ struct StepArgs {
   const uint64_t* cycle;
   const Terrain* terr;
   const int32_t* tile_terrain;  // per global tile
+  int32_t nterr;                // 1: every tile is terrain 0 (no tile_terrain lookup)
   int64_t ncell;                // cells of all terrains (track plane stride)
   int32_t ntiles;
   float mapscale, boundarylayer, suspension, gravity, abrasion, maxdiff, settling;

@@ -132,10 +132,11 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
   }
   // a listed tile's stamp is window + 1 (or window + 2 once its heavy
-  // workgroup has listed it again): either way the usual one skips it
-  if (hs > 0 && !listed && a.hstamp[u] > (uint32_t)a.window) return;
+  // workgroup has listed it again): either way the usual one skips it.  The
+  // stamp is checked once the relax loads are out (nothing is written before).
+  const uint32_t stamp = hs > 0 && !listed ? a.hstamp[u] : 0u;
   // the terrain this tile belongs to; all map indices below are terrain-local
-  const Terrain T = a.terr[a.tile_terrain[u]];
+  const Terrain T = a.terr[a.nterr == 1 ? 0 : a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
   const int32_t tiles_x = T.tiles_x, tiles_y = T.tiles_y;
   const int32_t t0 = T.tile0;
@@ -158,36 +159,21 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
 
   constexpr int kCellPer = (kCell + kStepThreads - 1) / kStepThreads;
 
-  // arrivals of this window: the source tile in direction -d of this one
-  // wrote the particles that moved by d into its sub-range d
-  if (threadIdx.x < 9) {
-    const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
-    int32_t sx = tx - dx, sy = ty - dy;
-    bool ok = !RELAX_ONLY;
-    if (PERIODIC) {
-      sx = wrapt(sx, tiles_x);
-      sy = wrapt(sy, tiles_y);
-    } else {
-      ok = ok && sx >= 0 && sy >= 0 && sx < tiles_x && sy < tiles_y;
-    }
-    int64_t rb = 0;
-    uint32_t rc = 0;
-    if (ok) {
-      const int64_t s9 = (int64_t)(t0 + sx * tiles_y + sy) * 9 + d;
-      rb = a.sub_base_prev[s9];
-      rc = a.sub_cnt_prev[s9];
-    }
-    s_rbase[d] = rb;
-    s_slot[d] = rc;  // lengths, turned into s_rpre below
-  }
   // this window's output region starts at 9 x (input counts of tiles 0 ..
   // u-1): the chunk sums of the earlier chunks plus the earlier tiles of this
-  // chunk; the loads are issued here and summed after the relax phase
-  int64_t part = 0;
+  // chunk.  The loads are issued here and summed after the relax phase (no
+  // wait on them before the map loads go out).
+  static_assert(kChunk <= kStepThreads, "one tile of the current chunk per thread");
+  uint32_t part_c = 0, part_n = 0;
   {
     const int32_t cu = u / kChunk;
-    for (int32_t c = threadIdx.x; c < cu; c += kStepThreads) part += a.csum_cur[c];
-    for (int32_t t = cu * kChunk + threadIdx.x; t < u; t += kStepThreads) part += a.nin_cur[t];
+    if (cu <= kStepThreads) {
+      if ((int32_t)threadIdx.x < cu) part_c = a.csum_cur[threadIdx.x];
+    } else {
+      for (int32_t c = threadIdx.x; c < cu; c += kStepThreads) part_c += a.csum_cur[c];
+    }
+    const int32_t t = cu * kChunk + (int32_t)threadIdx.x;
+    if (t < u) part_n = a.nin_cur[t];
   }
   const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
 
@@ -216,40 +202,80 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       zpre[i] = ok ? zm_in[g] : 0;
       if (LAYERS == 2) spre[i] = ok ? sm_in[g] : 0;
     }
-    for (int e = threadIdx.x; e < kH3X * kH3Y; e += kStepThreads) {
+    // snapshot (3-halo) and events (2-halo): every load is issued before the
+    // first LDS store, so the fill costs one memory round trip
+    constexpr int kH3Per = (kH3X * kH3Y + kStepThreads - 1) / kStepThreads;
+    constexpr int kH2Per = (kH2X * kH2Y + kStepThreads - 1) / kStepThreads;
+    float hv[kH3Per], hs3[LAYERS == 2 ? kH3Per : 1];
+    uint32_t vv[kH2Per];
+#pragma unroll
+    for (int i = 0; i < kH3Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
       int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
-      float v = 0.0f, vs = 0.0f;
-      bool ok = true;
+      bool ok = e < kH3X * kH3Y;
       if (PERIODIC) {
         gx = wrap(gx, nx, rcp_nx);
         gy = wrap(gy, ny, rcp_ny);
       } else {
-        ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
-      if (ok) {
-        const int64_t g = (int64_t)gx * ny + gy;
-        v = snap_in[g];
-        if (LAYERS == 2) vs = ssnap_in[g];
-      }
-      s_old[e] = v;
-      if (LAYERS == 2) s_sold[e] = vs;
+      const int64_t g = (int64_t)gx * ny + gy;
+      hv[i] = ok ? snap_in[g] : 0.0f;
+      if (LAYERS == 2) hs3[i] = ok ? ssnap_in[g] : 0.0f;
     }
-    for (int e = threadIdx.x; e < kH2X * kH2Y; e += kStepThreads) {
+#pragma unroll
+    for (int i = 0; i < kH2Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
       int32_t gx = x0 - 2 + e / kH2Y, gy = y0 - 2 + e % kH2Y;
-      uint32_t v = 0;
-      bool ok = true;
+      bool ok = e < kH2X * kH2Y;
       if (PERIODIC) {
         gx = wrap(gx, nx, rcp_nx);
         gy = wrap(gy, ny, rcp_ny);
       } else {
-        ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+        ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
-      if (ok) v = v_in[(int64_t)gx * ny + gy];
-      s_vold[e] = v;
+      vv[i] = ok ? v_in[(int64_t)gx * ny + gy] : 0u;
+    }
+    // arrivals of this window: the source tile in direction -d of this one
+    // wrote the particles that moved by d into its sub-range d (loaded after
+    // the map loads are out, so that their wait overlaps those)
+    if (threadIdx.x < 9) {
+      const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
+      int32_t sx = tx - dx, sy = ty - dy;
+      bool ok = !RELAX_ONLY;
+      if (PERIODIC) {
+        sx = wrapt(sx, tiles_x);
+        sy = wrapt(sy, tiles_y);
+      } else {
+        ok = ok && sx >= 0 && sy >= 0 && sx < tiles_x && sy < tiles_y;
+      }
+      int64_t rb = 0;
+      uint32_t rc = 0;
+      if (ok) {
+        const int64_t s9 = (int64_t)(t0 + sx * tiles_y + sy) * 9 + d;
+        rb = a.sub_base_prev[s9];
+        rc = a.sub_cnt_prev[s9];
+      }
+      s_rbase[d] = rb;
+      s_slot[d] = rc;  // lengths, turned into s_rpre below
+    }
+#pragma unroll
+    for (int i = 0; i < kH3Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      if (e < kH3X * kH3Y) {
+        s_old[e] = hv[i];
+        if (LAYERS == 2) s_sold[e] = hs3[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kH2Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      if (e < kH2X * kH2Y) s_vold[e] = vv[i];
     }
     uint16_t* s_cas = (uint16_t*)(s_casc + kH1X * kH1Y);  // H2 cells with a transfer
     for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
     if (threadIdx.x == 0) s_ncas = 0;
+    if (stamp > (uint32_t)a.window) return;  // uniform: the whole workgroup leaves
     __syncthreads();
 
     // visited cells within two cells of the tile whose cascade moves
@@ -373,6 +399,7 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     __syncthreads();
   }
   {
+    int64_t part = (int64_t)part_c + (int64_t)part_n;
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
     if ((threadIdx.x & 63) == 0) s_lost[threadIdx.x >> 6] = part;
     __syncthreads();
