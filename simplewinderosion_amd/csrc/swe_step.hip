@@ -144,6 +144,9 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   const int32_t tx = ul / tiles_y, ty = ul % tiles_y;
   const int32_t x0 = tx * kSX, y0 = ty * kSY;
   const float rcp_nx = T.rcp_nx, rcp_ny = T.rcp_ny;
+  // periodic map, tile and 3-cell halo inside it (most tiles of a large
+  // map): the halo loads below need no wrap
+  const bool inner = false;
   const int64_t c0 = T.cell0;
   const int64_t* zm_in = a.zm_in + c0;
   int64_t* zm_out = a.zm_out + c0;
@@ -193,8 +196,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 1 + e / kH1Y, gy = y0 - 1 + e % kH1Y;
       bool ok = e < kH1X * kH1Y;
       if (PERIODIC) {
-        gx = wrap(gx, nx, rcp_nx);
-        gy = wrap(gy, ny, rcp_ny);
+        if (!inner) {
+          gx = wrap(gx, nx, rcp_nx);
+          gy = wrap(gy, ny, rcp_ny);
+        }
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -214,8 +219,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
       bool ok = e < kH3X * kH3Y;
       if (PERIODIC) {
-        gx = wrap(gx, nx, rcp_nx);
-        gy = wrap(gy, ny, rcp_ny);
+        if (!inner) {
+          gx = wrap(gx, nx, rcp_nx);
+          gy = wrap(gy, ny, rcp_ny);
+        }
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -229,8 +236,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 2 + e / kH2Y, gy = y0 - 2 + e % kH2Y;
       bool ok = e < kH2X * kH2Y;
       if (PERIODIC) {
-        gx = wrap(gx, nx, rcp_nx);
-        gy = wrap(gy, ny, rcp_ny);
+        if (!inner) {
+          gx = wrap(gx, nx, rcp_nx);
+          gy = wrap(gy, ny, rcp_ny);
+        }
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -364,8 +373,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
       bool ok = true;
       if (PERIODIC) {
-        gx = wrap(gx, nx, rcp_nx);
-        gy = wrap(gy, ny, rcp_ny);
+        if (!inner) {
+          gx = wrap(gx, nx, rcp_nx);
+          gy = wrap(gy, ny, rcp_ny);
+        }
       } else {
         ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -606,12 +617,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     v_out[g] = acc_v[e];
     if (LAYERS == 2) sm_out[g] = acc_ds[e];
   }
+  // flow tracks of the cells this window's particles visited, read here
+  // rather than prefetched (16 fewer vector registers through the particle
+  // loop lets five workgroups share a CU): the loads are issued now and the
+  // sums stored at the very end, after the rest of the write-out
+  int64_t tv[TRACK && !RELAX_ONLY ? kCellPer : 1][4], old[TRACK && !RELAX_ONLY ? kCellPer : 1][4];
+  int64_t gi[TRACK && !RELAX_ONLY ? kCellPer : 1];
   if (TRACK && !RELAX_ONLY) {
-    // flow tracks of the cells this window's particles visited, read here
-    // rather than prefetched (16 fewer vector registers through the particle
-    // loop lets five workgroups share a CU): all loads are issued first,
-    // then the sums stored
-    int64_t tv[kCellPer][4], old[kCellPer][4], gi[kCellPer];
 #pragma unroll
     for (int i = 0; i < kCellPer; i++) {
       const int e = threadIdx.x + i * kStepThreads;
@@ -625,11 +637,6 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     for (int i = 0; i < kCellPer; i++)
 #pragma unroll
       for (int f = 0; f < 4; f++) old[i][f] = tv[i][f] != 0 ? trk[f * a.ncell + gi[i]] : 0;
-#pragma unroll
-    for (int i = 0; i < kCellPer; i++)
-#pragma unroll
-      for (int f = 0; f < 4; f++)
-        if (tv[i][f] != 0) trk[f * a.ncell + gi[i]] = old[i][f] + tv[i][f];
   }
   // s_rbase is free again (the arrivals are consumed): it carries the chunk
   // of each direction's destination tile to the aggregation below
@@ -717,6 +724,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if (l != 0)
       atomicAdd((unsigned long long*)&a.lost[(int64_t)a.tile_terrain[u] * kLostSlots + (u & (kLostSlots - 1))],
                 (unsigned long long)l);
+  }
+  if (TRACK && !RELAX_ONLY) {
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++)
+#pragma unroll
+      for (int f = 0; f < 4; f++)
+        if (tv[i][f] != 0) trk[f * a.ncell + gi[i]] = old[i][f] + tv[i][f];
   }
 }
 
