@@ -155,6 +155,8 @@ def main():
                     help="SWE_TUNE_TILE_ORDER for the step kernel (default: the library's)")
     ap.add_argument("--graph", type=int, default=None,
                     help="SWE_TUNE_GRAPH: 1 = a cycle is one HIP graph launch (used when timing is off)")
+    ap.add_argument("--heavy-first", type=int, default=None,
+                    help="SWE_TUNE_HEAVY_FIRST: 1 = tiles crowded in one window run first in the next")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -189,6 +191,8 @@ def main():
         world.set_tune(capi.SWE_TUNE_TILE_ORDER, args.tile_order)
     if args.graph is not None:
         world.set_tune(capi.SWE_TUNE_GRAPH, args.graph)
+    if args.heavy_first is not None:
+        world.set_tune(capi.SWE_TUNE_HEAVY_FIRST, args.heavy_first)
     if not p.periodic:
         raise SystemExit("bench counts particle-steps analytically: periodic workloads only")
     N = wl["particles"]
@@ -252,7 +256,8 @@ def main():
         "map_cycles_per_s": args.steps / elapsed,
         "mass_exact": None,
         "tiles": {"max_particles_per_tile": st["max_tile_particles"],
-                  "crowded_tile_windows": st["crowded_tile_windows"], "far_moves": st["far_particles"]},
+                  "crowded_tile_windows": st["crowded_tile_windows"], "far_moves": st["far_particles"],
+                  "heavy_first_tile_windows": st["heavy_tile_windows"]},
         "roofline": roofline,
     }
     line["mass_exact"] = bool(audit["height_sum"] + audit["lost"] == mass0)
