@@ -156,7 +156,8 @@ def main():
     ap.add_argument("--graph", type=int, default=None,
                     help="SWE_TUNE_GRAPH: 1 = a cycle is one HIP graph launch (used when timing is off)")
     ap.add_argument("--heavy-first", type=int, default=None,
-                    help="SWE_TUNE_HEAVY_FIRST: 1 = tiles crowded in one window run first in the next")
+                    help="SWE_TUNE_HEAVY_FIRST: 1 = tiles crowded in one window run first in the next, "
+                         "2 = and light ones last (the library's default)")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -98,6 +98,7 @@ typedef struct swe_stats_t {
   uint64_t crowded_tile_windows;/* (tile, window) pairs above the tile's LDS list capacity */
   uint64_t far_particles;       /* particle moves that skipped more than one tile */
   uint64_t heavy_tile_windows;  /* (tile, window) pairs run first as crowded (SWE_TUNE_HEAVY_FIRST) */
+  uint64_t light_tile_windows;  /* (tile, window) pairs run last as light (SWE_TUNE_HEAVY_FIRST = 2) */
 } swe_stats_t;
 
 /* Reference defaults (wind.hpp:13,23-26; world.hpp:30-32; cellpool.h:184). */
@@ -161,10 +162,13 @@ int swe_set_path(swe_ctx* ctx, int32_t path);
  *     kernel launches into a HIP graph once (per starting window parity and
  *     launch configuration) and launches the graph each cycle; 0 = launches
  *     each kernel.  Timing (swe_set_timing) uses individual launches.
- *   SWE_TUNE_HEAVY_FIRST: 1 (default) = a tile that received more than 1024
- *     particles in one window is handed to one of the first workgroups of
- *     the next window's launch, so crowded tiles start first instead of
- *     finishing last; 0 = every tile in the SWE_TUNE_TILE_ORDER order.
+ *   SWE_TUNE_HEAVY_FIRST: 1 = a tile that received more than 1024 particles
+ *     in one window is handed to one of the first workgroups of the next
+ *     window's launch, so crowded tiles start first instead of finishing
+ *     last; 2 (default) = also a tile that received at most 256 is handed
+ *     to one of the last workgroups (on its own XCD), so short workgroups
+ *     fill the launch's tail; 0 = every tile in the SWE_TUNE_TILE_ORDER
+ *     order.  Maps of fewer than 64 tiles always use 0.
  */
 enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2, SWE_TUNE_HEAVY_FIRST = 3 };
 int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);

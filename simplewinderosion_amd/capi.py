@@ -95,6 +95,7 @@ class SweStats(ctypes.Structure):
         ("crowded_tile_windows", ctypes.c_uint64),
         ("far_particles", ctypes.c_uint64),
         ("heavy_tile_windows", ctypes.c_uint64),
+        ("light_tile_windows", ctypes.c_uint64),
     ]
 
 

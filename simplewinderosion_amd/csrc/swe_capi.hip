@@ -35,7 +35,7 @@ struct swe_ctx {
   int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
   int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
   int32_t use_graph = 1;   // SWE_TUNE_GRAPH
-  int32_t heavy_first = 1; // SWE_TUNE_HEAVY_FIRST
+  int32_t heavy_first = 2; // SWE_TUNE_HEAVY_FIRST
   // one captured map-cycle of the tile path per starting window parity, and
   // the launch configuration it was captured for
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -81,8 +81,8 @@ struct swe_ctx {
   uint32_t* scnt[2] = {nullptr, nullptr};   // 9 x tiles: routing sub-range lengths
   uint32_t* nin = nullptr;     // 3 x tiles
   uint32_t* csum = nullptr;    // 3 x chunks of kStepChunk tiles
-  uint32_t* heavy[2] = {nullptr, nullptr};  // kHeavySlotsMax: crowded tiles listed for a window
-  uint32_t* hcnt = nullptr;    // 3: their counts
+  uint32_t* heavy[2] = {nullptr, nullptr};  // kHeavySlotsMax crowded + light_cap light tiles listed for a window
+  uint32_t* hcnt = nullptr;    // 3 x 9: their counts (heavy, light per XCD)
   uint32_t* hstamp = nullptr;  // per tile: window + 1 when listed
   int32_t* sp_tile = nullptr;
   uint32_t* sp_rank = nullptr;
@@ -193,6 +193,9 @@ int ensure_particles(swe_ctx* c, int64_t n) {
   return 0;
 }
 
+// light-tile slots of a launch: half the tiles, a multiple of 8
+int64_t light_cap(int64_t ntiles) { return (ntiles / 2 + 7) / 8 * 8; }
+
 // buffers of the tile-owned step path for n particles
 int ensure_step(swe_ctx* c, int64_t n) {
   int r;
@@ -204,8 +207,8 @@ int ensure_step(swe_ctx* c, int64_t n) {
         (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->sbase[0], 9 * nt)) ||
         (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
         (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
-        (r = dalloc(c, &c->csum, 3 * nch)) || (r = dalloc(c, &c->heavy[0], swe::kHeavySlotsMax)) ||
-        (r = dalloc(c, &c->heavy[1], swe::kHeavySlotsMax)) || (r = dalloc(c, &c->hcnt, 3)) ||
+        (r = dalloc(c, &c->csum, 3 * nch)) || (r = dalloc(c, &c->heavy[0], swe::kHeavySlotsMax + light_cap(nt))) ||
+        (r = dalloc(c, &c->heavy[1], swe::kHeavySlotsMax + light_cap(nt))) || (r = dalloc(c, &c->hcnt, 27)) ||
         (r = dalloc(c, &c->hstamp, nt)))
       return r;
   }
@@ -424,6 +427,11 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   // listed crowded tiles get the first workgroups of a launch; at most one
   // slot per 8 tiles, a multiple of 8 (none on maps under 64 tiles)
   sa.heavy_slots = c->heavy_first ? (int32_t)std::min<int64_t>(swe::kHeavySlotsMax, (c->ntiles / 8) & ~7LL) : 0;
+  // light tiles last (heavy-first mode 2, maps of 64 tiles or more): the
+  // usual order is padded to a multiple of 8 so that slot b runs on XCD b % 8
+  const bool light = c->heavy_first == 2 && sa.heavy_slots > 0;
+  sa.light_slots = light ? (int32_t)light_cap(c->ntiles) : 0;
+  sa.default_slots = light ? (int32_t)((c->ntiles + 7) / 8 * 8) : c->ntiles;
 
   const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
   swe::SpawnArgs sp{};
@@ -451,7 +459,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
       swe::launch_zero32(c->csum, 3 * nch, c->stream);
       swe::launch_zero32(c->farcnt, 3, c->stream);
       if (sa.heavy_slots > 0) {
-        swe::launch_zero32(c->hcnt, 3, c->stream);
+        swe::launch_zero32(c->hcnt, 27, c->stream);
         swe::launch_zero32(c->hstamp, nt, c->stream);
       }
       swe::launch_spawn(sp, c->stream);
@@ -491,9 +499,9 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.window = (int32_t)w;
         sa.heavy_cur = c->heavy[wp];
         sa.heavy_next = c->heavy[wp ^ 1];
-        sa.hcnt_cur = c->hcnt + w3;
-        sa.hcnt_next = c->hcnt + w3n;
-        sa.hcnt_clear = c->hcnt + w3c;
+        sa.hcnt_cur = c->hcnt + 9 * w3;
+        sa.hcnt_next = c->hcnt + 9 * w3n;
+        sa.hcnt_clear = c->hcnt + 9 * w3c;
         sa.hstamp = c->hstamp;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int rr;
@@ -966,7 +974,7 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
   double best = 1e300;
   int32_t best_order = order0, best_graph = graph0, best_heavy = heavy0, ncand = 0;
   // heavy-first only changes anything where a launch has heavy slots
-  const int32_t heavy_lo = c->ntiles / 8 >= 8 ? 0 : heavy0, heavy_hi = c->ntiles / 8 >= 8 ? 1 : heavy0;
+  const int32_t heavy_lo = c->ntiles / 8 >= 8 ? 0 : heavy0, heavy_hi = c->ntiles / 8 >= 8 ? 2 : heavy0;
   for (int32_t hv = heavy_hi; hv >= heavy_lo && r == 0 && he == hipSuccess; hv--)
   for (int32_t graph = 0; graph <= 1 && r == 0; graph++) {
     for (int32_t order : kOrders) {
@@ -1041,7 +1049,7 @@ int swe_set_tune(swe_ctx* c, int32_t knob, int64_t value) {
       c->use_graph = (int32_t)value;
       return 0;
     case SWE_TUNE_HEAVY_FIRST:
-      if (value != 0 && value != 1) return fail(c, SWE_EINVAL, "heavy-first must be 0 or 1");
+      if (value < 0 || value > 2) return fail(c, SWE_EINVAL, "heavy-first must be 0, 1 or 2");
       c->heavy_first = (int32_t)value;
       return 0;
     default:
@@ -1168,13 +1176,14 @@ int swe_stats(swe_ctx* c, swe_stats_t* out) {
   HIPCHK(c, hipMemcpyAsync(h, c->dstat, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   out->particle_steps = out->max_tile_particles = out->crowded_tile_windows = out->far_particles = 0;
-  out->heavy_tile_windows = 0;
+  out->heavy_tile_windows = out->light_tile_windows = 0;
   for (int q = 0; q < swe::kStatSlots; q++) {
     out->particle_steps += h[swe::kStatSteps * swe::kStatSlots + q];
     const uint64_t m = h[swe::kStatMaxTile * swe::kStatSlots + q];
     if (m > out->max_tile_particles) out->max_tile_particles = m;
     out->crowded_tile_windows += h[swe::kStatCrowded * swe::kStatSlots + q];
     out->heavy_tile_windows += h[swe::kStatHeavy * swe::kStatSlots + q];
+    out->light_tile_windows += h[swe::kStatLight * swe::kStatSlots + q];
     out->far_particles += h[swe::kStatFar * swe::kStatSlots + q];
   }
   return 0;

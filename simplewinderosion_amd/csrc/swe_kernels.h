@@ -14,7 +14,7 @@ constexpr int kLostSlots = 256;
 // over slots to keep the atomics off one address); summed (max for
 // kStatMaxTile) by the host
 constexpr int kStatSlots = 64;
-enum StatKind { kStatSteps = 0, kStatMaxTile = 1, kStatCrowded = 2, kStatFar = 3, kStatHeavy = 4, kStatKinds = 5 };
+enum StatKind { kStatSteps = 0, kStatMaxTile = 1, kStatCrowded = 2, kStatFar = 3, kStatHeavy = 4, kStatLight = 5, kStatKinds = 6 };
 
 struct MarchArgs {
   const float* S;      // height snapshot of the window

@@ -100,12 +100,14 @@ struct StepArgs {
   // workgroups take the listed tiles (so they start at once instead of
   // finishing last); the workgroup of the usual order then skips the tile
   int32_t heavy_slots;       // 0 = off, else a multiple of 8 (XCD mapping)
+  int32_t default_slots;     // workgroups of the usual order: ntiles (a multiple of 8 with light slots)
+  int32_t light_slots;       // 0 = off, else a multiple of 8: light tiles last, per XCD
   int32_t window;            // window index within the cycle
-  const uint32_t* heavy_cur; // this launch's listed tiles
+  const uint32_t* heavy_cur; // this launch's listed tiles: kHeavySlotsMax heavy, then light_slots light
   uint32_t* heavy_next;      // the next launch's (appended here)
-  const uint32_t* hcnt_cur;  // list lengths (may exceed heavy_slots)
+  const uint32_t* hcnt_cur;  // list lengths (may exceed the slots): [0] heavy, [1 + x] light on XCD x
   uint32_t* hcnt_next;
-  uint32_t* hcnt_clear;      // the length of the list after next
+  uint32_t* hcnt_clear;      // the lengths of the lists after next
   uint32_t* hstamp;          // per tile: w + 1 when listed for window w (0 at cycle start)
 };
 
@@ -115,7 +117,11 @@ struct StepArgs {
 #ifndef SWE_HEAVY_SLOTS
 #define SWE_HEAVY_SLOTS 256
 #endif
+#ifndef SWE_LIGHT_MAX
+#define SWE_LIGHT_MAX 256
+#endif
 constexpr uint32_t kHeavyMin = SWE_HEAVY_MIN;        // arrivals that make a tile heavy
+constexpr uint32_t kLightMax = SWE_LIGHT_MAX;        // arrivals that leave a tile light
 constexpr int32_t kHeavySlotsMax = SWE_HEAVY_SLOTS;  // listed tiles per launch (a multiple of 8)
 
 struct SpawnArgs {

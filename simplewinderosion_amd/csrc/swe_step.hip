@@ -115,14 +115,23 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   // previous launch (empty slots return at once); the rest follow the
   // usual order and skip a listed tile.  heavy_slots is a multiple of 8, so
   // b and b - heavy_slots fall on the same XCD.
+  // Light tiles (at most kLightMax arrivals last window) are listed per XCD
+  // for the last light_slots workgroups, which run on their home XCD and
+  // start last, when the launch's long workgroups are already running.
   const uint32_t hs = RELAX_ONLY ? 0u : (uint32_t)a.heavy_slots;
-  const uint32_t nb = gridDim.x - (uint32_t)a.heavy_slots, b = blockIdx.x - (uint32_t)a.heavy_slots;
+  const uint32_t ls = RELAX_ONLY ? 0u : (uint32_t)a.light_slots;
+  const uint32_t nb = (uint32_t)a.default_slots, b = blockIdx.x - (uint32_t)a.heavy_slots;
   const uint32_t xcd = b % 8;
   int32_t u;  // global tile
   bool listed = false;
   if (blockIdx.x < (uint32_t)a.heavy_slots) {
-    if (hs == 0 || blockIdx.x >= *a.hcnt_cur) return;
+    if (hs == 0 || blockIdx.x >= a.hcnt_cur[0]) return;
     u = (int32_t)a.heavy_cur[blockIdx.x];
+    listed = true;
+  } else if (b >= nb) {
+    const uint32_t bl = b - nb, k = bl / 8;
+    if (ls == 0 || k >= a.hcnt_cur[1 + xcd]) return;
+    u = (int32_t)a.heavy_cur[kHeavySlotsMax + xcd * (ls / 8) + k];
     listed = true;
   } else if (a.tile_order <= 0) {
     const uint32_t q8 = nb / 8, r8 = nb % 8;
@@ -131,10 +140,11 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C), j = b / 8;
     u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
   }
+  if (u >= a.ntiles) return;  // padding of the usual order to a multiple of 8
   // a listed tile's stamp is window + 1 (or window + 2 once its heavy
   // workgroup has listed it again): either way the usual one skips it.  The
   // stamp is checked once the relax loads are out (nothing is written before).
-  const uint32_t stamp = hs > 0 && !listed ? a.hstamp[u] : 0u;
+  const uint32_t stamp = (hs | ls) > 0 && !listed ? a.hstamp[u] : 0u;
   // the terrain this tile belongs to; all map indices below are terrain-local
   const Terrain T = a.terr[a.nterr == 1 ? 0 : a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
@@ -667,14 +677,29 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if (u % kChunk == 0) a.csum_clear[u / kChunk] = 0;
     if (u == 0) {
       *a.far_cnt_clear = 0;
-      if (hs > 0) *a.hcnt_clear = 0;
+      if ((hs | ls) > 0)
+        for (int i = 0; i < 9; i++) a.hcnt_clear[i] = 0;
     }
-    // crowded now: listed for the next launch (the count moves little from
-    // one window to the next)
+    // crowded (or light) now: listed for the next launch (the count moves
+    // little from one window to the next)
     if (hs > 0 && nin_u > kHeavyMin) {
-      const uint32_t slot = atomicAdd(a.hcnt_next, 1u);
+      const uint32_t slot = atomicAdd(&a.hcnt_next[0], 1u);
       if (slot < hs) {
         a.heavy_next[slot] = (uint32_t)u;
+        a.hstamp[u] = (uint32_t)a.window + 2u;
+      }
+    } else if (ls > 0 && nin_u <= kLightMax) {
+      // home XCD of u in the usual order (nb is a multiple of 8 here)
+      uint32_t home;
+      if (a.tile_order <= 0) {
+        home = (uint32_t)u / (nb / 8);
+      } else {
+        const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C);
+        home = (uint32_t)u < full ? ((uint32_t)u / C) % 8 : (uint32_t)u % 8;
+      }
+      const uint32_t slot = atomicAdd(&a.hcnt_next[1 + home], 1u);
+      if (slot < ls / 8) {
+        a.heavy_next[kHeavySlotsMax + home * (ls / 8) + slot] = (uint32_t)u;
         a.hstamp[u] = (uint32_t)a.window + 2u;
       }
     }
@@ -715,7 +740,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if (far) atomicAdd((unsigned long long*)&a.stat[kStatFar * kStatSlots + slot], (unsigned long long)far);
     if (nin_u) atomicMax((unsigned long long*)&a.stat[kStatMaxTile * kStatSlots + slot], (unsigned long long)nin_u);
     if (nin_u > kCrowded) atomicAdd((unsigned long long*)&a.stat[kStatCrowded * kStatSlots + slot], 1ull);
-    if (listed) atomicAdd((unsigned long long*)&a.stat[kStatHeavy * kStatSlots + slot], 1ull);
+    if (listed) {
+      const bool light_slot = blockIdx.x >= (uint32_t)(a.heavy_slots + a.default_slots);
+      atomicAdd((unsigned long long*)&a.stat[(light_slot ? kStatLight : kStatHeavy) * kStatSlots + slot], 1ull);
+    }
   }
   __syncthreads();
   if (!DEPOSIT && threadIdx.x == 0) {
@@ -826,7 +854,7 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
 // ---------------------------------------------------------------------------
 template <bool P, bool T, int L, bool D>
 static void step4(const StepArgs& a, bool relax_only, hipStream_t s) {
-  const unsigned g = (unsigned)(a.ntiles + a.heavy_slots);
+  const unsigned g = (unsigned)(a.heavy_slots + a.default_slots + a.light_slots);
   if (relax_only)
     hipLaunchKernelGGL((k_step<P, T, L, D, true>), dim3(g), dim3(kStepThreads), 0, s, a);
   else

@@ -190,34 +190,45 @@ def test_crowded_tiles(statics, nx, ny, particles, settling):
     w.close()
 
 
-def test_heavy_first_tiles(statics):
-    """Tiles that receive more than 1024 particles in a window run on the
-    first workgroups of the next launch (128x256: 64 tiles, 8 heavy slots;
-    ~1400 particles per tile, so tiles are listed, skipped and left over);
-    results are the oracle's bits with the knob on and off."""
-    say("case heavy-first")
+@pytest.mark.parametrize("particles,modes", [(90000, (1, 0, 2)), (20000, (2,))])
+def test_heavy_first_tiles(statics, particles, modes):
+    """SWE_TUNE_HEAVY_FIRST: tiles that received more than 1024 particles in a
+    window run on the first workgroups of the next launch (mode 1), and in
+    mode 2, the default, tiles with at most 256 also run on the last ones,
+    listed per XCD.  128x256 is 64 tiles: 8 heavy and 32 light slots, so tiles
+    are listed, skipped and left over (~1400 particles per tile at 90000,
+    ~300 at 20000).  Results are the oracle's bits in every mode."""
+    say("case heavy-first", particles, modes)
     Wind.suspension = 1e-4
-    nx, ny, particles = 128, 256, 90000
+    World.settling = 0.02
+    World.SEED = 6
+    nx, ny = 128, 256
     sed = np.full((nx, ny), 0.002, np.float32)
-    w, st = run_pair(nx, ny, particles, 1, seed=6, periodic=True, window=1, settling=0.02, path=2, layers=2,
-                     abrasion=1e-3, sed=sed)
-    assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 1
-    assert_same(w, st)
-    s = w.stats()
-    assert s["heavy_tile_windows"] > 0 and s["max_tile_particles"] > 1024
-    assert s["particle_steps"] == st.steps.value
-    w.close()
-    rng = np.random.default_rng(0)
-    h = (rng.random((nx, ny)) * 0.5).astype(np.float32)
-    w2 = World(nx, ny, periodic=True, window=1, layers=2, abrasion=1e-3)
-    w2.upload(h, sed)
-    w2.set_path(2)
-    w2.set_tune(capi.SWE_TUNE_HEAVY_FIRST, 0)
-    w2.erode(particles, n_cycles=1)
-    w2.synchronize()
-    assert_same(w2, st)
-    assert w2.stats()["heavy_tile_windows"] == 0
-    w2.close()
+    h = (np.random.default_rng(0).random((nx, ny)) * 0.5).astype(np.float32)
+    st = None
+    for mode in modes:
+        w = World(nx, ny, periodic=True, window=1, layers=2, abrasion=1e-3)
+        w.upload(h, sed)
+        w.set_path(2)
+        if st is None:
+            st = O.OracleState(nx, ny, h, sediment=sed, params=O.params_from(w.params()))
+            st.erode(particles, 1)
+        w.set_tune(capi.SWE_TUNE_HEAVY_FIRST, mode)
+        assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == mode
+        w.erode(particles, n_cycles=1)
+        w.synchronize()
+        assert_same(w, st)
+        s = w.stats()
+        assert s["particle_steps"] == st.steps.value
+        if mode == 0:
+            assert s["heavy_tile_windows"] == 0 and s["light_tile_windows"] == 0
+        if mode >= 1 and particles == 90000:
+            assert s["heavy_tile_windows"] > 0 and s["max_tile_particles"] > 1024
+        if mode == 1:
+            assert s["light_tile_windows"] == 0
+        if mode == 2 and particles == 20000:
+            assert s["light_tile_windows"] > 0
+        w.close()
 
 
 def test_parity_cpu256_workload(statics):
