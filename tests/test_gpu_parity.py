@@ -190,15 +190,20 @@ def test_crowded_tiles(statics, nx, ny, particles, settling):
     w.close()
 
 
-@pytest.mark.parametrize("particles,modes", [(90000, (1, 0, 2)), (20000, (2,))])
-def test_heavy_first_tiles(statics, particles, modes):
+@pytest.mark.parametrize("particles,modes,periodic,order", [
+    (90000, (1, 0, 2), True, 0),
+    (20000, (2,), True, 0),
+    (20000, (2,), False, 0),  # bounded map: particles leave, tiles at the edges
+    (20000, (2,), True, 4),   # light tiles' home XCD under a round-robin tile order
+])
+def test_heavy_first_tiles(statics, particles, modes, periodic, order):
     """SWE_TUNE_HEAVY_FIRST: tiles that received more than 1024 particles in a
     window run on the first workgroups of the next launch (mode 1), and in
     mode 2, the default, tiles with at most 256 also run on the last ones,
     listed per XCD.  128x256 is 64 tiles: 8 heavy and 32 light slots, so tiles
     are listed, skipped and left over (~1400 particles per tile at 90000,
     ~300 at 20000).  Results are the oracle's bits in every mode."""
-    say("case heavy-first", particles, modes)
+    say("case heavy-first", particles, modes, periodic, order)
     Wind.suspension = 1e-4
     World.settling = 0.02
     World.SEED = 6
@@ -207,9 +212,10 @@ def test_heavy_first_tiles(statics, particles, modes):
     h = (np.random.default_rng(0).random((nx, ny)) * 0.5).astype(np.float32)
     st = None
     for mode in modes:
-        w = World(nx, ny, periodic=True, window=1, layers=2, abrasion=1e-3)
+        w = World(nx, ny, periodic=periodic, window=1, layers=2, abrasion=1e-3)
         w.upload(h, sed)
         w.set_path(2)
+        w.set_tune(capi.SWE_TUNE_TILE_ORDER, order)
         if st is None:
             st = O.OracleState(nx, ny, h, sediment=sed, params=O.params_from(w.params()))
             st.erode(particles, 1)
