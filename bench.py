@@ -117,6 +117,11 @@ def bench_batch(args):
 
     terrains = workloads.batch64(seed=args.seed)
     b = workloads.make_batch(terrains)
+    from simplewinderosion_amd import capi
+    for knob, v in ((capi.SWE_TUNE_TILE_ORDER, args.tile_order), (capi.SWE_TUNE_GRAPH, args.graph),
+                    (capi.SWE_TUNE_HEAVY_FIRST, args.heavy_first)):
+        if v is not None:
+            b.set_tune(knob, v)
     parts = [t["particles"] for t in terrains]
     print("bench: batch64, %d terrains, %d cells, warmup %d, steps %d"
           % (len(terrains), sum(parts), args.warmup, args.steps), file=sys.stderr, flush=True)

@@ -35,7 +35,7 @@ struct swe_ctx {
   int32_t path = 0;  // 0 auto, 1 global-atomic march, 2 tile-owned step
   int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
   int32_t use_graph = 1;   // SWE_TUNE_GRAPH
-  int32_t heavy_first = 2; // SWE_TUNE_HEAVY_FIRST
+  int32_t heavy_first = -1;  // SWE_TUNE_HEAVY_FIRST; -1 = default (2 on one terrain, 1 on a batch)
   // one captured map-cycle of the tile path per starting window parity, and
   // the launch configuration it was captured for
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -192,6 +192,11 @@ int ensure_particles(swe_ctx* c, int64_t n) {
   c->cap = n;
   return 0;
 }
+
+// SWE_TUNE_HEAVY_FIRST in effect: by default light tiles also go last on a
+// one-terrain map (measured faster there) but not in a batch of many small
+// terrains (measured slower)
+int32_t heavy_first_mode(const swe_ctx* c) { return c->heavy_first >= 0 ? c->heavy_first : (c->nterr == 1 ? 2 : 1); }
 
 // light-tile slots of a launch: half the tiles, a multiple of 8
 int64_t light_cap(int64_t ntiles) { return (ntiles / 2 + 7) / 8 * 8; }
@@ -426,10 +431,11 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sa.tile_order = c->tile_order;
   // listed crowded tiles get the first workgroups of a launch; at most one
   // slot per 8 tiles, a multiple of 8 (none on maps under 64 tiles)
-  sa.heavy_slots = c->heavy_first ? (int32_t)std::min<int64_t>(swe::kHeavySlotsMax, (c->ntiles / 8) & ~7LL) : 0;
+  const int32_t hf = heavy_first_mode(c);
+  sa.heavy_slots = hf ? (int32_t)std::min<int64_t>(swe::kHeavySlotsMax, (c->ntiles / 8) & ~7LL) : 0;
   // light tiles last (heavy-first mode 2, maps of 64 tiles or more): the
   // usual order is padded to a multiple of 8 so that slot b runs on XCD b % 8
-  const bool light = c->heavy_first == 2 && sa.heavy_slots > 0;
+  const bool light = hf == 2 && sa.heavy_slots > 0;
   sa.light_slots = light ? (int32_t)light_cap(c->ntiles) : 0;
   sa.default_slots = light ? (int32_t)((c->ntiles + 7) / 8 * 8) : c->ntiles;
 
@@ -968,7 +974,7 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
     if (e0) (void)hipEventDestroy(e0);
     return fail(c, SWE_EHIP, std::string("autotune set-up: ") + hipGetErrorString(he));
   }
-  const int32_t order0 = c->tile_order, graph0 = c->use_graph, heavy0 = c->heavy_first, timing0 = c->timing;
+  const int32_t order0 = c->tile_order, graph0 = c->use_graph, heavy0 = heavy_first_mode(c), timing0 = c->timing;
   c->timing = 0;
   static const int32_t kOrders[] = {0, 1, 4, 16};
   double best = 1e300;
@@ -1067,7 +1073,7 @@ int swe_get_tune(swe_ctx* c, int32_t knob, int64_t* value) {
       *value = c->use_graph;
       return 0;
     case SWE_TUNE_HEAVY_FIRST:
-      *value = c->heavy_first;
+      *value = heavy_first_mode(c);
       return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");

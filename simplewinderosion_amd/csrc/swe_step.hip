@@ -141,10 +141,10 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
   }
   if (u >= a.ntiles) return;  // padding of the usual order to a multiple of 8
-  // a listed tile's stamp is window + 1 (or window + 2 once its heavy
-  // workgroup has listed it again): either way the usual one skips it.  The
-  // stamp is checked once the relax loads are out (nothing is written before).
-  const uint32_t stamp = (hs | ls) > 0 && !listed ? a.hstamp[u] : 0u;
+  // a listed tile's stamp is window + 1 (or window + 2 once its listed
+  // workgroup has listed it again): either way the usual one skips it,
+  // before it loads anything
+  if ((hs | ls) > 0 && !listed && a.hstamp[u] > (uint32_t)a.window) return;
   // the terrain this tile belongs to; all map indices below are terrain-local
   const Terrain T = a.terr[a.nterr == 1 ? 0 : a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
@@ -294,7 +294,6 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     uint16_t* s_cas = (uint16_t*)(s_casc + kH1X * kH1Y);  // H2 cells with a transfer
     for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
     if (threadIdx.x == 0) s_ncas = 0;
-    if (stamp > (uint32_t)a.window) return;  // uniform: the whole workgroup leaves
     __syncthreads();
 
     // visited cells within two cells of the tile whose cascade moves
