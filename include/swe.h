@@ -168,8 +168,9 @@ int swe_set_path(swe_ctx* ctx, int32_t path);
  *     last; 2 = also a tile that received at most 256 is handed to one of
  *     the last workgroups (on its own XCD), so short workgroups fill the
  *     launch's tail; 0 = every tile in the SWE_TUNE_TILE_ORDER order.
- *     Default: 2 on a one-terrain context, 1 on a batch (measured best on
- *     each).  Maps of fewer than 64 tiles always use 0.
+ *     Default: 2 for contexts of up to 4096 tiles (16x32 cells each), 1
+ *     above (measured best on a 1024^2 map and on a 64-terrain batch).
+ *     Contexts of fewer than 64 tiles always use 0.
  */
 enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2, SWE_TUNE_HEAVY_FIRST = 3 };
 int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);

@@ -53,7 +53,7 @@ def test_batch64_workload_builds_and_runs_one_cycle(statics):
     terr = workloads.batch64(seed=3)
     print("case batch64 cells", sum(t["nx"] * t["ny"] for t in terr), flush=True)
     b = workloads.make_batch(terr)
-    assert b.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 1  # the batch default
+    assert b.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 1  # the default above 4096 tiles
     m0 = [b.audit(k)["height_sum"] for k in range(len(terr))]
     b.erode([t["particles"] for t in terr], n_cycles=1)
     b.synchronize()

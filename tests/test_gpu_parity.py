@@ -199,8 +199,8 @@ def test_crowded_tiles(statics, nx, ny, particles, settling):
 def test_heavy_first_tiles(statics, particles, modes, periodic, order):
     """SWE_TUNE_HEAVY_FIRST: tiles that received more than 1024 particles in a
     window run on the first workgroups of the next launch (mode 1), and in
-    mode 2, the default on one terrain, tiles with at most 256 also run on
-    the last ones, listed per XCD.  128x256 is 64 tiles: 8 heavy and 32 light slots, so tiles
+    mode 2, the default up to 4096 tiles, tiles with at most 256 also run
+    on the last ones, listed per XCD.  128x256 is 64 tiles: 8 heavy and 32 light slots, so tiles
     are listed, skipped and left over (~1400 particles per tile at 90000,
     ~300 at 20000).  Results are the oracle's bits in every mode."""
     say("case heavy-first", particles, modes, periodic, order)
@@ -219,7 +219,7 @@ def test_heavy_first_tiles(statics, particles, modes, periodic, order):
         if st is None:
             st = O.OracleState(nx, ny, h, sediment=sed, params=O.params_from(w.params()))
             st.erode(particles, 1)
-        assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 2  # the one-terrain default
+        assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == 2  # the default at 64 tiles
         w.set_tune(capi.SWE_TUNE_HEAVY_FIRST, mode)
         assert w.get_tune(capi.SWE_TUNE_HEAVY_FIRST) == mode
         w.erode(particles, n_cycles=1)
