@@ -154,9 +154,6 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   const int32_t tx = ul / tiles_y, ty = ul % tiles_y;
   const int32_t x0 = tx * kSX, y0 = ty * kSY;
   const float rcp_nx = T.rcp_nx, rcp_ny = T.rcp_ny;
-  // periodic map, tile and 3-cell halo inside it (most tiles of a large
-  // map): the halo loads below need no wrap
-  const bool inner = false;
   const int64_t c0 = T.cell0;
   const int64_t* zm_in = a.zm_in + c0;
   int64_t* zm_out = a.zm_out + c0;
@@ -206,10 +203,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 1 + e / kH1Y, gy = y0 - 1 + e % kH1Y;
       bool ok = e < kH1X * kH1Y;
       if (PERIODIC) {
-        if (!inner) {
-          gx = wrap(gx, nx, rcp_nx);
-          gy = wrap(gy, ny, rcp_ny);
-        }
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -229,10 +224,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 3 + e / kH3Y, gy = y0 - 3 + e % kH3Y;
       bool ok = e < kH3X * kH3Y;
       if (PERIODIC) {
-        if (!inner) {
-          gx = wrap(gx, nx, rcp_nx);
-          gy = wrap(gy, ny, rcp_ny);
-        }
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -246,10 +239,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 2 + e / kH2Y, gy = y0 - 2 + e % kH2Y;
       bool ok = e < kH2X * kH2Y;
       if (PERIODIC) {
-        if (!inner) {
-          gx = wrap(gx, nx, rcp_nx);
-          gy = wrap(gy, ny, rcp_ny);
-        }
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
@@ -382,10 +373,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int32_t gx = x0 - 1 + lx, gy = y0 - 1 + ly;
       bool ok = true;
       if (PERIODIC) {
-        if (!inner) {
-          gx = wrap(gx, nx, rcp_nx);
-          gy = wrap(gy, ny, rcp_ny);
-        }
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
       } else {
         ok = gx >= 0 && gy >= 0 && gx < nx && gy < ny;
       }
