@@ -82,7 +82,7 @@ struct swe_ctx {
   uint32_t* nin = nullptr;     // 3 x tiles
   uint32_t* csum = nullptr;    // 3 x chunks of kStepChunk tiles
   uint32_t* heavy[2] = {nullptr, nullptr};  // kHeavySlotsMax crowded + light_cap light tiles listed for a window
-  uint32_t* hcnt = nullptr;    // 3 x 9: their counts (heavy, light per XCD)
+  uint32_t* hcnt = nullptr;    // 3 x 16: their counts (heavy per XCD, light per XCD)
   uint32_t* hstamp = nullptr;  // per tile: window + 1 when listed
   int32_t* sp_tile = nullptr;
   uint32_t* sp_rank = nullptr;
@@ -217,7 +217,7 @@ int ensure_step(swe_ctx* c, int64_t n) {
         (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
         (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
         (r = dalloc(c, &c->csum, 3 * nch)) || (r = dalloc(c, &c->heavy[0], swe::kHeavySlotsMax + light_cap(nt))) ||
-        (r = dalloc(c, &c->heavy[1], swe::kHeavySlotsMax + light_cap(nt))) || (r = dalloc(c, &c->hcnt, 27)) ||
+        (r = dalloc(c, &c->heavy[1], swe::kHeavySlotsMax + light_cap(nt))) || (r = dalloc(c, &c->hcnt, 48)) ||
         (r = dalloc(c, &c->hstamp, nt)))
       return r;
   }
@@ -469,7 +469,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
       swe::launch_zero32(c->csum, 3 * nch, c->stream);
       swe::launch_zero32(c->farcnt, 3, c->stream);
       if (sa.heavy_slots > 0) {
-        swe::launch_zero32(c->hcnt, 27, c->stream);
+        swe::launch_zero32(c->hcnt, 48, c->stream);
         swe::launch_zero32(c->hstamp, nt, c->stream);
       }
       swe::launch_spawn(sp, c->stream);
@@ -509,9 +509,9 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         sa.window = (int32_t)w;
         sa.heavy_cur = c->heavy[wp];
         sa.heavy_next = c->heavy[wp ^ 1];
-        sa.hcnt_cur = c->hcnt + 9 * w3;
-        sa.hcnt_next = c->hcnt + 9 * w3n;
-        sa.hcnt_clear = c->hcnt + 9 * w3c;
+        sa.hcnt_cur = c->hcnt + 16 * w3;
+        sa.hcnt_next = c->hcnt + 16 * w3n;
+        sa.hcnt_clear = c->hcnt + 16 * w3c;
         sa.hstamp = c->hstamp;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int rr;

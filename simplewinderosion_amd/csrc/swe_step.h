@@ -103,9 +103,10 @@ struct StepArgs {
   int32_t default_slots;     // workgroups of the usual order: ntiles (a multiple of 8 with light slots)
   int32_t light_slots;       // 0 = off, else a multiple of 8: light tiles last, per XCD
   int32_t window;            // window index within the cycle
-  const uint32_t* heavy_cur; // this launch's listed tiles: kHeavySlotsMax heavy, then light_slots light
+  const uint32_t* heavy_cur; // this launch's listed tiles: kHeavySlotsMax heavy, then light_slots light,
+                             // each part in 8 equal per-XCD lists
   uint32_t* heavy_next;      // the next launch's (appended here)
-  const uint32_t* hcnt_cur;  // list lengths (may exceed the slots): [0] heavy, [1 + x] light on XCD x
+  const uint32_t* hcnt_cur;  // list lengths (may exceed the slots): [x] heavy, [8 + x] light on XCD x
   uint32_t* hcnt_next;
   uint32_t* hcnt_clear;      // the lengths of the lists after next
   uint32_t* hstamp;          // per tile: w + 1 when listed for window w (0 at cycle start)

@@ -112,12 +112,12 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   // dealt round-robin over the XCDs (evens out crowded regions); the last
   // nb mod 8C workgroups keep their own index.  Both are bijections.
   // The first heavy_slots workgroups take the tiles listed as crowded by the
-  // previous launch (empty slots return at once); the rest follow the
-  // usual order and skip a listed tile.  heavy_slots is a multiple of 8, so
-  // b and b - heavy_slots fall on the same XCD.
-  // Light tiles (at most kLightMax arrivals last window) are listed per XCD
-  // for the last light_slots workgroups, which run on their home XCD and
-  // start last, when the launch's long workgroups are already running.
+  // previous launch, the last light_slots ones the tiles listed as light
+  // (at most kLightMax arrivals), both listed per home XCD (the XCD that
+  // takes the tile in the usual order), so a listed tile keeps its L2;
+  // empty slots return at once.  The rest follow the usual order and skip
+  // a listed tile.  heavy_slots and default_slots (with light slots) are
+  // multiples of 8, so slot b of each part runs on XCD b % 8.
   const uint32_t hs = RELAX_ONLY ? 0u : (uint32_t)a.heavy_slots;
   const uint32_t ls = RELAX_ONLY ? 0u : (uint32_t)a.light_slots;
   const uint32_t nb = (uint32_t)a.default_slots, b = blockIdx.x - (uint32_t)a.heavy_slots;
@@ -125,12 +125,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   int32_t u;  // global tile
   bool listed = false;
   if (blockIdx.x < (uint32_t)a.heavy_slots) {
-    if (hs == 0 || blockIdx.x >= a.hcnt_cur[0]) return;
-    u = (int32_t)a.heavy_cur[blockIdx.x];
+    const uint32_t k = blockIdx.x / 8;  // heavy slot k of this XCD (blockIdx.x % 8 == xcd)
+    if (hs == 0 || k >= a.hcnt_cur[xcd]) return;
+    u = (int32_t)a.heavy_cur[xcd * (hs / 8) + k];
     listed = true;
   } else if (b >= nb) {
     const uint32_t bl = b - nb, k = bl / 8;
-    if (ls == 0 || k >= a.hcnt_cur[1 + xcd]) return;
+    if (ls == 0 || k >= a.hcnt_cur[8 + xcd]) return;
     u = (int32_t)a.heavy_cur[kHeavySlotsMax + xcd * (ls / 8) + k];
     listed = true;
   } else if (a.tile_order <= 0) {
@@ -141,6 +142,15 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     u = b < full ? (int32_t)(((j / C) * 8 + xcd) * C + j % C) : (int32_t)b;
   }
   if (u >= a.ntiles) return;  // padding of the usual order to a multiple of 8
+  // home XCD of tile t in the usual order: the XCD whose workgroups take it
+  auto home_of = [=](uint32_t t) __attribute__((always_inline)) -> uint32_t {
+    if (a.tile_order <= 0) {
+      const uint32_t q8 = nb / 8, r8 = nb % 8;
+      return t < r8 * (q8 + 1) ? t / (q8 + 1) : r8 + (t - r8 * (q8 + 1)) / q8;
+    }
+    const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C);
+    return t < full ? (t / C) % 8 : t % 8;
+  };
   // a listed tile's stamp is window + 1 (or window + 2 once its listed
   // workgroup has listed it again): either way the usual one skips it,
   // before it loads anything
@@ -666,26 +676,20 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if (u == 0) {
       *a.far_cnt_clear = 0;
       if ((hs | ls) > 0)
-        for (int i = 0; i < 9; i++) a.hcnt_clear[i] = 0;
+        for (int i = 0; i < 16; i++) a.hcnt_clear[i] = 0;
     }
     // crowded (or light) now: listed for the next launch (the count moves
     // little from one window to the next)
     if (hs > 0 && nin_u > kHeavyMin) {
-      const uint32_t slot = atomicAdd(&a.hcnt_next[0], 1u);
-      if (slot < hs) {
-        a.heavy_next[slot] = (uint32_t)u;
+      const uint32_t home = home_of((uint32_t)u);
+      const uint32_t slot = atomicAdd(&a.hcnt_next[home], 1u);
+      if (slot < hs / 8) {
+        a.heavy_next[home * (hs / 8) + slot] = (uint32_t)u;
         a.hstamp[u] = (uint32_t)a.window + 2u;
       }
     } else if (ls > 0 && nin_u <= kLightMax) {
-      // home XCD of u in the usual order (nb is a multiple of 8 here)
-      uint32_t home;
-      if (a.tile_order <= 0) {
-        home = (uint32_t)u / (nb / 8);
-      } else {
-        const uint32_t C = (uint32_t)a.tile_order, full = nb / (8 * C) * (8 * C);
-        home = (uint32_t)u < full ? ((uint32_t)u / C) % 8 : (uint32_t)u % 8;
-      }
-      const uint32_t slot = atomicAdd(&a.hcnt_next[1 + home], 1u);
+      const uint32_t home = home_of((uint32_t)u);
+      const uint32_t slot = atomicAdd(&a.hcnt_next[8 + home], 1u);
       if (slot < ls / 8) {
         a.heavy_next[kHeavySlotsMax + home * (ls / 8) + slot] = (uint32_t)u;
         a.hstamp[u] = (uint32_t)a.window + 2u;
