@@ -708,19 +708,23 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     if ((threadIdx.x & 63) == 0) s_wstat[threadIdx.x >> 6] = ((uint64_t)far_here << 32) | steps_here;
   }
   __syncthreads();
-  if (!RELAX_ONLY && threadIdx.x == 0) {
+  if (!RELAX_ONLY && threadIdx.x < 9) {
     // chunk sums of the next window's arrivals: one atomic per distinct chunk
-    // (the nine destinations mostly share one or two)
-    for (int d = 0; d < 9; d++) {
-      const int32_t ch = s_dchunk[d];
-      if (ch < 0) continue;
-      bool first = true;
-      for (int e = 0; e < d; e++) first = first && s_dchunk[e] != ch;
-      if (!first) continue;
+    // (the nine destinations mostly share one or two), by the thread of the
+    // chunk's first direction
+    const int d = threadIdx.x;
+    const int32_t ch = s_dchunk[d];
+    bool first = ch >= 0;
+#pragma unroll
+    for (int e = 0; e < 9; e++) first = first && !(e < d && s_dchunk[e] == ch);
+    if (first) {
       uint32_t n = 0;
-      for (int e = d; e < 9; e++) n += s_dchunk[e] == ch ? s_slot[e] : 0u;
+#pragma unroll
+      for (int e = 0; e < 9; e++) n += e >= d && s_dchunk[e] == ch ? s_slot[e] : 0u;
       atomicAdd(&a.csum_next[ch], n);
     }
+  }
+  if (!RELAX_ONLY && threadIdx.x == 0) {
     // statistics: one atomic per workgroup and kind
     uint64_t steps = 0, far = 0;
     for (int i = 0; i < kStepThreads / 64; i++) {
