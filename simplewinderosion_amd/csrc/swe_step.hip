@@ -197,6 +197,8 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
   }
   const uint32_t nin_u = RELAX_ONLY ? 0u : a.nin_cur[u];
 
+  PRec rec0{};       // this thread's first arrival (loaded in the relax phase)
+  uint32_t sr0 = 0;  // its sub-range
   // ---------------- 1. relax of the previous window ----------------
   {
     float* s_old = (float*)s_work;
@@ -296,6 +298,16 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     for (int e = threadIdx.x; e < kH1X * kH1Y; e += kStepThreads) s_casc[e] = 0;
     if (threadIdx.x == 0) s_ncas = 0;
     __syncthreads();
+    // start of each arrival sub-range in this tile's arrival order (read
+    // after the next barrier)
+    if (threadIdx.x == 0) {
+      uint32_t r = 0;
+      for (int d = 0; d < 9; d++) {
+        s_rpre[d] = r;
+        r += s_slot[d];
+      }
+      s_rpre[9] = r;
+    }
 
     // visited cells within two cells of the tile whose cascade moves
     // material: one neighbour beyond the threshold at the window's start
@@ -375,6 +387,12 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     static_assert(8 * kCell * (LAYERS == 2 ? 2 : 1) <= 4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y,
                   "the pickup accumulators must not overlap s_casc");
     int64_t* acc_z = (int64_t*)s_work;
+    // each thread's first arrival record, loaded now so that it arrives
+    // during the rest of the relax
+    if (!RELAX_ONLY && threadIdx.x < s_rpre[9]) {
+      while (sr0 < 8 && threadIdx.x >= s_rpre[sr0 + 1]) sr0++;
+      rec0 = a.in[s_rbase[sr0] + (int64_t)(threadIdx.x - s_rpre[sr0])];
+    }
 #pragma unroll
     for (int i = 0; i < kH1Per; i++) {
       const int e = threadIdx.x + i * kStepThreads;
@@ -426,12 +444,6 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       int64_t b0 = 0;
       for (int i = 0; i < kStepThreads / 64; i++) b0 += s_lost[i];
       s_base = 9 * b0;
-      uint32_t r = 0;
-      for (int d = 0; d < 9; d++) {
-        s_rpre[d] = r;
-        r += s_slot[d];
-      }
-      s_rpre[9] = r;
     }
   }
 
@@ -578,14 +590,13 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     // thread's e increase, so its sub-range cursor only moves forward.  The
     // next record is loaded while the current one is stepped.
     const uint32_t n_in = s_rpre[9];
-    uint32_t sr = 0;
+    uint32_t sr = sr0;
     auto index_of = [&](uint32_t e) __attribute__((always_inline)) {
       while (sr < 8 && e >= s_rpre[sr + 1]) sr++;
       return s_rbase[sr] + (int64_t)(e - s_rpre[sr]);
     };
     {
-      Rec cur{};
-      if (threadIdx.x < n_in) cur = load(a.in, index_of(threadIdx.x));
+      Rec cur = rec0;
       for (uint32_t e = threadIdx.x; e < n_in; e += kStepThreads) {
         Rec nxt{};
         if (e + kStepThreads < n_in) nxt = load(a.in, index_of(e + kStepThreads));
