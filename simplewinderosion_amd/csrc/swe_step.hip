@@ -378,6 +378,12 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
       if (centre != 0 && hx >= 0 && hy >= 0 && hx < kH1X && hy < kH1Y)
         atomicAdd((unsigned long long*)&s_casc[hx * kH1Y + hy], (unsigned long long)centre);
     }
+    // each thread's first arrival record, loaded now so that it arrives
+    // while the other threads finish their cascades and the relax ends
+    if (!RELAX_ONLY && threadIdx.x < s_rpre[9]) {
+      while (sr0 < 8 && threadIdx.x >= s_rpre[sr0 + 1]) sr0++;
+      rec0 = a.in[s_rbase[sr0] + (int64_t)(threadIdx.x - s_rpre[sr0])];
+    }
     __syncthreads();
 
     // master += cascade; next snapshot for the tile and its halo.  The
@@ -387,12 +393,6 @@ __global__ __launch_bounds__(kStepThreads) __attribute__((amdgpu_waves_per_eu(SW
     static_assert(8 * kCell * (LAYERS == 2 ? 2 : 1) <= 4 * kH3X * kH3Y * (LAYERS == 2 ? 2 : 1) + 4 * kH2X * kH2Y,
                   "the pickup accumulators must not overlap s_casc");
     int64_t* acc_z = (int64_t*)s_work;
-    // each thread's first arrival record, loaded now so that it arrives
-    // during the rest of the relax
-    if (!RELAX_ONLY && threadIdx.x < s_rpre[9]) {
-      while (sr0 < 8 && threadIdx.x >= s_rpre[sr0 + 1]) sr0++;
-      rec0 = a.in[s_rbase[sr0] + (int64_t)(threadIdx.x - s_rpre[sr0])];
-    }
 #pragma unroll
     for (int i = 0; i < kH1Per; i++) {
       const int e = threadIdx.x + i * kStepThreads;
