@@ -173,9 +173,9 @@ def main():
         return
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the map would shard by "
-              "terrain or by particles with an int64 all-reduce of the delta map; see DESIGN.md)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (a batch would shard by "
+              "terrain; one map by bands of tiles that exchange snapshot halos and crossing particles "
+              "with their two neighbours after each window)", file=sys.stderr)
         sys.exit(2)
 
     if args.workload == "batch64":
