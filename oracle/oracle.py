@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SWE_ORACLE_LIB names another build of swe_oracle.c (tests/test_oracle_mutants.py
+# points it at copies with planted mistakes)
+LIB_PATH = os.environ.get("SWE_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 
 class OrParams(ctypes.Structure):
@@ -122,6 +124,12 @@ def load():
         "or_erode_seq": (ctypes.c_int64, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _F, ctypes.c_int64,
                                           ctypes.c_int32]),
         "or_params_size": (ctypes.c_int32, []),
+        "or_fly_once": (ctypes.c_int32, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _F,
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.c_uint64, _F, _I64]),
+        "or_fly_seq_once": (ctypes.c_int32, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, _F,
+                                             ctypes.POINTER(ctypes.c_int32), _F]),
+        "or_cascade_seq": (None, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, ctypes.c_int32, ctypes.c_int32]),
+        "or_trace_len": (ctypes.c_int32, []),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -218,3 +226,63 @@ def erode_seq(params, nx, ny, height, sediment=None, ema=None, particles=0, n_cy
         lib.or_srand(int(seed) & 0xFFFFFFFF)
     return lib.or_erode_seq(ctypes.byref(params), int(nx), int(ny), _f(height), _f(sediment), _f(ema),
                             int(particles), int(n_cycles))
+
+
+# Stage trace of one fly() call (or_fly_once / or_fly_seq_once): name -> slice
+TRACE = {
+    "n": slice(0, 3), "h": 3, "hfac": 4, "shadow": 5, "sp_prevailing": slice(6, 9),
+    "sp_gravity": slice(9, 12), "collision": 12, "rspeed": slice(13, 16), "sp_terrain": slice(16, 19),
+    "draws": slice(19, 22), "sp_turbulence": slice(22, 25), "drag": 25, "speed": slice(26, 29),
+    "force": 29, "lift": 30, "capacity": 31, "diff": 32,
+}
+
+
+def _trace_dict(tr):
+    return {k: (tr[v].copy() if isinstance(v, slice) else np.float32(tr[v])) for k, v in TRACE.items()}
+
+
+def fly_once(params, snapshot, state, age, key, sed_snapshot=None):
+    """One batched fly() call of a particle in `state` (pos xyz, speed xyz,
+    sediment) with `age` calls made, against a frozen float32 snapshot.
+    Returns (alive, new_state, new_age, trace dict, fixed-point pickup)."""
+    lib = load()
+    S = np.ascontiguousarray(snapshot, np.float32)
+    nx, ny = S.shape
+    SS = np.zeros_like(S) if sed_snapshot is None else np.ascontiguousarray(sed_snapshot, np.float32)
+    st = np.ascontiguousarray(state, np.float32).copy()
+    a = ctypes.c_int32(int(age))
+    tr = np.full(lib.or_trace_len(), np.nan, np.float32)
+    pick = ctypes.c_int64(0)
+    ok = lib.or_fly_once(ctypes.byref(params), nx, ny, _f(S), _f(SS), _f(st), ctypes.byref(a),
+                         ctypes.c_uint64(int(key)), _f(tr), ctypes.byref(pick))
+    if ok < 0:
+        raise MemoryError("or_fly_once")
+    return bool(ok), st, a.value, _trace_dict(tr), pick.value
+
+
+def fly_seq_once(params, height, state, age, seed, sediment=None):
+    """One sequential fly() call (glibc rand after srand(seed), libm expf, the
+    map written in place).  Returns (alive, new_state, new_age, trace, height
+    after, sediment after)."""
+    lib = load()
+    H = np.ascontiguousarray(height, np.float32).copy()
+    nx, ny = H.shape
+    Sd = None if sediment is None else np.ascontiguousarray(sediment, np.float32).copy()
+    st = np.ascontiguousarray(state, np.float32).copy()
+    a = ctypes.c_int32(int(age))
+    tr = np.full(lib.or_trace_len(), np.nan, np.float32)
+    lib.or_srand(int(seed) & 0xFFFFFFFF)
+    ok = lib.or_fly_seq_once(ctypes.byref(params), nx, ny, _f(H), _f(Sd), _f(st), ctypes.byref(a), _f(tr))
+    if ok < 0:
+        raise MemoryError("or_fly_seq_once")
+    return bool(ok), st, a.value, _trace_dict(tr), H, Sd
+
+
+def cascade_seq(params, height, x, y, sediment=None):
+    """One sequential World::cascade at (x, y) on a float map; returns the new map(s)."""
+    lib = load()
+    H = np.ascontiguousarray(height, np.float32).copy()
+    nx, ny = H.shape
+    Sd = None if sediment is None else np.ascontiguousarray(sediment, np.float32).copy()
+    lib.or_cascade_seq(ctypes.byref(params), nx, ny, _f(H), _f(Sd), int(x), int(y))
+    return H, Sd

@@ -436,6 +436,74 @@ static inline void add32(uint32_t* a, uint32_t v)
   *a += v;
 }
 
+/*
+ * Stages of one fly() call, exported by or_fly_once / or_fly_seq_once so that
+ * tests can check each line of wind.hpp:40-112 on its own.  Layout (floats):
+ *   0-2 n (:40)           3 h (cell height)      4 hfac (:50-52)
+ *   5 shadow (:56-59)     6-8 speed after :61    9-11 after gravity :65-66
+ *   12 collision (:70-71) 13-15 rspeed (:75)     16-18 after :79
+ *   19-21 turbulence draws (:83)                 22-24 after :83
+ *   25 drag factor (:87)  26-28 speed after :87  29 force (:100-102)
+ *   30 lift (:104)        31 capacity (:106)     32 diff (:110)
+ */
+#define OR_TRACE_LEN 33
+
+/*
+ * wind.hpp:56-106 -- from the surface normal, hfac, the speed and the load
+ * to the new speed, the collision factor and the transport capacity.  The
+ * batched and the sequential restatement both call this one copy; they
+ * differ only in where n, hfac and the three turbulence draws come from.
+ *   airborne: pos.y > cell height after the ground clamp (:65)
+ */
+static v3 fly_dynamics(v3 ps, v3 nps, float gravity, v3 n, int airborne, float hfac, v3 sp, float sed,
+                       const int32_t r[3], float* capacity_out, float* tr)
+{
+  float shadow = dot3(nps, n);                                      /* :56-59 */
+  if (shadow < 0.0f) shadow = 0.0f;
+  shadow = 1.0f - shadow;
+
+  /* :61 -- (0.1f+0.9f*shadow)*pspeed: glm's scalar*vec multiplies each component */
+  sp = vadd(sp, smul(0.05f, vsub(mulsv(ps, 0.1f + 0.9f * shadow), sp)));
+  if (tr) { tr[5] = shadow; tr[6] = sp.x; tr[7] = sp.y; tr[8] = sp.z; }
+
+  if (airborne) sp.y -= gravity * sed;                               /* :65-66 */
+  if (tr) { tr[9] = sp.x; tr[10] = sp.y; tr[11] = sp.z; }
+
+  float collision = -dot3(norm3(sp), n);                             /* :70-71 */
+  if (collision < 0.0f) collision = 0.0f;
+
+  v3 rspeed = cross3(n, cross3(smul(1.0f - collision, sp), n));     /* :75 */
+  if (tr) { tr[12] = collision; tr[13] = rspeed.x; tr[14] = rspeed.y; tr[15] = rspeed.z; }
+
+  sp = vadd(sp, smul(0.9f, vsub(smul(shadow, mix3(ps, rspeed, shadow * hfac)), sp)));  /* :79 */
+  if (tr) { tr[16] = sp.x; tr[17] = sp.y; tr[18] = sp.z; }
+
+  {                                                                  /* :83 */
+    /* 0.1f*hfac*collision*(vec3(r)-500.0f)/500.0f, left to right */
+    float k = 0.1f * hfac * collision;
+    v3 t = vsubs(V3((float)r[0], (float)r[1], (float)r[2]), 500.0f);
+    sp = vadd(sp, vdivs(smul(k, t), 500.0f));
+  }
+  if (tr) {
+    tr[19] = (float)r[0]; tr[20] = (float)r[1]; tr[21] = (float)r[2];
+    tr[22] = sp.x; tr[23] = sp.y; tr[24] = sp.z;
+  }
+
+  /* :87 -- 1.0f - 0.3*sediment is a double; glm's vec3 *= double converts
+     the factor to float first */
+  const float drag = (float)(1.0 - 0.3 * (double)sed);
+  sp = mulsv(sp, drag);
+  if (tr) { tr[25] = drag; tr[26] = sp.x; tr[27] = sp.y; tr[28] = sp.z; }
+
+  float force = -dot3(norm3(sp), n) * len3(sp);                      /* :100-102 */
+  if (force < 0.0f) force = 0.0f;
+  const float lift = (1.0f - collision) * len3(sp);                  /* :104 */
+  const float capacity = force * hfac + 0.02f * lift * hfac;        /* :106 */
+  if (tr) { tr[29] = force; tr[30] = lift; tr[31] = capacity; tr[32] = capacity - sed; }
+  *capacity_out = capacity;
+  return sp;
+}
+
 static void or_die(or_march* m, or_wind* w, int64_t cell)
 {
   if (m->p->deposit_on_death && w->ledger != 0) {
@@ -452,7 +520,7 @@ static void or_die(or_march* m, or_wind* w, int64_t cell)
  * Wind::fly (wind.hpp:28-117) against the snapshot.  Returns 1 when the call
  * is a particle-step (fly() == true), 0 when the particle dies.
  */
-static int or_fly(or_march* m, or_wind* w)
+static int or_fly(or_march* m, or_wind* w, float* tr)
 {
   const or_params* p = m->p;
   const int32_t nx = m->nx, ny = m->ny;
@@ -476,36 +544,13 @@ static int or_fly(or_march* m, or_wind* w)
 
   float hfac = or_expf(-(w->py - h) / p->boundarylayer);            /* :50-52 */
   if (hfac < 0.0f) hfac = 0.0f;
-
-  float shadow = dot3(m->nps, n);                                   /* :56-59 */
-  if (shadow < 0.0f) shadow = 0.0f;
-  shadow = 1.0f - shadow;
-
-  v3 sp = V3(w->sx, w->sy, w->sz);
-  sp = vadd(sp, smul(0.05f, vsub(mulsv(m->ps, 0.1f + 0.9f * shadow), sp)));  /* :61 */
-  /* (0.1f+0.9f*shadow)*pspeed: glm's scalar*vec multiplies each component */
-
-  if (w->py > h) sp.y -= p->gravity * w->sed;                        /* :65-66 */
-
-  float collision = -dot3(norm3(sp), n);                             /* :70-71 */
-  if (collision < 0.0f) collision = 0.0f;
-
-  v3 rspeed = cross3(n, cross3(smul(1.0f - collision, sp), n));     /* :75 */
-
-  sp = vadd(sp, smul(0.9f, vsub(smul(shadow, mix3(m->ps, rspeed, shadow * hfac)), sp)));  /* :79 */
+  if (tr) { tr[0] = n.x; tr[1] = n.y; tr[2] = n.z; tr[3] = h; tr[4] = hfac; }
 
   int32_t r[3];                                                      /* :83 */
   or_turb(w->key, age, r);
-  {
-    float k = 0.1f * hfac * collision;
-    v3 t = vsubs(V3((float)r[0], (float)r[1], (float)r[2]), 500.0f);
-    sp = vadd(sp, vdivs(smul(k, t), 500.0f));
-  }
-
-  {                                                                  /* :87 */
-    float drag = (float)(1.0 - 0.3 * (double)w->sed);
-    sp = mulsv(sp, drag);
-  }
+  float capacity;
+  v3 sp = fly_dynamics(m->ps, m->nps, p->gravity, n, w->py > h, hfac, V3(w->sx, w->sy, w->sz), w->sed, r,
+                       &capacity, tr);
 
   w->px += sp.x; w->py += sp.y; w->pz += sp.z;                       /* :91 */
 
@@ -515,11 +560,6 @@ static int or_fly(or_march* m, or_wind* w)
     add64(&m->T[2 * (int64_t)nx * ny + cell], or_fx(sp.y));
     add64(&m->T[3 * (int64_t)nx * ny + cell], or_fx(sp.z));
   }
-
-  float force = -dot3(norm3(sp), n) * len3(sp);                      /* :100-102 */
-  if (force < 0.0f) force = 0.0f;
-  float lift = (1.0f - collision) * len3(sp);                        /* :104 */
-  float capacity = force * hfac + 0.02f * lift * hfac;              /* :106 */
 
   float diff = capacity - w->sed;                                    /* :110-112 */
   float q;
@@ -646,7 +686,7 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
 #endif
       for (int64_t i = 0; i < particles; i++) {
         or_wind* w = &W[i];
-        for (int32_t s = 0; s < R && w->age >= 0; s++) nsteps += or_fly(&m, w);
+        for (int32_t s = 0; s < R && w->age >= 0; s++) nsteps += or_fly(&m, w, NULL);
       }
       or_relax(p, nx, ny, S, SS, V, D, DS);
       for (int64_t c = 0; c < ncell; c++) {
@@ -719,7 +759,7 @@ int32_t or_trajectory(const or_params* p, int32_t nx, int32_t ny, const float* S
   int32_t n = 0;
   for (int32_t i = 0; i < max_calls; i++) {
     float sed0 = w.sed;
-    if (!or_fly(&m, &w)) break;
+    if (!or_fly(&m, &w, NULL)) break;
     float* o = out + (int64_t)n * 8;
     o[0] = w.px; o[1] = w.py; o[2] = w.pz;
     o[3] = w.sx; o[4] = w.sy; o[5] = w.sz;
@@ -830,8 +870,9 @@ static void seq_cascade(or_seqmap* m, int32_t x, int32_t y)
 
 typedef struct { v3 pos, speed; float sediment; int32_t age; int64_t cell; } or_seqwind;
 
-/* Wind::fly (wind.hpp:28-117), sequential: glibc rand(), libm expf */
-static int seq_fly(or_seqmap* m, or_seqwind* w)
+/* Wind::fly (wind.hpp:28-117), sequential: glibc rand(), libm expf; the
+   arithmetic of :56-106 is fly_dynamics, shared with the batched method */
+static int seq_fly(or_seqmap* m, or_seqwind* w, float* tr)
 {
   const or_params* p = m->p;
   float rx = roundf(w->pos.x), rz = roundf(w->pos.z);
@@ -845,33 +886,21 @@ static int seq_fly(or_seqmap* m, or_seqwind* w)
   if (w->age++ > p->max_age) return 0;
   float hfac = expf(-(w->pos.y - cell->height) / p->boundarylayer);
   if (hfac < 0.0f) hfac = 0.0f;
-  float shadow = dot3(m->nps, n);
-  if (shadow < 0.0f) shadow = 0.0f;
-  shadow = 1.0f - shadow;
-  v3 sp = w->speed;
-  sp = vadd(sp, smul(0.05f, vsub(mulsv(m->ps, 0.1f + 0.9f * shadow), sp)));
-  if (w->pos.y > cell->height) sp.y -= p->gravity * w->sediment;
-  float collision = -dot3(norm3(sp), n);
-  if (collision < 0.0f) collision = 0.0f;
-  v3 rspeed = cross3(n, cross3(smul(1.0f - collision, sp), n));
-  sp = vadd(sp, smul(0.9f, vsub(smul(shadow, mix3(m->ps, rspeed, shadow * hfac)), sp)));
-  {
-    /* vec3(rand()%1001, rand()%1001, rand()%1001): drawn left to right here */
-    int32_t r0 = rand() % 1001, r1 = rand() % 1001, r2 = rand() % 1001;
-    float k = 0.1f * hfac * collision;
-    v3 t = vsubs(V3((float)r0, (float)r1, (float)r2), 500.0f);
-    sp = vadd(sp, vdivs(smul(k, t), 500.0f));
-  }
-  sp = mulsv(sp, (float)(1.0 - 0.3 * (double)w->sediment));
+  if (tr) { tr[0] = n.x; tr[1] = n.y; tr[2] = n.z; tr[3] = cell->height; tr[4] = hfac; }
+  /* vec3(rand()%1001, rand()%1001, rand()%1001) (:83): drawn left to right
+     here; nothing else draws between :45 and :83 */
+  int32_t r[3];
+  r[0] = rand() % 1001;
+  r[1] = rand() % 1001;
+  r[2] = rand() % 1001;
+  float capacity;
+  v3 sp = fly_dynamics(m->ps, m->nps, p->gravity, n, w->pos.y > cell->height, hfac, w->speed, w->sediment, r,
+                       &capacity, tr);
   w->pos = vadd(w->pos, sp);
   cell->momentumx_track += sp.x;
   cell->momentumy_track += sp.y;
   cell->momentumz_track += sp.z;
   cell->massflow_track += w->sediment;
-  float force = -dot3(norm3(sp), n) * len3(sp);
-  if (force < 0.0f) force = 0.0f;
-  float lift = (1.0f - collision) * len3(sp);
-  float capacity = force * hfac + 0.02f * lift * hfac;
   float diff = capacity - w->sediment;
   int64_t ci = cell - m->cells;
   if (p->layers == 2 && diff > 0.0f && m->sed[ci] <= 0.0f) {
@@ -938,7 +967,7 @@ int64_t or_erode_seq(const or_params* p, int32_t nx, int32_t ny, float* height, 
       w.sediment = 0.0f;
       w.age = 0;
       w.cell = -1;
-      while (seq_fly(&m, &w)) steps++;
+      while (seq_fly(&m, &w, NULL)) steps++;
       if (p->deposit_on_death && w.cell >= 0) {
         /* the option the batched method offers: the load stays on the map */
         cells[w.cell].height += w.sediment;
@@ -967,3 +996,96 @@ int64_t or_erode_seq(const or_params* p, int32_t nx, int32_t ny, float* height, 
 }
 
 int32_t or_params_size(void) { return (int32_t)sizeof(or_params); }
+
+/* ------------------------------------------------------------------------ */
+/* One-step exports for known-answer tests                                  */
+/* ------------------------------------------------------------------------ */
+/*
+ * One batched fly() call of a particle in an arbitrary state against a
+ * frozen snapshot.
+ *   st    float[7] in/out: pos x, y, z, speed x, y, z, sediment
+ *   age   in/out (0 = a fresh particle: pos.y snaps to the ground)
+ *   key   the particle's RNG key (or_particle_key); draws are or_turb(key, age)
+ *   tr    float[OR_TRACE_LEN] stage trace or NULL
+ *   pick  out: the fixed-point pickup (2^-32 units) taken from the cell
+ * Returns 1 for a particle-step, 0 when the particle dies.
+ */
+int32_t or_fly_once(const or_params* p, int32_t nx, int32_t ny, const float* S, const float* SS,
+                    float* st, int32_t* age, uint64_t key, float* tr, int64_t* pick)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  int64_t* D = (int64_t*)calloc(ncell, sizeof(int64_t));
+  int64_t* DS = (int64_t*)calloc(ncell, sizeof(int64_t));
+  uint32_t* V = (uint32_t*)calloc(ncell, sizeof(uint32_t));
+  int64_t* T = (int64_t*)calloc(4 * ncell, sizeof(int64_t));
+  int64_t lost = 0;
+  if (!D || !DS || !V || !T) { free(D); free(DS); free(V); free(T); return -1; }
+  or_march m;
+  m.p = p; m.nx = nx; m.ny = ny; m.S = S; m.SS = SS; m.D = D; m.DS = DS; m.V = V; m.T = T;
+  m.lost = &lost;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);
+  or_wind w;
+  w.key = key; w.px = st[0]; w.py = st[1]; w.pz = st[2];
+  w.sx = st[3]; w.sy = st[4]; w.sz = st[5]; w.sed = st[6];
+  w.age = *age; w.ledger = 0; w.last_cell = 0;
+  int32_t ok = or_fly(&m, &w, tr);
+  st[0] = w.px; st[1] = w.py; st[2] = w.pz;
+  st[3] = w.sx; st[4] = w.sy; st[5] = w.sz; st[6] = w.sed;
+  *age = w.age;
+  if (pick) *pick = w.ledger;
+  free(D); free(DS); free(V); free(T);
+  return ok;
+}
+
+/*
+ * One sequential fly() call (the reference's algorithm: glibc rand(), libm
+ * expf, pickup and World::cascade written into the float map in place).
+ *   height, sed  float[nx*ny] in/out (sed: layers == 2, else NULL)
+ *   st, age, tr  as or_fly_once
+ * Call or_srand first to fix the three draws.
+ */
+int32_t or_fly_seq_once(const or_params* p, int32_t nx, int32_t ny, float* height, float* sed,
+                        float* st, int32_t* age, float* tr)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  or_cell* cells = (or_cell*)calloc(ncell, sizeof(or_cell));
+  if (!cells) return -1;
+  for (int64_t c = 0; c < ncell; c++) cells[c].height = height[c];
+  or_seqmap m;
+  m.p = p; m.nx = nx; m.ny = ny; m.cells = cells; m.sed = sed;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);
+  or_seqwind w;
+  w.pos = V3(st[0], st[1], st[2]);
+  w.speed = V3(st[3], st[4], st[5]);
+  w.sediment = st[6];
+  w.age = *age;
+  w.cell = -1;
+  int32_t ok = seq_fly(&m, &w, tr);
+  st[0] = w.pos.x; st[1] = w.pos.y; st[2] = w.pos.z;
+  st[3] = w.speed.x; st[4] = w.speed.y; st[5] = w.speed.z; st[6] = w.sediment;
+  *age = w.age;
+  for (int64_t c = 0; c < ncell; c++) height[c] = cells[c].height;
+  free(cells);
+  return ok;
+}
+
+/* One sequential World::cascade(x, y) (world.hpp:79-153) on a float map in
+   place (sed: layers == 2, else NULL). */
+void or_cascade_seq(const or_params* p, int32_t nx, int32_t ny, float* height, float* sed, int32_t x, int32_t y)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  or_cell* cells = (or_cell*)calloc(ncell, sizeof(or_cell));
+  if (!cells) return;
+  for (int64_t c = 0; c < ncell; c++) cells[c].height = height[c];
+  or_seqmap m;
+  m.p = p; m.nx = nx; m.ny = ny; m.cells = cells; m.sed = sed;
+  m.ps = V3(p->pspeed[0], p->pspeed[1], p->pspeed[2]);
+  m.nps = norm3(m.ps);
+  seq_cascade(&m, x, y);
+  for (int64_t c = 0; c < ncell; c++) height[c] = cells[c].height;
+  free(cells);
+}
+
+int32_t or_trace_len(void) { return OR_TRACE_LEN; }
