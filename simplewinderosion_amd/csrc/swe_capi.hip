@@ -40,6 +40,10 @@ struct swe_ctx {
   // the launch configuration it was captured for
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::string gkey[2];
+  // bumped by every device allocation of the context: a captured graph holds
+  // buffer addresses, and a freed buffer's address can come back from
+  // hipMalloc for another one, so the generation is part of the graph key
+  uint64_t alloc_gen = 0;
 
   // map (device); index [g & 1] is current, g counts snapshot windows
   uint64_t g = 0;
@@ -126,6 +130,7 @@ template <typename T>
 int dalloc(swe_ctx* c, T** ptr, int64_t count) {
   *ptr = nullptr;
   if (count <= 0) return 0;
+  c->alloc_gen++;
   hipError_t e = hipMalloc((void**)ptr, sizeof(T) * (size_t)count);
   if (e != hipSuccess) {
     *ptr = nullptr;
@@ -541,6 +546,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     key.append((const char*)&c->tile_order, sizeof(c->tile_order));
     key.append((const char*)&c->heavy_first, sizeof(c->heavy_first));
     key.append((const char*)&nwin, sizeof(nwin));
+    key.append((const char*)&c->alloc_gen, sizeof(c->alloc_gen));
   }
   const uint64_t windows_per_cycle = total > 0 ? (uint64_t)nwin + 1 : 0;
 

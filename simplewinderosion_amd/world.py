@@ -123,14 +123,24 @@ class World:
         s = None if sediment is None else np.ascontiguousarray(sediment, dtype=np.float32).reshape(self.nx * self.ny)
         self._check(self.lib.swe_upload_layers(self.ctx, _fp(h), _fp(s)), "swe_upload_layers")
 
+    def _flat(self, a, dtype, count, what):
+        """a as a contiguous 1-D array of exactly `count` elements: the C side
+        copies that many from the pointer."""
+        a = np.ascontiguousarray(a, dtype=dtype)
+        if a.size != count:
+            raise ValueError("%s: %d values for a %dx%d map, expected %d" % (what, a.size, self.nx, self.ny, count))
+        return a.reshape(count)
+
     def upload_master(self, height_fx, sediment_fx=None):
-        h = np.ascontiguousarray(height_fx, dtype=np.int64).reshape(self.nx * self.ny)
-        s = None if sediment_fx is None else np.ascontiguousarray(sediment_fx, dtype=np.int64).reshape(-1)
+        n = self.nx * self.ny
+        h = self._flat(height_fx, np.int64, n, "height_fx")
+        s = None if sediment_fx is None else self._flat(sediment_fx, np.int64, n, "sediment_fx")
         self._check(self.lib.swe_upload_master(self.ctx, _ip(h), _ip(s)), "swe_upload_master")
 
     def upload_fields(self, massflow=None, momentum=None):
-        m = None if massflow is None else np.ascontiguousarray(massflow, dtype=np.float32).reshape(-1)
-        mo = None if momentum is None else np.ascontiguousarray(momentum, dtype=np.float32).reshape(-1)
+        n = self.nx * self.ny
+        m = None if massflow is None else self._flat(massflow, np.float32, n, "massflow")
+        mo = None if momentum is None else self._flat(momentum, np.float32, 3 * n, "momentum")
         self._check(self.lib.swe_upload_fields(self.ctx, _fp(m), _fp(mo)), "swe_upload_fields")
 
     def download(self):

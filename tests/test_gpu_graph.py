@@ -45,3 +45,36 @@ def test_graph_replays_equal_eager_launches(statics):
     assert g.stats()["particle_steps"] == e.stats()["particle_steps"] == st.steps.value
     g.close()
     e.close()
+
+
+def test_graph_after_buffers_grow_and_shrink(statics):
+    """A larger count reallocates the particle buffers; a later smaller count
+    must not replay a graph captured against the freed ones (the allocation
+    generation is part of the graph key).  Every cycle is checked against the
+    oracle."""
+    print("case graph grow/shrink", flush=True)
+    Wind.suspension = 1e-3
+    World.settling = 0.02
+    World.SEED = 5
+    rng = np.random.default_rng(8)
+    h = (rng.random((96, 160)) * 0.5).astype(np.float32)
+    s = np.full((96, 160), 0.01, np.float32)
+    g = make(1, h, s)
+    st = O.OracleState(96, 160, h, sediment=s, params=O.params_from(g.params()))
+    for n in (2000, 2000, 7000, 2000, 2000, 9000, 2000):
+        g.erode(n)
+        st.erode(n, 1)
+        zg, sg = g.master()
+        assert np.array_equal(zg.ravel(), st.zm) and np.array_equal(sg.ravel(), st.sm), n
+    assert g.stats()["particle_steps"] == st.steps.value
+    g.close()
+
+
+def test_upload_size_is_checked():
+    w = World(16, 24, periodic=True)
+    with pytest.raises(ValueError):
+        w.upload_master(np.zeros(16 * 24 - 1, np.int64))
+    with pytest.raises(ValueError):
+        w.upload_fields(momentum=np.zeros(16 * 24, np.float32))
+    w.upload_fields(massflow=np.zeros((16, 24), np.float32), momentum=np.zeros((3, 16, 24), np.float32))
+    w.close()

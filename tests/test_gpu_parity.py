@@ -293,7 +293,8 @@ def test_normals_match_oracle(statics):
 
 def test_full_size_one_cycle(statics):
     """1024x1024 two-layer map with pyramids, 1,000,000 particles, one cycle,
-    16-call snapshot windows: bit-exact against the oracle on all host cores."""
+    one-call snapshot windows (the benchmark workload): bit-exact against the
+    oracle on all host cores."""
     from simplewinderosion_amd import workloads
 
     say("case gpu1024 1 cycle")
@@ -334,7 +335,27 @@ def test_checkpoint_resume_is_bit_exact(statics, tmp_path):
     for k in ("massflow", "momentum", "height", "sediment"):
         assert np.array_equal(ref[1][k], got[1][k]), k
     assert ref[2] == got[2]
+    # and the resumed run is the oracle's four cycles
+    st = O.OracleState(40, 72, h, sediment=sed, params=O.params_from(w2.params()))
+    st.erode(900, 4)
+    assert_same(w2, st)
     w2.close()
+
+
+@pytest.mark.parametrize("periodic,nx,ny", [(True, 96, 128), (False, 112, 100), (True, 40, 200)])
+def test_far_particle_routing(statics, periodic, nx, ny):
+    """A fast prevailing wind moves particles more than one tile (16 x 32
+    cells) per call: they travel through the far list that every workgroup
+    scans.  Results stay the oracle's bits, and the device counts far moves."""
+    say("case far", periodic, nx, ny)
+    Wind.suspension = 1e-3
+    Wind.pspeed = (40.0, 0.0, 35.0)
+    w, st = run_pair(nx, ny, 3000, 2, seed=13, periodic=periodic, window=1, settling=0.05, path=2)
+    assert_same(w, st)
+    s = w.stats()
+    assert s["far_particles"] > 0
+    assert s["particle_steps"] == st.steps.value
+    w.close()
 
 
 @pytest.mark.parametrize("window", [1, 3])
