@@ -270,6 +270,17 @@ int swe_mesh(swe_ctx* ctx, float* out);
 int swe_normals(swe_ctx* ctx, float* out);
 
 /*
+ * swe_normals without any device allocation: the same kernel writes into a
+ * staging array the library holds in device memory for the life of the
+ * process (one per device, 12 MiB: maps of at most 1,048,576 cells; larger
+ * maps return SWE_EINVAL).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (per-frame render loops,
+ * serving loops, code that forbids hipMalloc).  Calls that overlap on one
+ * device race on the staging array.
+ */
+int swe_normals_noalloc(swe_ctx* ctx, float* out);
+
+/*
  * Terrain generator for the benchmark inputs: seeded fBm gradient noise
  * (hashed lattice gradients, quintic fade, `octaves` octaves with lacunarity
  * 2 and gain 0.5, base frequency `base_cycles` periods across the map,

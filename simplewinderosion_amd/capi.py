@@ -143,6 +143,7 @@ SIGNATURES = {
     "swe_stats": (ctypes.c_int, [_P, ctypes.POINTER(SweStats)]),
     "swe_reset_stats": (ctypes.c_int, [_P]),
     "swe_normals": (ctypes.c_int, [_P, _FP]),
+    "swe_normals_noalloc": (ctypes.c_int, [_P, _FP]),
     "swe_windmap": (ctypes.c_int, [_P, _FP]),
     "swe_mesh": (ctypes.c_int, [_P, _FP]),
     "swe_generate_terrain": (

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <array>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,37 @@
 
 namespace {
 
+// Staging of swe_normals_noalloc: three float planes of at most
+// kNoAllocCells cells.  Every device the library runs on holds its own copy
+// of this array until the process ends.  Host code never passes the array's
+// name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress returns for the current device (scratch_of below).
+constexpr int64_t kNoAllocCells = (int64_t)1 << 20;
+__device__ __align__(256) float scratch[3 * kNoAllocCells];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves this device's copy of `scratch` once per device, with `device`
+// current.  Called when a context is created, so the no-allocation call
+// itself only reads the table.
+hipError_t resolve_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = (float*)ptr;
+  return hipSuccess;
+}
+
+float* scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 }  // namespace
 
@@ -709,7 +741,8 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
   c->tps_set.assign((size_t)n_terrains, 0);
   swe_params_default(&c->p);
   int r = 0;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      resolve_scratch(device) != hipSuccess) {
     swe_destroy(c);
     return SWE_EHIP;
   }
@@ -1259,6 +1292,28 @@ int swe_normals(swe_ctx* c, float* out) {
   if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 3 * c->ncell, hipMemcpyDeviceToHost, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("normals: ") + hipGetErrorString(e));
+  if (e2 != hipSuccess) return fail(c, SWE_EHIP, std::string("normals sync: ") + hipGetErrorString(e2));
+  return 0;
+}
+
+int swe_normals_noalloc(swe_ctx* c, float* out) {
+  if (!c || !out) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (c->ncell > kNoAllocCells)
+    return fail(c, SWE_EINVAL,
+                "swe_normals_noalloc: " + std::to_string(c->ncell) + " cells exceed its staging of " +
+                    std::to_string(kNoAllocCells) + " cells; use swe_normals");
+  float* stage = scratch_of(c->device);
+  if (!stage) return fail(c, SWE_ESTATE, "swe_normals_noalloc: staging not resolved on this device");
+  HIPCHK(c, hipSetDevice(c->device));
+  // every plane element is written by the kernel before the copy reads it:
+  // the staging needs no clearing, and a concurrent call on the same device
+  // would race on it
+  swe::launch_normals(c->zs[c->cur()], stage, c->nx, c->ny, c->p.mapscale, c->p.periodic != 0, c->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, stage, sizeof(float) * 3 * c->ncell, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e2 = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("normals: ") + hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(c, SWE_EHIP, std::string("normals sync: ") + hipGetErrorString(e2));
   return 0;
