@@ -12,14 +12,29 @@
  *
  * Semantics (batched-synchronous): within one snapshot window every particle
  * makes `window` fly() calls reading the same float32 snapshot of the map;
- * pickups are summed as int64 fixed point with 2^-32 resolution; cascade
- * events are counted per cell, and at the end of the window each cell's
- * events run one after another on a private copy of its 3x3 neighbourhood
- * taken from the snapshot, their net transfers summed in the same fixed
+ * pickups are summed as int64 fixed point with 2^-32 resolution.  Each call
+ * counts one cascade event at its pre-move cell; at the end of the window
+ * every cell with V > 0 events runs World::cascade once against the
+ * snapshot (8 neighbours in the reference's offset order, stable sort by
+ * height, walk with the centre evolving) with the effective settling factor
+ * 1 - (1 - settling)^V -- what V consecutive events leave of an isolated
+ * pair's excess -- and the net transfers are summed in the same fixed
  * point.  The default window of 1 makes the method step-synchronous: every
  * particle takes one step against the same map, then the map is updated.
  * Results are deterministic for given (params, seed, cycle counter) and
  * independent of thread and launch order.
+ *
+ * Stability: where the neighbourhoods of cells visited in one window
+ * overlap, shared edges move once per cascade that touches them, so large
+ * settling factors overshoot.  settling <= 0.1 is stable on every workload
+ * measured (the benchmark workloads use 0.01); swe_set_params rejects
+ * settling > 0.25 with the synchronous cascade (cascade_order 0); the
+ * colour-ordered cascade (cascade_order 1, or -1 above 0.25) runs the
+ * reference's own settling = 1.0.  Independently of the cascade, pickups of
+ * many particles in one cell in one window add up: with the reference's
+ * suspension 0.05 the method stays bounded at the reference's density
+ * (4096 particles on 512^2) but not at one particle per cell, where the
+ * benchmark workloads use suspension 1e-4.
  *
  * Map layout: cell (x, y) lives at index x*ny + y (math::flatten,
  * source/include/math.h:11-14): x is the slow axis, y the fast axis.
@@ -75,6 +90,13 @@ typedef struct swe_params {
   int32_t spawn;          /* 0 = uniform over the map (world.hpp:52-54), 1 = on the x = 0 / y = 0
                              edges (the commented variant, world.hpp:56-61) */
   uint64_t seed;          /* World::SEED: feeds the counter RNG */
+  int32_t cascade_order;  /* -1 = automatic (default): colour-ordered when settling > 0.25, else
+                             synchronous; 0 = synchronous (every visited cell's cascade reads the
+                             window's snapshot; settling <= 0.25 only); 1 = colour-ordered (cells in
+                             3x3 colour classes, one class after another, each reading the map the
+                             earlier classes left: the reference's sequential cascades in class
+                             order; global path, single maps) */
+  int32_t reserved;       /* 0 */
 } swe_params;
 
 /* Mass audit: exact in int64 fixed point (2^-32). */
@@ -216,6 +238,20 @@ int swe_download_terrain_master(swe_ctx* ctx, int32_t k, int64_t* height_fx, int
 int swe_audit_terrain(swe_ctx* ctx, int32_t k, struct swe_audit_t* out);
 /* One map-cycle per n_cycles for every terrain; particles[k] per terrain. */
 int swe_erode_batch(swe_ctx* ctx, const int64_t* particles, int32_t n_cycles);
+/*
+ * Terrain k's size, effective seed and prevailing wind (the context's when
+ * not overridden); overrides: bit 0 = seed set by swe_set_terrain, bit 1 =
+ * wind set.  Any output pointer may be NULL.
+ */
+int swe_get_terrain(const swe_ctx* ctx, int32_t k, int32_t* nx, int32_t* ny, uint64_t* seed, float* pspeed,
+                    int32_t* overrides);
+/* Batch counterparts of swe_upload_master, swe_upload_fields and swe_set_lost. */
+int swe_upload_terrain_master(swe_ctx* ctx, int32_t k, const int64_t* height_fx, const int64_t* sediment_fx);
+int swe_upload_terrain_fields(swe_ctx* ctx, int32_t k, const float* massflow, const float* momentum);
+int swe_set_terrain_lost(swe_ctx* ctx, int32_t k, int64_t lost);
+/* Batch counterparts of swe_windmap and swe_normals (same layouts, terrain k's cells). */
+int swe_windmap_terrain(swe_ctx* ctx, int32_t k, float* out);
+int swe_normals_terrain(swe_ctx* ctx, int32_t k, float* out);
 
 /* Exact mass audit of the current state (synchronizes). */
 int swe_audit(swe_ctx* ctx, swe_audit_t* out);
@@ -229,9 +265,11 @@ int swe_set_lost(swe_ctx* ctx, int64_t lost);
 int swe_get_size(const swe_ctx* ctx, int32_t* nx, int32_t* ny);
 
 /*
- * Checkpoint / resume: the exact int64 master layers, EMA fields, params,
- * cycle counter and lost-mass counter in one little-endian file, so that
- * load + erode continues bit for bit.  No counterpart in the reference,
+ * Checkpoint / resume: the exact int64 master layers, EMA fields, params
+ * (as JSON text), cycle counter, and per terrain its seed, wind and
+ * lost-mass counter, in one little-endian file, so that load + erode
+ * continues bit for bit; single maps and batches (a checkpoint loads into a
+ * context of the same terrain sizes).  No counterpart in the reference,
  * whose only state is its seed (SimpleWindErosion.cpp:27-34).
  */
 int swe_save_state(swe_ctx* ctx, const char* path);
@@ -253,6 +291,19 @@ int swe_reset_stats(swe_ctx* ctx);
  * EMA momentum field.  out: 4*nx*ny floats, RGBA interleaved per cell.
  */
 int swe_windmap(swe_ctx* ctx, float* out);
+
+/*
+ * Flow-field and map products written to a file (the reference draws its
+ * wind map into a texture each frame, SimpleWindErosion.cpp:224-230, and
+ * writes no files).  SWE_EXPORT_NPZ: a numpy archive with height, sediment,
+ * massflow (nx, ny), momentum and normals (3, nx, ny), windmap (nx, ny, 4),
+ * all float32, and the exact masters height_fx, sediment_fx (nx, ny) int64.
+ * SWE_EXPORT_PNG: the wind map as 8-bit RGBA, nx wide and ny high, pixel
+ * (x, y) = cell (x, y), channels round(255 v).
+ */
+enum swe_export_format { SWE_EXPORT_NPZ = 0, SWE_EXPORT_PNG = 1 };
+int swe_export(swe_ctx* ctx, const char* path, int32_t format);
+int swe_export_terrain(swe_ctx* ctx, int32_t k, const char* path, int32_t format);
 
 /*
  * Render-mesh vertex attributes as updatenode writes them (cellpool.h:302-333):

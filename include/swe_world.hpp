@@ -40,6 +40,9 @@ class World {
   inline static float mapscale = 80.0f;
   inline static int periodic = 0, track = 1, layers = 1, window = 1, deposit_on_death = 0, spawn = 0;
   inline static float abrasion = 0.0f;
+  // -1 automatic: the colour-ordered cascade when settling > 0.25 (the
+  // reference's 1.0 included), else the synchronous one
+  inline static int cascade_order = -1;
   inline static int device = 0;
 
   // map.init(...) counterpart: create the device map and upload heights
@@ -113,6 +116,7 @@ class World {
     p.deposit_on_death = deposit_on_death;
     p.spawn = spawn;
     p.seed = SEED;
+    p.cascade_order = cascade_order;
     check(swe_set_params(ctx_, &p), "swe_set_params");
   }
 };

@@ -37,6 +37,8 @@ class OrParams(ctypes.Structure):
         ("deposit_on_death", ctypes.c_int32),
         ("spawn", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("cascade_order", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -60,6 +62,7 @@ def reference_params(**kw):
     p.deposit_on_death = 0
     p.spawn = 0
     p.seed = 0
+    p.cascade_order = -1
     for k, v in kw.items():
         if k == "pspeed":
             for i in range(3):
@@ -130,6 +133,12 @@ def load():
                                              ctypes.POINTER(ctypes.c_int32), _F]),
         "or_cascade_seq": (None, [_PP, ctypes.c_int32, ctypes.c_int32, _F, _F, ctypes.c_int32, ctypes.c_int32]),
         "or_trace_len": (ctypes.c_int32, []),
+        "or_relax_colour_only": (ctypes.c_int, [_PP, ctypes.c_int32, ctypes.c_int32, _I64, _I64,
+                                                ctypes.POINTER(ctypes.c_uint32)]),
+        "or_colour_classes": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
+        "or_colour_count": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+        "or_colour_cell": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+        "or_colour_mode": (ctypes.c_int32, [_PP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -276,6 +285,26 @@ def fly_seq_once(params, height, state, age, seed, sediment=None):
     if ok < 0:
         raise MemoryError("or_fly_seq_once")
     return bool(ok), st, a.value, _trace_dict(tr), H, Sd
+
+
+def colour_cells(n, periodic):
+    """Cells of each colour class along one axis, in class order."""
+    lib = load()
+    return [[lib.or_colour_cell(n, int(periodic), col, i) for i in range(lib.or_colour_count(n, int(periodic), col))]
+            for col in range(lib.or_colour_classes(n, int(periodic)))]
+
+
+def relax_colour(params, zm, V, sm=None):
+    """The colour-ordered relax of one window (no pickups) on fixed-point
+    masters; returns the new masters."""
+    z = np.ascontiguousarray(zm, np.int64).ravel().copy()
+    nx, ny = np.shape(zm)
+    s = None if sm is None else np.ascontiguousarray(sm, np.int64).ravel().copy()
+    v = np.ascontiguousarray(V, np.uint32).ravel().copy()
+    rc = load().or_relax_colour_only(ctypes.byref(params), nx, ny, _i(z), _i(s),
+                                     v.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    assert rc == 0
+    return z.reshape(nx, ny), (None if s is None else s.reshape(nx, ny))
 
 
 def cascade_seq(params, height, x, y, sediment=None):

@@ -60,7 +60,14 @@ typedef struct {
   int32_t deposit_on_death;
   int32_t spawn;          /* 0: uniform (world.hpp:52-54), 1: edges (world.hpp:56-61) */
   uint64_t seed;
+  int32_t cascade_order;  /* -1: automatic, 0: synchronous, 1: colour-ordered */
+  int32_t reserved;
 } or_params;
+
+/* Settling factors above this bound use the colour-ordered cascade when
+   cascade_order is automatic (-1); the library rejects them with the
+   synchronous one. */
+#define OR_SETTLING_MAX_SYNC 0.25f
 
 /* ------------------------------------------------------------------------ */
 /* Bit helpers                                                              */
@@ -609,6 +616,93 @@ static void or_relax(const or_params* p, int32_t nx, int32_t ny, const float* S,
   }
 }
 
+/* Colour classes of the colour-ordered cascade along one axis of n cells.
+   Bounded: x mod 3 (three classes).  Periodic, n = 3q + r: x mod 3 for
+   x < 3q, and each of the last r cells a class of its own (3 + r classes),
+   so that two cells of one class are at least three apart across the wrap
+   too, and their 3x3 neighbourhoods never overlap. */
+int32_t or_colour_classes(int32_t n, int32_t periodic)
+{
+  return periodic ? 3 + n % 3 : 3;
+}
+int32_t or_colour_count(int32_t n, int32_t periodic, int32_t col)
+{
+  const int32_t lim = periodic ? n - n % 3 : n;
+  if (col < 3) return lim > col ? (lim - col + 2) / 3 : 0;
+  return (periodic && col - 3 < n % 3) ? 1 : 0;
+}
+int32_t or_colour_cell(int32_t n, int32_t periodic, int32_t col, int32_t i)
+{
+  return col < 3 ? col + 3 * i : n - n % 3 + (col - 3);
+}
+
+/* cascade_order resolved for a window: 1 colour-ordered, 0 synchronous */
+int32_t or_colour_mode(const or_params* p)
+{
+  return p->cascade_order == 1 || (p->cascade_order < 0 && p->settling > OR_SETTLING_MAX_SYNC);
+}
+
+static void refresh(const or_params* p, int64_t c, const int64_t* zm, const int64_t* sm, float* S, float* SS)
+{
+  S[c] = or_unfx(zm[c]);
+  if (p->layers == 2) SS[c] = or_unfx(sm[c]);
+}
+
+/*
+ * The colour-ordered relax of one window: the window's pickups go into the
+ * masters first; then, one colour class after another (x class major, y
+ * class minor), every cell of the class with V > 0 events runs its cascade
+ * (or_cascade_events, settling factor 1 - (1 - settling)^V) against the
+ * current map -- the masters after the pickups and the earlier classes'
+ * transfers -- and its transfers go into the masters at once.  Cells of one
+ * class are at least three apart, so their cascades read and write disjoint
+ * 3x3 neighbourhoods: the class is the reference's sequential cascade calls
+ * (world.hpp:47-63,79-153) in any order.  The sediment layer is clamped at
+ * zero after the last class.
+ */
+static void or_relax_colour(const or_params* p, int32_t nx, int32_t ny, float* S, float* SS, uint32_t* V,
+                            int64_t* D, int64_t* DS, int64_t* zm, int64_t* sm)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  for (int64_t c = 0; c < ncell; c++) {
+    zm[c] += D[c];
+    D[c] = 0;
+    if (p->layers == 2) { sm[c] += DS[c]; DS[c] = 0; }
+    refresh(p, c, zm, sm, S, SS);
+  }
+  const int32_t ncx = or_colour_classes(nx, p->periodic), ncy = or_colour_classes(ny, p->periodic);
+  for (int32_t cx = 0; cx < ncx; cx++)
+    for (int32_t cy = 0; cy < ncy; cy++) {
+      const int32_t ni = or_colour_count(nx, p->periodic, cx), nj = or_colour_count(ny, p->periodic, cy);
+      for (int32_t i = 0; i < ni; i++)
+        for (int32_t j = 0; j < nj; j++) {
+          const int32_t x = or_colour_cell(nx, p->periodic, cx, i), y = or_colour_cell(ny, p->periodic, cy, j);
+          const int64_t c = (int64_t)x * ny + y;
+          if (!V[c]) continue;
+          int64_t nbc[8], nbf[8];
+          const int32_t cnt = or_cascade_events(S, SS, nx, ny, p, x, y, V[c], nbc, nbf);
+          int64_t centre = 0;
+          for (int32_t k = 0; k < cnt; k++) {
+            if (!nbf[k]) continue;
+            zm[nbc[k]] += nbf[k];
+            if (p->layers == 2) sm[nbc[k]] += nbf[k];
+            centre -= nbf[k];
+          }
+          zm[c] += centre;
+          if (p->layers == 2) sm[c] += centre;
+          for (int32_t k = 0; k < cnt; k++) refresh(p, nbc[k], zm, sm, S, SS);
+          refresh(p, c, zm, sm, S, SS);
+        }
+    }
+  for (int64_t c = 0; c < ncell; c++) {
+    V[c] = 0;
+    if (p->layers == 2) {
+      if (sm[c] < 0) sm[c] = 0;
+      SS[c] = or_unfx(sm[c]);
+    }
+  }
+}
+
 static int64_t or_windows(const or_params* p, int32_t* R)
 {
   int64_t calls = (int64_t)p->max_age + 2;           /* 513 steps + the dying call */
@@ -687,6 +781,10 @@ int or_erode(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* s
       for (int64_t i = 0; i < particles; i++) {
         or_wind* w = &W[i];
         for (int32_t s = 0; s < R && w->age >= 0; s++) nsteps += or_fly(&m, w, NULL);
+      }
+      if (or_colour_mode(p)) {
+        or_relax_colour(p, nx, ny, S, SS, V, D, DS, zm, sm);
+        continue;
       }
       or_relax(p, nx, ny, S, SS, V, D, DS);
       for (int64_t c = 0; c < ncell; c++) {
@@ -1086,6 +1184,24 @@ void or_cascade_seq(const or_params* p, int32_t nx, int32_t ny, float* height, f
   seq_cascade(&m, x, y);
   for (int64_t c = 0; c < ncell; c++) height[c] = cells[c].height;
   free(cells);
+}
+
+/*
+ * The colour-ordered relax of one window on its own (known-answer tests):
+ * masters zm (and sm when layers == 2) in/out, V events per cell; no
+ * pickups.  V is cleared.
+ */
+int or_relax_colour_only(const or_params* p, int32_t nx, int32_t ny, int64_t* zm, int64_t* sm, uint32_t* V)
+{
+  const int64_t ncell = (int64_t)nx * ny;
+  float* S = (float*)malloc(sizeof(float) * ncell);
+  float* SS = p->layers == 2 ? (float*)malloc(sizeof(float) * ncell) : NULL;
+  int64_t* D = (int64_t*)calloc(ncell, sizeof(int64_t));
+  int64_t* DS = p->layers == 2 ? (int64_t*)calloc(ncell, sizeof(int64_t)) : NULL;
+  int ok = S && D && (p->layers != 2 || (SS && DS && sm));
+  if (ok) or_relax_colour(p, nx, ny, S, SS, V, D, DS, zm, sm);
+  free(S); free(SS); free(D); free(DS);
+  return ok ? 0 : -1;
 }
 
 int32_t or_trace_len(void) { return OR_TRACE_LEN; }

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libswe.so")
 
-SOURCES = ["swe_kernels.hip", "swe_step.hip", "swe_capi.hip", "swe_terrain.cpp", "swe_state.cpp"]
+SOURCES = ["swe_kernels.hip", "swe_step.hip", "swe_capi.hip", "swe_terrain.cpp", "swe_state.cpp", "swe_export.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",

@@ -21,6 +21,9 @@ SWE_TUNE_TILE_ORDER = 1
 SWE_TUNE_GRAPH = 2
 SWE_TUNE_HEAVY_FIRST = 3
 
+SWE_EXPORT_NPZ = 0
+SWE_EXPORT_PNG = 1
+
 
 class SweParams(ctypes.Structure):
     """swe_params (include/swe.h)."""
@@ -43,6 +46,8 @@ class SweParams(ctypes.Structure):
         ("deposit_on_death", ctypes.c_int32),
         ("spawn", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("cascade_order", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -161,6 +166,16 @@ SIGNATURES = {
     "swe_download_terrain_master": (ctypes.c_int, [_P, ctypes.c_int32, _I64P, _I64P]),
     "swe_audit_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(SweAudit)]),
     "swe_erode_batch": (ctypes.c_int, [_P, _I64P, ctypes.c_int32]),
+    "swe_get_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint64), _FP,
+                                       ctypes.POINTER(ctypes.c_int32)]),
+    "swe_upload_terrain_master": (ctypes.c_int, [_P, ctypes.c_int32, _I64P, _I64P]),
+    "swe_upload_terrain_fields": (ctypes.c_int, [_P, ctypes.c_int32, _FP, _FP]),
+    "swe_set_terrain_lost": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int64]),
+    "swe_windmap_terrain": (ctypes.c_int, [_P, ctypes.c_int32, _FP]),
+    "swe_normals_terrain": (ctypes.c_int, [_P, ctypes.c_int32, _FP]),
+    "swe_export": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
+    "swe_export_terrain": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]),
 }
 
 _lib = None

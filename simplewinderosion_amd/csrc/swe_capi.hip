@@ -197,7 +197,19 @@ int check_params(swe_ctx* c, const swe_params* p) {
   if (p->track != 0 && p->track != 1) return fail(c, SWE_EINVAL, "track must be 0 or 1");
   if (p->deposit_on_death != 0 && p->deposit_on_death != 1)
     return fail(c, SWE_EINVAL, "deposit_on_death must be 0 or 1");
+  if (p->cascade_order < -1 || p->cascade_order > 1)
+    return fail(c, SWE_EINVAL, "cascade_order must be -1 (automatic), 0 (synchronous) or 1 (colour-ordered)");
+  if (p->cascade_order == 0 && p->settling > swe::kSettlingMaxSync)
+    return fail(c, SWE_EINVAL,
+                "settling above 0.25 overshoots with the synchronous cascade: use cascade_order 1 (colour-ordered) "
+                "or -1 (automatic)");
   return 0;
+}
+
+// the colour-ordered cascade in effect (cascade_order 1, or automatic above
+// the synchronous cascade's settling bound)
+bool colour_mode(const swe_params& p) {
+  return p.cascade_order == 1 || (p.cascade_order < 0 && p.settling > swe::kSettlingMaxSync);
 }
 
 int ensure_layers(swe_ctx* c) {
@@ -396,7 +408,8 @@ int erode_global(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin, 
       ra.sm_out = layers2 ? c->smv[cur ^ 1] : nullptr;
       ra.ss_out = layers2 ? c->ss[cur ^ 1] : nullptr;
       if ((r = timed_begin(c, &e0, &e1))) return r;
-      swe::launch_relax(ra, c->p.periodic != 0, c->p.layers, c->stream);
+      if (colour_mode(c->p)) swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, c->stream);
+      else swe::launch_relax(ra, c->p.periodic != 0, c->p.layers, c->stream);
       HIPCHK(c, hipGetLastError());
       if ((r = timed_end(c, e0, e1, 1))) return r;
       c->g++;
@@ -645,6 +658,8 @@ void swe_params_default(swe_params* p) {
   p->deposit_on_death = 0;
   p->spawn = 0;
   p->seed = 0;
+  p->cascade_order = -1;
+  p->reserved = 0;
 }
 
 const char* swe_version(void) { return "swe-mi355x 0.1 (gfx950)"; }
@@ -789,6 +804,19 @@ int swe_get_terrains(const swe_ctx* c, int32_t* n_terrains) {
   return 0;
 }
 
+int swe_get_terrain(const swe_ctx* c, int32_t k, int32_t* nx, int32_t* ny, uint64_t* seed, float* pspeed,
+                    int32_t* overrides) {
+  if (!c || k < 0 || k >= c->nterr) return SWE_EINVAL;
+  if (nx) *nx = c->tnx[(size_t)k];
+  if (ny) *ny = c->tny[(size_t)k];
+  const bool ss = c->tseed_set[(size_t)k] != 0, ps = c->tps_set[(size_t)k] != 0;
+  if (seed) *seed = ss ? c->tseed[(size_t)k] : c->p.seed;
+  if (pspeed)
+    for (int q = 0; q < 3; q++) pspeed[q] = ps ? c->tps[(size_t)k][q] : c->p.pspeed[q];
+  if (overrides) *overrides = (ss ? 1 : 0) | (ps ? 2 : 0);
+  return 0;
+}
+
 int swe_set_params(swe_ctx* c, const swe_params* p) {
   if (!c || !p) return SWE_EINVAL;
   int r = check_params(c, p);
@@ -850,34 +878,50 @@ int swe_upload_layers(swe_ctx* c, const float* height, const float* sediment) {
   return swe_upload_terrain(c, 0, height, sediment);
 }
 
-int swe_upload_master(swe_ctx* c, const int64_t* height_fx, const int64_t* sediment_fx) {
+int swe_upload_terrain_master(swe_ctx* c, int32_t k, const int64_t* height_fx, const int64_t* sediment_fx) {
   if (!c) return SWE_EINVAL;
-  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  if (sediment_fx && !c->smv[0]) return fail(c, SWE_ESTATE, "no sediment layer: set layers = 2 first");
   if (height_fx) {
-    HIPCHK(c, hipMemcpyAsync(c->zmv[c->cur()], height_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
-    swe::launch_snap(c->zmv[c->cur()], c->zs[c->cur()], c->ncell, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->zmv[c->cur()] + c0, height_fx, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+    swe::launch_snap(c->zmv[c->cur()] + c0, c->zs[c->cur()] + c0, n, c->stream);
     HIPCHK(c, hipGetLastError());
   }
   if (sediment_fx) {
-    if (!c->smv[0]) return fail(c, SWE_ESTATE, "no sediment layer: set layers = 2 first");
-    HIPCHK(c, hipMemcpyAsync(c->smv[c->cur()], sediment_fx, sizeof(int64_t) * c->ncell, hipMemcpyHostToDevice, c->stream));
-    swe::launch_snap(c->smv[c->cur()], c->ss[c->cur()], c->ncell, c->stream);
+    HIPCHK(c, hipMemcpyAsync(c->smv[c->cur()] + c0, sediment_fx, sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+    swe::launch_snap(c->smv[c->cur()] + c0, c->ss[c->cur()] + c0, n, c->stream);
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
-int swe_upload_fields(swe_ctx* c, const float* massflow, const float* momentum) {
+int swe_upload_master(swe_ctx* c, const int64_t* height_fx, const int64_t* sediment_fx) {
   if (!c) return SWE_EINVAL;
-  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context uploads with swe_upload_terrain_master");
+  return swe_upload_terrain_master(c, 0, height_fx, sediment_fx);
+}
+
+int swe_upload_terrain_fields(swe_ctx* c, int32_t k, const float* massflow, const float* momentum) {
+  if (!c) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
-  if (massflow) HIPCHK(c, hipMemcpyAsync(c->ema, massflow, sizeof(float) * c->ncell, hipMemcpyHostToDevice, c->stream));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
+  if (massflow) HIPCHK(c, hipMemcpyAsync(c->ema + c0, massflow, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
   if (momentum)
-    HIPCHK(c, hipMemcpyAsync(c->ema + c->ncell, momentum, sizeof(float) * 3 * c->ncell, hipMemcpyHostToDevice, c->stream));
+    for (int q = 0; q < 3; q++)
+      HIPCHK(c, hipMemcpyAsync(c->ema + (q + 1) * c->ncell + c0, momentum + q * n, sizeof(float) * n,
+                               hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int swe_upload_fields(swe_ctx* c, const float* massflow, const float* momentum) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context uploads with swe_upload_terrain_fields");
+  return swe_upload_terrain_fields(c, 0, massflow, momentum);
 }
 
 int swe_download_terrain(swe_ctx* c, int32_t k, float* height, float* sediment, float* massflow, float* momentum) {
@@ -945,7 +989,10 @@ int swe_erode_batch(swe_ctx* c, const int64_t* particles, int32_t n_cycles) {
   if ((r = ensure_layers(c))) return r;
   int32_t R;
   const int64_t nwin = windows_of(c->p, &R);
-  const bool tiled = c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1;
+  const bool colour = colour_mode(c->p);
+  if (colour && (c->path == 2 || c->nterr > 1))
+    return fail(c, SWE_EINVAL, "the colour-ordered cascade runs on the global path of single-map contexts");
+  const bool tiled = !colour && (c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1);
   if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path (and batches) need window == 1");
   if (tiled) return erode_tiled(c, particles, n_cycles, nwin);
   return erode_global(c, particles[0], n_cycles, nwin, R);
@@ -1194,15 +1241,22 @@ int swe_set_cycle(swe_ctx* c, uint64_t cycle) {
   return 0;
 }
 
-int swe_set_lost(swe_ctx* c, int64_t lost) {
+int swe_set_terrain_lost(swe_ctx* c, int32_t k, int64_t lost) {
   if (!c) return SWE_EINVAL;
-  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
   int64_t slots[swe::kLostSlots] = {0};
   slots[0] = lost;
-  HIPCHK(c, hipMemcpyAsync(c->lost, slots, sizeof(slots), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->lost + (int64_t)k * swe::kLostSlots, slots, sizeof(slots), hipMemcpyHostToDevice,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int swe_set_lost(swe_ctx* c, int64_t lost) {
+  if (!c) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context sets it with swe_set_terrain_lost");
+  return swe_set_terrain_lost(c, 0, lost);
 }
 
 int swe_set_timing(swe_ctx* c, int32_t on) {
@@ -1265,15 +1319,17 @@ int swe_mesh(swe_ctx* c, float* out) {
   return 0;
 }
 
-int swe_windmap(swe_ctx* c, float* out) {
+int swe_windmap_terrain(swe_ctx* c, int32_t k, float* out) {
   if (!c || !out) return SWE_EINVAL;
-  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)c->tnx[(size_t)k] * c->tny[(size_t)k];
   float* tmp = nullptr;
-  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 4 * c->ncell));
-  swe::launch_windmap(c->ema + c->ncell, tmp, c->ncell, c->stream);
+  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 4 * n));
+  // the momentum planes of terrain k: x at ema[ncell + c0], y and z a plane stride further
+  swe::launch_windmap(c->ema + c->ncell + c0, c->ncell, tmp, n, c->stream);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 4 * c->ncell, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("windmap: ") + hipGetErrorString(e));
@@ -1281,20 +1337,34 @@ int swe_windmap(swe_ctx* c, float* out) {
   return 0;
 }
 
-int swe_normals(swe_ctx* c, float* out) {
+int swe_windmap(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
-  if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context uses swe_windmap_terrain");
+  return swe_windmap_terrain(c, 0, out);
+}
+
+int swe_normals_terrain(swe_ctx* c, int32_t k, float* out) {
+  if (!c || !out) return SWE_EINVAL;
+  if (k < 0 || k >= c->nterr) return fail(c, SWE_EINVAL, "terrain index out of range");
   HIPCHK(c, hipSetDevice(c->device));
+  const int32_t nx = c->tnx[(size_t)k], ny = c->tny[(size_t)k];
+  const int64_t c0 = c->tcell0[(size_t)k], n = (int64_t)nx * ny;
   float* tmp = nullptr;
-  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 3 * c->ncell));
-  swe::launch_normals(c->zs[c->cur()], tmp, c->nx, c->ny, c->p.mapscale, c->p.periodic != 0, c->stream);
+  HIPCHK(c, hipMalloc((void**)&tmp, sizeof(float) * 3 * n));
+  swe::launch_normals(c->zs[c->cur()] + c0, tmp, nx, ny, c->p.mapscale, c->p.periodic != 0, c->stream);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 3 * c->ncell, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream);
   hipError_t e2 = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(c, SWE_EHIP, std::string("normals: ") + hipGetErrorString(e));
   if (e2 != hipSuccess) return fail(c, SWE_EHIP, std::string("normals sync: ") + hipGetErrorString(e2));
   return 0;
+}
+
+int swe_normals(swe_ctx* c, float* out) {
+  if (!c || !out) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "a batch context uses swe_normals_terrain");
+  return swe_normals_terrain(c, 0, out);
 }
 
 int swe_normals_noalloc(swe_ctx* c, float* out) {

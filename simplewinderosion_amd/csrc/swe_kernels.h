@@ -59,15 +59,36 @@ struct RelaxArgs {
   float maxdiff, settling;
 };
 
+// Colour classes of the colour-ordered cascade along one axis of n cells.
+// Bounded: x mod 3.  Periodic, n = 3q + r: x mod 3 for x < 3q and each of
+// the last r cells a class of its own, so that two cells of one class are at
+// least three apart across the wrap too (their 3x3 neighbourhoods are
+// disjoint).  Classes run in order, x class major.
+__host__ __device__ inline int32_t colour_classes(int32_t n, bool periodic) { return periodic ? 3 + n % 3 : 3; }
+__host__ __device__ inline int32_t colour_count(int32_t n, bool periodic, int32_t col) {
+  const int32_t lim = periodic ? n - n % 3 : n;
+  if (col < 3) return lim > col ? (lim - col + 2) / 3 : 0;
+  return (periodic && col - 3 < n % 3) ? 1 : 0;
+}
+__host__ __device__ inline int32_t colour_cell(int32_t n, bool periodic, int32_t col, int32_t i) {
+  (void)periodic;
+  return col < 3 ? col + 3 * i : n - n % 3 + (col - 3);
+}
+// Settling factors above this use the colour-ordered cascade when
+// cascade_order is automatic; the synchronous cascade rejects them.
+constexpr float kSettlingMaxSync = 0.25f;
+
 void launch_march(const MarchArgs& a, bool first, hipStream_t s);
 void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
+// the window's relax as masters += pickups, then one launch per colour class
+void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
 void launch_zero64(int64_t* p, int64_t n, hipStream_t s);
 void launch_zero32(uint32_t* p, int64_t n, hipStream_t s);
 void launch_from_f32(const float* in, int64_t* master, float* snap, int64_t n, hipStream_t s);
 void launch_snap(const int64_t* master, float* snap, int64_t n, hipStream_t s);
 void launch_ema(float* ema, const int64_t* trk, int64_t n4, float lrate, hipStream_t s);
 void launch_bump(uint64_t* cycle, hipStream_t s);
-void launch_windmap(const float* mom, float* out, int64_t n, hipStream_t s);
+void launch_windmap(const float* mom, int64_t plane, float* out, int64_t n, hipStream_t s);
 void launch_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic, hipStream_t s);
 void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic,
                     hipStream_t s);

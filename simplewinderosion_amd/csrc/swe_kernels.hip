@@ -281,6 +281,93 @@ __global__ __launch_bounds__(kRelaxThreads) void k_relax(RelaxArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Colour-ordered relax: the window's cascades one colour class after another
+// ---------------------------------------------------------------------------
+// Masters after the window's pickups, their snapshots, and the next window's
+// event buffer cleared.
+template <int LAYERS>
+__global__ void k_colour_begin(RelaxArgs a, int64_t n) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = a.zm_in[c] + a.dz[c];
+    a.zm_out[c] = z;
+    a.zs_out[c] = f32_from_fx(z);
+    a.dz[c] = 0;
+    a.v_clear[c] = 0;
+    if (LAYERS == 2) {
+      const int64_t v = a.sm_in[c] + a.ds[c];
+      a.sm_out[c] = v;
+      a.ss_out[c] = f32_from_fx(v);
+      a.ds[c] = 0;
+    }
+  }
+}
+
+// One colour class (cx, cy): every cell of the class with events runs its
+// cascade against the current map (zs_out / ss_out) and adds its transfers
+// to the masters at once.  Cells of one class are at least three apart, so
+// no two threads touch the same cell: plain read-modify-writes.
+template <bool PERIODIC, int LAYERS>
+__global__ void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, int32_t nj) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ni * nj) return;
+  const int32_t nx = a.nx, ny = a.ny;
+  const int32_t x = colour_cell(nx, PERIODIC, cx, (int32_t)(t / nj));
+  const int32_t y = colour_cell(ny, PERIODIC, cy, (int32_t)(t % nj));
+  const int64_t c = (int64_t)x * ny + y;
+  const uint32_t v = a.V[c];
+  if (v == 0) return;
+  int64_t nb[8];
+  float hn[8], sn[8];
+  uint32_t valid = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    int32_t qx = x + c_offx[k], qy = y + c_offy[k];
+    bool ok = true;
+    if (PERIODIC) {
+      qx = qx < 0 ? qx + nx : (qx >= nx ? qx - nx : qx);
+      qy = qy < 0 ? qy + ny : (qy >= ny ? qy - ny : qy);
+    } else {
+      ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
+    }
+    nb[k] = ok ? (int64_t)qx * ny + qy : c;
+    hn[k] = ok ? a.zs_out[nb[k]] : 0.0f;
+    sn[k] = (LAYERS == 2 && ok) ? a.ss_out[nb[k]] : 0.0f;
+    valid |= (ok ? 1u : 0u) << k;
+  }
+  const float hc = a.zs_out[c];
+  const float sc = LAYERS == 2 ? a.ss_out[c] : 0.0f;
+  int64_t centre = 0;
+  const auto sed_of = [&](int32_t k) { return sn[k]; };
+  cascade_cell_dev<LAYERS>(hn, sed_of, hc, sc, valid, settle_factor_dev(a.settling, v), a.maxdiff,
+                           [&](int32_t k, int64_t amt) {
+                             a.zm_out[nb[k]] += amt;
+                             if (LAYERS == 2) a.sm_out[nb[k]] += amt;
+                             centre -= amt;
+                           });
+  a.zm_out[c] += centre;
+  if (LAYERS == 2) a.sm_out[c] += centre;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (!((valid >> k) & 1u)) continue;
+    a.zs_out[nb[k]] = f32_from_fx(a.zm_out[nb[k]]);
+    if (LAYERS == 2) a.ss_out[nb[k]] = f32_from_fx(a.sm_out[nb[k]]);
+  }
+  a.zs_out[c] = f32_from_fx(a.zm_out[c]);
+  if (LAYERS == 2) a.ss_out[c] = f32_from_fx(a.sm_out[c]);
+}
+
+// the sediment layer clamped at zero after the last class
+__global__ void k_colour_end(RelaxArgs a, int64_t n) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t v = a.sm_out[c];
+    if (v < 0) {
+      a.sm_out[c] = 0;
+      a.ss_out[c] = 0.0f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Elementwise kernels
 // ---------------------------------------------------------------------------
 __global__ void k_zero64(int64_t* p, int64_t n) {
@@ -354,10 +441,11 @@ __global__ void k_mesh(const float* S, float* out, int32_t nx, int32_t ny, float
   }
 }
 
-// SimpleWindErosion.cpp:224-230: colour = 0.5 (1 + erf(2 m)) per momentum axis
-__global__ void k_windmap(const float* mom, float* out, int64_t n) {
+// SimpleWindErosion.cpp:224-230: colour = 0.5 (1 + erf(2 m)) per momentum axis;
+// the x, y, z planes of n cells lie `plane` floats apart
+__global__ void k_windmap(const float* mom, int64_t plane, float* out, int64_t n) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
-    const float mx = mom[c], my = mom[n + c], mz = mom[2 * n + c];
+    const float mx = mom[c], my = mom[plane + c], mz = mom[2 * plane + c];
     out[4 * c + 0] = 0.5f * (1.0f + erff(2.0f * mx));
     out[4 * c + 1] = 0.5f * (1.0f + erff(2.0f * my));
     out[4 * c + 2] = 0.5f * (1.0f + erff(2.0f * mz));
@@ -456,6 +544,28 @@ void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s) 
   }
 }
 
+void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, hipStream_t s) {
+  const int64_t n = (int64_t)a.nx * a.ny;
+  if (layers == 2) hipLaunchKernelGGL(k_colour_begin<2>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+  else hipLaunchKernelGGL(k_colour_begin<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+  const int32_t ncx = colour_classes(a.nx, periodic), ncy = colour_classes(a.ny, periodic);
+  for (int32_t cx = 0; cx < ncx; cx++)
+    for (int32_t cy = 0; cy < ncy; cy++) {
+      const int32_t ni = colour_count(a.nx, periodic, cx), nj = colour_count(a.ny, periodic, cy);
+      const int64_t m = (int64_t)ni * nj;
+      if (m == 0) continue;
+      const dim3 g((unsigned)((m + 255) / 256));
+      if (periodic) {
+        if (layers == 2) hipLaunchKernelGGL((k_colour_pass<true, 2>), g, dim3(256), 0, s, a, cx, cy, ni, nj);
+        else hipLaunchKernelGGL((k_colour_pass<true, 1>), g, dim3(256), 0, s, a, cx, cy, ni, nj);
+      } else {
+        if (layers == 2) hipLaunchKernelGGL((k_colour_pass<false, 2>), g, dim3(256), 0, s, a, cx, cy, ni, nj);
+        else hipLaunchKernelGGL((k_colour_pass<false, 1>), g, dim3(256), 0, s, a, cx, cy, ni, nj);
+      }
+    }
+  if (layers == 2) hipLaunchKernelGGL(k_colour_end, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+}
+
 void launch_zero64(int64_t* p, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero64, dim3(grid_for(n, 256)), dim3(256), 0, s, p, n);
 }
@@ -476,8 +586,8 @@ void launch_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapsc
   if (periodic) hipLaunchKernelGGL((k_mesh<true>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
   else hipLaunchKernelGGL((k_mesh<false>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
 }
-void launch_windmap(const float* mom, float* out, int64_t n, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_windmap, dim3(grid_for(n, 256)), dim3(256), 0, s, mom, out, n);
+void launch_windmap(const float* mom, int64_t plane, float* out, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_windmap, dim3(grid_for(n, 256)), dim3(256), 0, s, mom, plane, out, n);
 }
 void launch_bump(uint64_t* cycle, hipStream_t s) {
   hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, s, cycle);

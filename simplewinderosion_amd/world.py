@@ -37,7 +37,7 @@ class SweError(RuntimeError):
 
 
 def build_params(abrasion=0.0, mapscale=80.0, window=1, periodic=False, track=True, layers=1,
-                 deposit_on_death=False, spawn=0):
+                 deposit_on_death=False, spawn=0, cascade_order=-1):
     """swe_params from the Wind/World statics plus per-map options (no device needed)."""
     p = capi.SweParams()
     capi.load().swe_params_default(ctypes.byref(p))
@@ -47,7 +47,7 @@ def build_params(abrasion=0.0, mapscale=80.0, window=1, periodic=False, track=Tr
         lrate=float(World.lrate), maxdiff=float(World.maxdiff), settling=float(World.settling),
         abrasion=float(abrasion), mapscale=float(mapscale), window=int(window), periodic=int(bool(periodic)),
         track=int(bool(track)), layers=int(layers), deposit_on_death=int(bool(deposit_on_death)),
-        spawn=int(spawn), seed=int(World.SEED) & 0xFFFFFFFFFFFFFFFF,
+        spawn=int(spawn), seed=int(World.SEED) & 0xFFFFFFFFFFFFFFFF, cascade_order=int(cascade_order),
     )
     return p
 
@@ -69,8 +69,9 @@ class World:
     settling = 1.0
 
     def __init__(self, nx=512, ny=512, device=0, periodic=False, track=True, layers=1, window=1,
-                 abrasion=0.0, deposit_on_death=False, mapscale=80.0, spawn=0):
+                 abrasion=0.0, deposit_on_death=False, mapscale=80.0, spawn=0, cascade_order=-1):
         self.lib = capi.load()
+        self.cascade_order = int(cascade_order)
         self.nx, self.ny = int(nx), int(ny)
         self.periodic = bool(periodic)
         self.track = bool(track)
@@ -90,7 +91,7 @@ class World:
     def options(self):
         return dict(abrasion=self.abrasion, mapscale=self.mapscale, window=self.window,
                     periodic=self.periodic, track=self.track, layers=self.layers,
-                    deposit_on_death=self.deposit_on_death, spawn=self.spawn)
+                    deposit_on_death=self.deposit_on_death, spawn=self.spawn, cascade_order=self.cascade_order)
 
     def params(self):
         return build_params(**self.options())
@@ -207,14 +208,18 @@ class World:
         self._check(self.lib.swe_load_state(self.ctx, str(path).encode()), "swe_load_state")
         p = capi.SweParams()
         self._check(self.lib.swe_get_params(self.ctx, ctypes.byref(p)), "swe_get_params")
-        Wind.maxAge, Wind.boundarylayer, Wind.suspension, Wind.gravity = (
-            p.max_age, p.boundarylayer, p.suspension, p.gravity)
-        Wind.pspeed = tuple(p.pspeed)
-        World.lrate, World.maxdiff, World.settling, World.SEED = p.lrate, p.maxdiff, p.settling, p.seed
+        _statics_from(p)
         self.abrasion, self.mapscale, self.window = p.abrasion, p.mapscale, p.window
         self.periodic, self.track, self.layers = bool(p.periodic), bool(p.track), p.layers
         self.deposit_on_death = bool(p.deposit_on_death)
         self.spawn = p.spawn
+        self.cascade_order = p.cascade_order
+
+    def export(self, path, fmt="npz"):
+        """Write the flow-field and map products: fmt "npz" (height, sediment,
+        massflow, momentum, windmap, normals and the exact masters) or "png"
+        (the wind map of SimpleWindErosion.cpp:224-230 as RGBA8)."""
+        self._check(self.lib.swe_export(self.ctx, str(path).encode(), _export_format(fmt)), "swe_export")
 
     def set_timing(self, on):
         """False/0: off; True/1: time every kernel launch; n > 1: every n-th."""
@@ -274,6 +279,21 @@ def generate_terrain(nx, ny, seed, octaves=6, base_cycles=4.0, n_pyramids=0, pyr
     if rc != 0:
         raise SweError("swe_generate_terrain failed (%d)" % rc)
     return out.reshape(int(nx), int(ny))
+
+
+def _statics_from(p):
+    """Set the Wind / World statics from a swe_params (after a checkpoint load)."""
+    Wind.maxAge, Wind.boundarylayer, Wind.suspension, Wind.gravity = (
+        p.max_age, p.boundarylayer, p.suspension, p.gravity)
+    Wind.pspeed = tuple(p.pspeed)
+    World.lrate, World.maxdiff, World.settling, World.SEED = p.lrate, p.maxdiff, p.settling, p.seed
+
+
+def _export_format(fmt):
+    formats = {"npz": capi.SWE_EXPORT_NPZ, "png": capi.SWE_EXPORT_PNG}
+    if fmt not in formats:
+        raise ValueError("export format must be 'npz' or 'png', not %r" % (fmt,))
+    return formats[fmt]
 
 
 class Batch:
@@ -376,6 +396,48 @@ class Batch:
         a = capi.SweAudit()
         self._check(self.lib.swe_audit_terrain(self.ctx, int(k), ctypes.byref(a)), "swe_audit_terrain")
         return {"height_sum": a.height_sum, "sediment_sum": a.sediment_sum, "lost": a.lost, "cycle": a.cycle}
+
+    def windmap(self, k):
+        nx, ny = self.sizes[k]
+        out = np.empty(4 * nx * ny, np.float32)
+        self._check(self.lib.swe_windmap_terrain(self.ctx, int(k), _fp(out)), "swe_windmap_terrain")
+        return out.reshape(nx, ny, 4)
+
+    def normals(self, k):
+        nx, ny = self.sizes[k]
+        out = np.empty(3 * nx * ny, np.float32)
+        self._check(self.lib.swe_normals_terrain(self.ctx, int(k), _fp(out)), "swe_normals_terrain")
+        return out.reshape(3, nx, ny)
+
+    def terrain(self, k):
+        """Terrain k's (nx, ny, seed, pspeed) as the context has them."""
+        nx, ny, ov = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        seed = ctypes.c_uint64()
+        ps = (ctypes.c_float * 3)()
+        self._check(self.lib.swe_get_terrain(self.ctx, int(k), ctypes.byref(nx), ctypes.byref(ny),
+                                             ctypes.byref(seed), ps, ctypes.byref(ov)), "swe_get_terrain")
+        return nx.value, ny.value, seed.value, tuple(ps)
+
+    def save(self, path):
+        """Checkpoint every terrain's exact state, its seed and wind, the shared
+        params and the counters to `path`."""
+        self._check(self.lib.swe_save_state(self.ctx, str(path).encode()), "swe_save_state")
+
+    def load(self, path):
+        """Restore a checkpoint written by save() into a batch of the same
+        terrain sizes; the statics and options follow the file."""
+        self._check(self.lib.swe_load_state(self.ctx, str(path).encode()), "swe_load_state")
+        p = capi.SweParams()
+        self._check(self.lib.swe_get_params(self.ctx, ctypes.byref(p)), "swe_get_params")
+        _statics_from(p)
+        self.opts = dict(abrasion=p.abrasion, mapscale=p.mapscale, window=1, periodic=bool(p.periodic),
+                         track=bool(p.track), layers=p.layers, deposit_on_death=bool(p.deposit_on_death),
+                         spawn=p.spawn)
+
+    def export(self, k, path, fmt="npz"):
+        """World.export for terrain k."""
+        self._check(self.lib.swe_export_terrain(self.ctx, int(k), str(path).encode(), _export_format(fmt)),
+                    "swe_export_terrain")
 
     def close(self):
         if getattr(self, "ctx", None) is not None and self.ctx.value:

@@ -1,0 +1,68 @@
+"""The colour-ordered cascade on the GPU (cascade_order 1, or automatic above
+settling 0.25): bit-exact against the oracle, including the reference's own
+defaults at its density (4096 particles on 512^2, SimpleWindErosion.cpp:217)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from simplewinderosion_amd import Batch, World, Wind
+from simplewinderosion_amd.world import SweError
+
+from test_gpu_parity import assert_same, run_pair, say
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # name, nx, ny, particles, cycles, options
+    ("bounded-auto-s1", 40, 37, 300, 2, dict(periodic=False, window=1, settling=1.0)),
+    ("periodic-2layer-s1", 32, 29, 500, 2, dict(periodic=True, window=1, settling=1.0, layers=2, abrasion=1e-3)),
+    ("periodic-tiny-1x7", 1, 7, 20, 2, dict(periodic=True, window=1, settling=1.0)),
+    ("periodic-tiny-2x5", 2, 5, 20, 2, dict(periodic=True, window=1, settling=1.0)),
+    ("bounded-window4", 48, 30, 400, 2, dict(periodic=False, window=4, settling=0.6, cascade_order=1)),
+    ("periodic-explicit-low", 64, 64, 2000, 1, dict(periodic=True, window=1, settling=0.05, cascade_order=1,
+                                                     deposit_on_death=True)),
+]
+
+
+@pytest.mark.parametrize("name,nx,ny,particles,cycles,opts", CASES, ids=[c[0] for c in CASES])
+def test_colour_cascade_matches_oracle(statics, name, nx, ny, particles, cycles, opts):
+    say("case colour", name)
+    Wind.suspension = 0.01
+    w, st = run_pair(nx, ny, particles, cycles, seed=7, **opts)
+    assert O.load().or_colour_mode(st.p) == 1
+    assert_same(w, st)
+    w.close()
+
+
+def test_reference_defaults_at_reference_density(statics):
+    """Every parameter at the reference's value (bounded 512^2 map, settling
+    1, suspension 0.05, maxdiff 0.005, pspeed (1,0,0)), 4096 particles per
+    cycle as SimpleWindErosion.cpp:217, two cycles: the oracle's bits, the
+    map bounded."""
+    say("case reference defaults 512^2 x 4096")
+    from simplewinderosion_amd import world as Wm
+
+    h = (Wm.generate_terrain(512, 512, 3, 6, 4.0) * 0.5).astype(np.float32)
+    w, st = run_pair(512, 512, 4096, 2, seed=1, height=h)
+    assert w.params().settling == 1.0 and w.params().cascade_order == -1
+    assert_same(w, st)
+    z = O.unfx_vec(st.zm)
+    assert z.min() >= h.min() - 0.05 and z.max() <= h.max() + 0.05
+    w.close()
+
+
+def test_synchronous_cascade_refuses_large_settling(statics):
+    say("case colour refusals")
+    World.settling = 1.0
+    with pytest.raises(SweError):
+        World(16, 16, cascade_order=0)
+    # batches and the tile path need the synchronous cascade
+    b = Batch([(32, 32), (16, 16)], periodic=True)
+    with pytest.raises(SweError):
+        b.erode([100, 50])
+    b.close()
+    w = World(32, 32, periodic=True)
+    w.set_path(2)
+    with pytest.raises(SweError):
+        w.erode(100)
+    w.close()

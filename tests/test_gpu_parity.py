@@ -29,7 +29,8 @@ def run_pair(nx, ny, particles, cycles, seed=1, sed=None, map_seed=0, height=Non
     rng = np.random.default_rng(map_seed)
     h = height if height is not None else (rng.random((nx, ny)) * 0.5).astype(np.float32)
     w = World(nx, ny, **{k: v for k, v in opts.items() if k in
-                         ("periodic", "track", "layers", "window", "abrasion", "deposit_on_death")})
+                         ("periodic", "track", "layers", "window", "abrasion", "deposit_on_death",
+                          "cascade_order")})
     w.upload(h, sed)
     w.set_path(path)
     World.SEED = seed
