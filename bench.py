@@ -61,31 +61,72 @@ def measured_traffic(workload, p):
     return float(vals["traffic_bytes_per_launch"])
 
 
-def cpu_baseline(workload, seed, target_s):
-    """The reference's sequential algorithm (oracle port), one core, on a bounded sample."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+class pinned_to_one_core:
+    """Pin the calling thread (the one that runs the sequential port) to the
+    first core it may use, and restore its affinity afterwards."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+
+
+def _seq_run(O, p, w, h, s, particles, cycles, seed):
     import numpy as np
+
+    hh, ss = h.ravel().copy(), None if s is None else s.ravel().copy()
+    ema = np.zeros(4 * h.size, np.float32)
+    t0 = time.perf_counter()
+    steps = O.erode_seq(p, w["nx"], w["ny"], hh, ss, ema, particles=particles, n_cycles=cycles, seed=seed)
+    return steps, time.perf_counter() - t0
+
+
+def cpu_baseline(workload, seed, target_s, full=False):
+    """The reference's sequential algorithm (oracle port) on one pinned core:
+    by default a bounded sample of the workload's first cycle (~target_s
+    seconds); with full=True one whole cycle of it, the 256^2 CPU workload in
+    full (100 cycles x 2,000 particles), and the batched oracle on every core
+    this process may use, for information."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from simplewinderosion_amd import workloads
 
     w = workloads.WORKLOADS[workload]
     h, s = workloads.make_inputs(workload, seed)
     p = O.params_from(workloads.params(workload, seed))
-    ema = np.zeros(4 * h.size, np.float32)
-    # calibrate on a small sample, then run ~target_s seconds of particles
-    hh, ss = h.ravel().copy(), None if s is None else s.ravel().copy()
+    cores_all = len(os.sched_getaffinity(0))
+    with pinned_to_one_core() as pin:
+        if not full:
+            # calibrate on a small sample, then run ~target_s seconds of particles
+            steps, dt = _seq_run(O, p, w, h, s, 200, 1, seed)
+            n = max(200, int(target_s * steps / dt / (p.max_age + 1)))
+            steps, dt = _seq_run(O, p, w, h, s, n, 1, seed)
+            return {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+                    "sample": "%d particles (%d particle-steps) of one %dx%d cycle, sequential glibc-rand order, "
+                              "%.1f s on core %d" % (n, steps, w["nx"], w["ny"], dt, pin.core)}
+        steps, dt = _seq_run(O, p, w, h, s, w["particles"], 1, seed)
+        out = {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+               "sample": "one full %dx%d cycle: %d particles, %d particle-steps, sequential glibc-rand order, "
+                         "%.1f s on core %d" % (w["nx"], w["ny"], w["particles"], steps, dt, pin.core)}
+        a = workloads.WORKLOADS["cpu256"]
+        ha, sa = workloads.make_inputs("cpu256", seed)
+        pa = O.params_from(workloads.params("cpu256", seed))
+        steps_a, dt_a = _seq_run(O, pa, a, ha, sa, a["particles"], a["cycles"], seed)
+        out["cpu256_full"] = {"value": steps_a / dt_a, "seconds": dt_a, "particle_steps": steps_a, "cores": 1,
+                              "sample": "256x256, %d cycles x %d particles" % (a["cycles"], a["particles"])}
+    # the batched oracle (OpenMP over particles) on all cores, one cycle
+    st = O.OracleState(w["nx"], w["ny"], h, sediment=s, params=p)
     t0 = time.perf_counter()
-    steps = O.erode_seq(p, w["nx"], w["ny"], hh, ss, ema, particles=200, n_cycles=1, seed=seed)
-    dt = time.perf_counter() - t0
-    rate = steps / dt
-    n = max(200, int(target_s * rate / (p.max_age + 1)))
-    hh, ss = h.ravel().copy(), None if s is None else s.ravel().copy()
-    t0 = time.perf_counter()
-    steps = O.erode_seq(p, w["nx"], w["ny"], hh, ss, ema, particles=n, n_cycles=1, seed=seed)
-    dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": "%d particles (%d particle-steps) of one %dx%d cycle, sequential glibc-rand order, %.1f s"
-                      % (n, steps, w["nx"], w["ny"], dt)}
+    st.erode(w["particles"], 1, nthreads=cores_all)
+    dt_b = time.perf_counter() - t0
+    out["batched_oracle"] = {"value": st.steps.value / dt_b,
+                             "seconds": dt_b, "cores": cores_all,
+                             "sample": "one full %dx%d cycle, batched-synchronous method" % (w["nx"], w["ny"])}
+    return out
 
 
 def calibrate_copy(mib):
@@ -154,6 +195,9 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="CPU baseline on one whole cycle, plus the 256^2 CPU workload in full and the batched "
+                         "oracle on all cores (several minutes)")
     ap.add_argument("--window", type=int, default=None)
     ap.add_argument("--track", type=int, default=None)
     ap.add_argument("--tile-order", type=int, default=None,
@@ -270,7 +314,7 @@ def main():
     world.close()
     if not args.no_cpu:
         print("bench: CPU baseline (sequential port, 1 core, ~%.0f s)" % args.cpu_seconds, file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(args.workload, args.seed, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(args.workload, args.seed, args.cpu_seconds, full=args.cpu_full)
         line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     else:
         line["cpu_baseline"] = None
