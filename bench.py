@@ -150,6 +150,27 @@ def calibrate_copy(mib):
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "copies": 3}), flush=True)
 
 
+def apply_tuning(target, args, particles):
+    """Launch knobs from the command line, or the autotuner's choice
+    (--autotune); returns the setting in effect."""
+    from simplewinderosion_amd import capi
+
+    res = None
+    if args.autotune:
+        res = target.autotune(particles, trial_cycles=2, cache=None)
+    for knob, v in ((capi.SWE_TUNE_SHAPE, args.shape), (capi.SWE_TUNE_TILE_ORDER, args.tile_order),
+                    (capi.SWE_TUNE_GRAPH, args.graph), (capi.SWE_TUNE_HEAVY_FIRST, args.heavy_first)):
+        if v is not None:
+            target.set_tune(knob, v)
+    out = {"autotuned": bool(args.autotune),
+           "shape": target.get_tune(capi.SWE_TUNE_SHAPE), "tile_order": target.get_tune(capi.SWE_TUNE_TILE_ORDER),
+           "graph": target.get_tune(capi.SWE_TUNE_GRAPH), "heavy_first": target.get_tune(capi.SWE_TUNE_HEAVY_FIRST)}
+    if res is not None:
+        out["candidates"] = res["candidates"]
+        out["tuned_ms_per_cycle"] = res["ms_per_cycle"]
+    return out
+
+
 def bench_batch(args):
     """The multi-terrain workload: 64 terrains (sides log-uniform in
     128..1024, own seed, wind azimuth uniform at 45 degrees elevation,
@@ -158,12 +179,8 @@ def bench_batch(args):
 
     terrains = workloads.batch64(seed=args.seed)
     b = workloads.make_batch(terrains)
-    from simplewinderosion_amd import capi
-    for knob, v in ((capi.SWE_TUNE_TILE_ORDER, args.tile_order), (capi.SWE_TUNE_GRAPH, args.graph),
-                    (capi.SWE_TUNE_HEAVY_FIRST, args.heavy_first)):
-        if v is not None:
-            b.set_tune(knob, v)
     parts = [t["particles"] for t in terrains]
+    tuned = apply_tuning(b, args, parts)
     print("bench: batch64, %d terrains, %d cells, warmup %d, steps %d"
           % (len(terrains), sum(parts), args.warmup, args.steps), file=sys.stderr, flush=True)
     b.erode(parts, n_cycles=args.warmup)
@@ -181,7 +198,7 @@ def bench_batch(args):
             "config": {"workload": "batch64", "terrains": len(terrains), "cells": sum(parts),
                        "particles_per_cycle": sum(parts), "window": 1, "layers": 2, "periodic": 1,
                        "parallelism": "single GPU"},
-            "tiles": {"max_particles_per_tile": st["max_tile_particles"]}}
+            "tiles": {"max_particles_per_tile": st["max_tile_particles"]}, "tuning": tuned}
     b.close()
     print(json.dumps(line), flush=True)
 
@@ -207,6 +224,10 @@ def main():
     ap.add_argument("--heavy-first", type=int, default=None,
                     help="SWE_TUNE_HEAVY_FIRST: 1 = tiles crowded in one window run first in the next, "
                          "2 = and light ones last (the library's default)")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="SWE_TUNE_SHAPE: the step kernel's compiled tile shape (0 = 16x32 cells, the default)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the library's autotuner (shapes, tile orders, graphs, heavy-first) before timing")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -236,13 +257,7 @@ def main():
         over["track"] = bool(args.track)
     world = workloads.make_world(args.workload, seed=args.seed, **over)
     p = world.params()
-    from simplewinderosion_amd import capi
-    if args.tile_order is not None:
-        world.set_tune(capi.SWE_TUNE_TILE_ORDER, args.tile_order)
-    if args.graph is not None:
-        world.set_tune(capi.SWE_TUNE_GRAPH, args.graph)
-    if args.heavy_first is not None:
-        world.set_tune(capi.SWE_TUNE_HEAVY_FIRST, args.heavy_first)
+    tuned = apply_tuning(world, args, [wl["particles"]])
     if not p.periodic:
         raise SystemExit("bench counts particle-steps analytically: periodic workloads only")
     N = wl["particles"]
@@ -309,6 +324,7 @@ def main():
                   "crowded_tile_windows": st["crowded_tile_windows"], "far_moves": st["far_particles"],
                   "heavy_first_tile_windows": st["heavy_tile_windows"]},
         "roofline": roofline,
+        "tuning": tuned,
     }
     line["mass_exact"] = bool(audit["height_sum"] + audit["lost"] == mass0)
     world.close()

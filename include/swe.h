@@ -193,17 +193,23 @@ int swe_set_path(swe_ctx* ctx, int32_t path);
  *     Default: 2 for contexts of up to 4096 tiles (16x32 cells each), 1
  *     above (measured best on a 1024^2 map and on a 64-terrain batch).
  *     Contexts of fewer than 64 tiles always use 0.
+ *   SWE_TUNE_SHAPE: the step kernel's tile shape, an index into the
+ *     compiled shapes 0 = 16x32 cells / 256 threads / 5 waves per SIMD
+ *     (default), 1 = 32x16 / 256 / 5, 2 = 16x16 / 128 / 5, 3 = 16x32 / 256 / 4,
+ *     4 = 32x32 / 512 / 4 (x the slow axis).  Changing it synchronizes the
+ *     context's stream and rebuilds its per-tile buffers.
  */
-enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2, SWE_TUNE_HEAVY_FIRST = 3 };
+enum swe_tune_knob { SWE_TUNE_TILE_ORDER = 1, SWE_TUNE_GRAPH = 2, SWE_TUNE_HEAVY_FIRST = 3, SWE_TUNE_SHAPE = 4 };
 int swe_set_tune(swe_ctx* ctx, int32_t knob, int64_t value);
 int swe_get_tune(swe_ctx* ctx, int32_t knob, int64_t* value);
 
 /*
  * Autotuner (no reference counterpart): times `trial_cycles` map-cycles of
  * the context's own workload (particles per terrain, as for
- * swe_erode_batch) for every tile order in {0, 1, 4, 16} with and without
- * graphs (and, on maps of 64 tiles or more, with and without
- * SWE_TUNE_HEAVY_FIRST), each from the same saved state, then restores the state (map,
+ * swe_erode_batch), each from the same saved state: first every tile shape
+ * (SWE_TUNE_SHAPE) with graphs, then on the fastest shape every tile order
+ * in {0, 1, 4, 16} with and without graphs (and, on maps of 64 tiles or
+ * more, every SWE_TUNE_HEAVY_FIRST mode); then restores the state (map,
  * fields, counters and statistics are as before the call) and keeps the
  * fastest setting.  Tile step path (window == 1) only.
  */
@@ -213,6 +219,8 @@ typedef struct swe_tune_result {
   int32_t candidates;    /* settings timed */
   int32_t heavy_first;
   double ms_per_cycle;   /* of the setting kept */
+  int32_t shape;         /* SWE_TUNE_SHAPE kept */
+  int32_t reserved;
 } swe_tune_result;
 int swe_autotune(swe_ctx* ctx, const int64_t* particles, int32_t trial_cycles, swe_tune_result* out);
 /* The GPU architecture of the context's device (e.g. "gfx950:sramecc+:xnack-"), the key of tuning caches. */

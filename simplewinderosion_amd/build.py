@@ -37,17 +37,46 @@ def hipcc():
     return exe
 
 
-def build(verbose=False):
-    """Compile every HIP source into simplewinderosion_amd/libswe.so."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+# the step kernel is compiled once per tile shape (SWE_STEP_SHAPE = index
+# into kStepShape in csrc/swe_step.h)
+SHAPE_SOURCE = "swe_step_shapes.hip"
+N_SHAPES = 5
+OBJ_DIR = os.path.join(ROOT, "_build", "obj")
+
+
+def _compile(args):
+    src, obj, defines = args
+    cmd = [hipcc()] + HIP_FLAGS + defines + ["-c", "-o", obj, src]
+    res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
+    return cmd, res
+
+
+def build(verbose=False, jobs=None):
+    """Compile every HIP source into simplewinderosion_amd/libswe.so: one
+    object per source (the step kernel once per tile shape), compiled in
+    parallel, then linked."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    units = [(os.path.join(CSRC, s), os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o"), []) for s in SOURCES]
+    units += [(os.path.join(CSRC, SHAPE_SOURCE), os.path.join(OBJ_DIR, "swe_step_shape%d.o" % i),
+               ["-DSWE_STEP_SHAPE=%d" % i]) for i in range(N_SHAPES)]
+    jobs = jobs or max(1, min(len(units), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 4)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, res in ex.map(_compile, units):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError("hipcc failed compiling %s" % cmd[-1])
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIP_FLAGS + ["-shared", "-o", tmp] + srcs
+    cmd = [hipcc()] + HIP_FLAGS + ["-shared", "-o", tmp] + [u[1] for u in units]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, cwd=ROOT, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libswe.so")
+        raise RuntimeError("hipcc failed linking libswe.so")
     os.replace(tmp, LIB)
     build_nn(verbose)
     build_demo(verbose)

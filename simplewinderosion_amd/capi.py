@@ -20,6 +20,7 @@ SWE_ESTATE = -5
 SWE_TUNE_TILE_ORDER = 1
 SWE_TUNE_GRAPH = 2
 SWE_TUNE_HEAVY_FIRST = 3
+SWE_TUNE_SHAPE = 4
 
 SWE_EXPORT_NPZ = 0
 SWE_EXPORT_PNG = 1
@@ -111,6 +112,8 @@ class SweTuneResult(ctypes.Structure):
         ("candidates", ctypes.c_int32),
         ("heavy_first", ctypes.c_int32),
         ("ms_per_cycle", ctypes.c_double),
+        ("shape", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

@@ -68,6 +68,7 @@ struct swe_ctx {
   int32_t tile_order = 0;  // SWE_TUNE_TILE_ORDER
   int32_t use_graph = 1;   // SWE_TUNE_GRAPH
   int32_t heavy_first = -1;  // SWE_TUNE_HEAVY_FIRST; -1 = default (2 up to 4096 tiles, else 1)
+  int32_t shape = 0;         // SWE_TUNE_SHAPE: index into swe::kStepShape
   // one captured map-cycle of the tile path per starting window parity, and
   // the launch configuration it was captured for
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -254,15 +255,63 @@ int32_t heavy_first_mode(const swe_ctx* c) {
 // light-tile slots of a launch: half the tiles, a multiple of 8
 int64_t light_cap(int64_t ntiles) { return (ntiles / 2 + 7) / 8 * 8; }
 
+// Tiles of every terrain for step-kernel shape `shape`: per-terrain tile
+// grids, the per-tile terrain index on the device, and the per-tile routing
+// buffers of the previous shape released (ensure_step makes new ones).
+int set_tiles(swe_ctx* c, int32_t shape) {
+  const swe::StepShape sh = swe::kStepShape[shape];
+  int64_t tiles = 0;
+  for (int32_t k = 0; k < c->nterr; k++)
+    tiles += (int64_t)((c->tnx[(size_t)k] + sh.sx - 1) / sh.sx) * ((c->tny[(size_t)k] + sh.sy - 1) / sh.sy);
+  if (tiles > ((int64_t)1 << 30)) return fail(c, SWE_EINVAL, "too many tiles");
+  std::vector<int32_t> tile_terrain((size_t)tiles);
+  std::vector<int32_t> ttx, tty, ttile0;
+  int32_t tile0 = 0;
+  for (int32_t k = 0; k < c->nterr; k++) {
+    const int32_t tx = (c->tnx[(size_t)k] + sh.sx - 1) / sh.sx, ty = (c->tny[(size_t)k] + sh.sy - 1) / sh.sy;
+    ttx.push_back(tx);
+    tty.push_back(ty);
+    ttile0.push_back(tile0);
+    for (int32_t t = 0; t < tx * ty; t++) tile_terrain[(size_t)(tile0 + t)] = k;
+    tile0 += tx * ty;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // no launch may still use the old buffers
+  dfree(c->dtile_terrain);
+  for (int k = 0; k < 2; k++) {
+    dfree(c->sbase[k]);
+    dfree(c->scnt[k]);
+    dfree(c->heavy[k]);
+  }
+  dfree(c->nin);
+  dfree(c->csum);
+  dfree(c->hcnt);
+  dfree(c->hstamp);
+  int r;
+  if ((r = dalloc(c, &c->dtile_terrain, tiles))) return r;
+  HIPCHK(c, hipMemcpyAsync(c->dtile_terrain, tile_terrain.data(), sizeof(int32_t) * tile_terrain.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->ttx = ttx;
+  c->tty = tty;
+  c->ttile0 = ttile0;
+  c->ntiles = (int32_t)tiles;
+  c->shape = shape;
+  return 0;
+}
+
 // buffers of the tile-owned step path for n particles
 int ensure_step(swe_ctx* c, int64_t n) {
   int r;
   // particle indices and record positions are 32-bit on this path
   if (n > (int64_t)UINT32_MAX) return fail(c, SWE_EINVAL, "at most 2^32 - 1 particles per cycle on the tile step path");
   if (!c->fV[0]) {
-    const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
     if ((r = dalloc(c, &c->fV[0], c->ncell)) || (r = dalloc(c, &c->fV[1], c->ncell)) ||
-        (r = dalloc(c, &c->farcnt, 3)) || (r = dalloc(c, &c->sbase[0], 9 * nt)) ||
+        (r = dalloc(c, &c->farcnt, 3)))
+      return r;
+  }
+  if (!c->sbase[0]) {  // per-tile buffers of the current tile shape
+    const int64_t nt = c->ntiles, nch = (nt + swe::kStepChunk - 1) / swe::kStepChunk;
+    if ((r = dalloc(c, &c->sbase[0], 9 * nt)) ||
         (r = dalloc(c, &c->sbase[1], 9 * nt)) || (r = dalloc(c, &c->scnt[0], 9 * nt)) ||
         (r = dalloc(c, &c->scnt[1], 9 * nt)) || (r = dalloc(c, &c->nin, 3 * nt)) ||
         (r = dalloc(c, &c->csum, 3 * nch)) || (r = dalloc(c, &c->heavy[0], swe::kHeavySlotsMax + light_cap(nt))) ||
@@ -509,6 +558,8 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sp.ntiles = c->ntiles;
   sp.periodic = c->p.periodic;
   sp.spawn_mode = c->p.spawn;
+  sp.tile_sx = swe::kStepShape[c->shape].sx;
+  sp.tile_sy = swe::kStepShape[c->shape].sy;
 
   // the launches of one map-cycle (kernels only: no copies, fills or host
   // synchronisation, so that the sequence can be captured into a graph)
@@ -566,8 +617,8 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         int rr;
         if ((rr = timed_begin(c, &e0, &e1))) return rr;
-        swe::launch_step(sa, c->p.periodic != 0, c->p.track != 0, c->p.layers, c->p.deposit_on_death != 0,
-                         w == nwin, c->stream);
+        swe::launch_step(c->shape, sa, c->p.periodic != 0, c->p.track != 0, c->p.layers,
+                         c->p.deposit_on_death != 0, w == nwin, c->stream);
         HIPCHK(c, hipGetLastError());
         if ((rr = timed_end(c, e0, e1, 2))) return rr;
         c->g++;
@@ -590,6 +641,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     key.append((const char*)ptrs, sizeof(ptrs));
     key.append((const char*)&c->tile_order, sizeof(c->tile_order));
     key.append((const char*)&c->heavy_first, sizeof(c->heavy_first));
+    key.append((const char*)&c->shape, sizeof(c->shape));
     key.append((const char*)&nwin, sizeof(nwin));
     key.append((const char*)&c->alloc_gen, sizeof(c->alloc_gen));
   }
@@ -716,14 +768,12 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
   if (!out) return SWE_EINVAL;
   *out = nullptr;
   if (n_terrains < 1 || n_terrains > 65536 || !nx || !ny) return SWE_EINVAL;
-  int64_t cells = 0, tiles = 0;
+  int64_t cells = 0;
   for (int32_t k = 0; k < n_terrains; k++) {
     if (nx[k] < 1 || ny[k] < 1) return SWE_EINVAL;
     if ((int64_t)nx[k] * ny[k] > ((int64_t)1 << 31) - 1) return SWE_EINVAL;  // cell index fits int32
     cells += (int64_t)nx[k] * ny[k];
-    tiles += (int64_t)((nx[k] + swe::kStepTileX - 1) / swe::kStepTileX) * ((ny[k] + swe::kStepTileY - 1) / swe::kStepTileY);
   }
-  if (tiles > ((int64_t)1 << 30)) return SWE_EINVAL;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return SWE_EHIP;
   if (device < 0 || device >= ndev) return SWE_EINVAL;
@@ -733,22 +783,12 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
   c->nx = nx[0];
   c->ny = ny[0];
   c->ncell = cells;
-  c->ntiles = (int32_t)tiles;
-  std::vector<int32_t> tile_terrain((size_t)tiles);
   int64_t cell0 = 0;
-  int32_t tile0 = 0;
   for (int32_t k = 0; k < n_terrains; k++) {
-    const int32_t tx = (nx[k] + swe::kStepTileX - 1) / swe::kStepTileX;
-    const int32_t ty = (ny[k] + swe::kStepTileY - 1) / swe::kStepTileY;
     c->tnx.push_back(nx[k]);
     c->tny.push_back(ny[k]);
-    c->ttx.push_back(tx);
-    c->tty.push_back(ty);
-    c->ttile0.push_back(tile0);
     c->tcell0.push_back(cell0);
-    for (int32_t t = 0; t < tx * ty; t++) tile_terrain[(size_t)(tile0 + t)] = k;
     cell0 += (int64_t)nx[k] * ny[k];
-    tile0 += tx * ty;
   }
   c->tseed.assign((size_t)n_terrains, 0);
   c->tps.assign((size_t)n_terrains, {1.0f, 0.0f, 0.0f});
@@ -767,16 +807,10 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
       (r = dalloc(c, &c->vis[0], c->ncell)) || (r = dalloc(c, &c->vis[1], c->ncell)) ||
       (r = dalloc(c, &c->ema, 4 * c->ncell)) || (r = dalloc(c, &c->lost, (int64_t)swe::kLostSlots * n_terrains)) ||
       (r = dalloc(c, &c->dstat, (int64_t)swe::kStatKinds * swe::kStatSlots)) ||
-      (r = dalloc(c, &c->dcycle, 1)) || (r = dalloc(c, &c->dterr, n_terrains)) ||
-      (r = dalloc(c, &c->dtile_terrain, tiles)) || (r = ensure_layers(c))) {
+      (r = dalloc(c, &c->dcycle, 1)) || (r = dalloc(c, &c->dterr, n_terrains)) || (r = set_tiles(c, 0)) ||
+      (r = ensure_layers(c))) {
     swe_destroy(c);
     return r;
-  }
-  if (hipMemcpyAsync(c->dtile_terrain, tile_terrain.data(), sizeof(int32_t) * tile_terrain.size(),
-                     hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess) {
-    swe_destroy(c);
-    return SWE_EHIP;
   }
   *out = c;
   return 0;
@@ -1064,11 +1098,44 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
     if (e0) (void)hipEventDestroy(e0);
     return fail(c, SWE_EHIP, std::string("autotune set-up: ") + hipGetErrorString(he));
   }
-  const int32_t order0 = c->tile_order, graph0 = c->use_graph, heavy0 = heavy_first_mode(c), timing0 = c->timing;
+  const int32_t order0 = c->tile_order, graph0 = c->use_graph, heavy_set0 = c->heavy_first, timing0 = c->timing;
+  const int32_t shape0 = c->shape;
   c->timing = 0;
   static const int32_t kOrders[] = {0, 1, 4, 16};
   double best = 1e300;
-  int32_t best_order = order0, best_graph = graph0, best_heavy = heavy0, ncand = 0;
+  int32_t ncand = 0;
+  // one trial: an untimed cycle (graph capture, caches), then the timed
+  // ones, each from the saved state; returns the ms per cycle or < 0
+  auto trial = [&]() -> double {
+    if ((he = save_or_restore(false)) != hipSuccess) return -1.0;
+    if ((r = swe_erode_batch(c, particles, 1))) return -1.0;
+    if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) return -1.0;
+    if ((r = swe_erode_batch(c, particles, trial_cycles))) return -1.0;
+    if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) return -1.0;
+    if ((he = hipEventSynchronize(e1)) != hipSuccess) return -1.0;
+    float ms = 0.0f;
+    if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return -1.0;
+    ncand++;
+    return ms / trial_cycles;
+  };
+  // the tile shape first, with graphs, the first tile order and the
+  // default heavy-first mode
+  int32_t best_shape = shape0;
+  for (int32_t sh = 0; sh < swe::kStepShapes && r == 0 && he == hipSuccess; sh++) {
+    if ((r = swe_set_tune(c, SWE_TUNE_SHAPE, sh))) break;
+    c->tile_order = 0;
+    c->use_graph = 1;
+    c->heavy_first = -1;
+    const double per = trial();
+    if (per < 0) break;
+    if (per < best) {
+      best = per;
+      best_shape = sh;
+    }
+  }
+  if (r == 0 && he == hipSuccess) r = swe_set_tune(c, SWE_TUNE_SHAPE, best_shape);
+  const int32_t heavy0 = heavy_first_mode(c);
+  int32_t best_order = 0, best_graph = 1, best_heavy = -1;
   // heavy-first only changes anything where a launch has heavy slots
   const int32_t heavy_lo = c->ntiles / 8 >= 8 ? 0 : heavy0, heavy_hi = c->ntiles / 8 >= 8 ? 2 : heavy0;
   for (int32_t hv = heavy_hi; hv >= heavy_lo && r == 0 && he == hipSuccess; hv--)
@@ -1077,18 +1144,8 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
       c->tile_order = order;
       c->use_graph = graph;
       c->heavy_first = hv;
-      // one untimed cycle (graph capture, caches), then the timed ones, each
-      // from the saved state
-      if ((he = save_or_restore(false)) != hipSuccess) break;
-      if ((r = swe_erode_batch(c, particles, 1))) break;
-      if ((he = hipEventRecord(e0, c->stream)) != hipSuccess) break;
-      if ((r = swe_erode_batch(c, particles, trial_cycles))) break;
-      if ((he = hipEventRecord(e1, c->stream)) != hipSuccess) break;
-      if ((he = hipEventSynchronize(e1)) != hipSuccess) break;
-      float ms = 0.0f;
-      if ((he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
-      const double per = ms / trial_cycles;
-      ncand++;
+      const double per = trial();
+      if (per < 0) break;
       if (per < best) {
         best = per;
         best_order = order;
@@ -1105,16 +1162,19 @@ int swe_autotune(swe_ctx* c, const int64_t* particles, int32_t trial_cycles, swe
   (void)hipEventDestroy(e1);
   release();
   c->timing = timing0;
-  c->tile_order = r == 0 && he == hipSuccess ? best_order : order0;
-  c->use_graph = r == 0 && he == hipSuccess ? best_graph : graph0;
-  c->heavy_first = r == 0 && he == hipSuccess ? best_heavy : heavy0;
+  const bool ok = r == 0 && he == hipSuccess;
+  c->tile_order = ok ? best_order : order0;
+  c->use_graph = ok ? best_graph : graph0;
+  c->heavy_first = ok ? best_heavy : heavy_set0;
+  if (!ok && c->shape != shape0) (void)swe_set_tune(c, SWE_TUNE_SHAPE, shape0);
   if (r) return r;
   if (he != hipSuccess) return fail(c, SWE_EHIP, std::string("autotune: ") + hipGetErrorString(he));
   out->tile_order = c->tile_order;
   out->graph = c->use_graph;
-  out->heavy_first = c->heavy_first;
+  out->heavy_first = heavy_first_mode(c);
   out->candidates = ncand;
   out->ms_per_cycle = best;
+  out->shape = c->shape;
   return 0;
 }
 
@@ -1148,6 +1208,11 @@ int swe_set_tune(swe_ctx* c, int32_t knob, int64_t value) {
       if (value < 0 || value > 2) return fail(c, SWE_EINVAL, "heavy-first must be 0, 1 or 2");
       c->heavy_first = (int32_t)value;
       return 0;
+    case SWE_TUNE_SHAPE:
+      if (value < 0 || value >= swe::kStepShapes) return fail(c, SWE_EINVAL, "unknown step-kernel shape");
+      if ((int32_t)value == c->shape) return 0;
+      HIPCHK(c, hipSetDevice(c->device));
+      return set_tiles(c, (int32_t)value);
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");
   }
@@ -1164,6 +1229,9 @@ int swe_get_tune(swe_ctx* c, int32_t knob, int64_t* value) {
       return 0;
     case SWE_TUNE_HEAVY_FIRST:
       *value = heavy_first_mode(c);
+      return 0;
+    case SWE_TUNE_SHAPE:
+      *value = c->shape;
       return 0;
     default:
       return fail(c, SWE_EINVAL, "unknown tuning knob");

@@ -6,23 +6,20 @@
 
 namespace swe {
 
-// Tile shape and workgroup size of k_step (compile-time; overridable for tuning)
-#ifndef SWE_STEP_TILE_X
-#define SWE_STEP_TILE_X 16
-#endif
-#ifndef SWE_STEP_TILE_Y
-#define SWE_STEP_TILE_Y 32
-#endif
-#ifndef SWE_STEP_THREADS
-#define SWE_STEP_THREADS 256
-#endif
-// occupancy target of k_step: waves per SIMD the register budget must allow
-#ifndef SWE_STEP_WAVES_PER_EU
-#define SWE_STEP_WAVES_PER_EU 5
-#endif
-constexpr int kStepTileX = SWE_STEP_TILE_X;    // tile extent along x (slow axis)
-constexpr int kStepTileY = SWE_STEP_TILE_Y;    // tile extent along y (fast axis)
-constexpr int kStepTileThreads = SWE_STEP_THREADS;
+// Tile shapes of the step kernel: SX x SY cells (x the slow axis), NT
+// threads, a register budget that keeps WPE waves per SIMD.  Every shape
+// computes the same bits; swe_autotune times them (SWE_TUNE_SHAPE).
+struct StepShape {
+  int32_t sx, sy, threads, waves;
+};
+constexpr int kStepShapes = 5;
+constexpr StepShape kStepShape[kStepShapes] = {
+    {16, 32, 256, 5},  // default: the best measured on the 1024^2 and 64-terrain workloads
+    {32, 16, 256, 5},
+    {16, 16, 128, 5},
+    {16, 32, 256, 4},
+    {32, 32, 512, 4},
+};
 constexpr int kStepChunk = 32;                 // tiles per chunk sum of the input counts
 
 // One terrain of a context: its cells, tiles and particles are contiguous
@@ -139,10 +136,20 @@ struct SpawnArgs {
   int64_t n;         // particles of all terrains
   int32_t ntiles;
   int32_t periodic, spawn_mode;
+  int32_t tile_sx, tile_sy;  // the tile shape the particles are binned for
 };
 
-void launch_step(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,
-                 hipStream_t s);
+void launch_step(int shape, const StepArgs& a, bool periodic, bool track, int layers, bool deposit,
+                 bool relax_only, hipStream_t s);
+#define SWE_DECLARE_SHAPE(i)                                                                                    \
+  void launch_step_shape##i(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only, \
+                            hipStream_t s);
+SWE_DECLARE_SHAPE(0)
+SWE_DECLARE_SHAPE(1)
+SWE_DECLARE_SHAPE(2)
+SWE_DECLARE_SHAPE(3)
+SWE_DECLARE_SHAPE(4)
+#undef SWE_DECLARE_SHAPE
 void launch_spawn(const SpawnArgs& a, hipStream_t s);
 
 }  // namespace swe

@@ -1,8 +1,10 @@
 """Autotuning of the tile step path with a persistent cache.
 
 `autotune(target, particles)` runs the library's autotuner (swe_autotune:
-every tile order in {0, 1, 4, 16} with and without graphs, timed on the
-context's own workload from a saved state that is restored afterwards) and
+every compiled tile shape of the step kernel, then on the fastest every tile
+order in {0, 1, 4, 16} with and without graphs and every heavy-first mode,
+timed on the context's own workload from a saved state that is restored
+afterwards) and
 records the winner in a JSON cache keyed by GPU architecture, terrain sizes,
 particles per terrain and the parameters that shape the launches; a later
 call with the same key applies the cached setting without timing.  Settings
@@ -43,13 +45,14 @@ def _load(path):
 
 def autotune(target, particles, trial_cycles=2, cache=DEFAULT_CACHE, retune=False):
     """Tune a World or Batch for `particles` (an int, or one per terrain);
-    returns {"tile_order", "graph", "heavy_first", "ms_per_cycle", "candidates", "cached"}."""
+    returns {"shape", "tile_order", "graph", "heavy_first", "ms_per_cycle", "candidates", "cached"}."""
     if isinstance(particles, int):
         particles = [particles]
     key = _key(target, particles)
     table = _load(cache) if cache else {}
     if not retune and key in table:
         hit = table[key]
+        target.set_tune(capi.SWE_TUNE_SHAPE, hit.get("shape", 0))
         target.set_tune(capi.SWE_TUNE_TILE_ORDER, hit["tile_order"])
         target.set_tune(capi.SWE_TUNE_GRAPH, hit["graph"])
         target.set_tune(capi.SWE_TUNE_HEAVY_FIRST, hit.get("heavy_first", 1))
@@ -58,7 +61,7 @@ def autotune(target, particles, trial_cycles=2, cache=DEFAULT_CACHE, retune=Fals
     arr = (ctypes.c_int64 * len(particles))(*particles)
     res = capi.SweTuneResult()
     target._check(target.lib.swe_autotune(target.ctx, arr, int(trial_cycles), ctypes.byref(res)), "swe_autotune")
-    out = {"tile_order": res.tile_order, "graph": res.graph, "heavy_first": res.heavy_first,
+    out = {"shape": res.shape, "tile_order": res.tile_order, "graph": res.graph, "heavy_first": res.heavy_first,
            "ms_per_cycle": res.ms_per_cycle, "candidates": res.candidates}
     if cache:
         table = _load(cache)

@@ -101,7 +101,7 @@ def test_reference_defaults_at_reference_density_follow_the_sequential_run():
     p = O.reference_params(periodic=0, seed=1)
     assert p.settling == 1.0 and abs(p.suspension - 0.05) < 1e-9 and p.cascade_order == -1
     st = O.OracleState(n, n, h, params=p)
-    st.erode(npart, cycles)
+    st.erode(npart, cycles, nthreads=1)  # 144 particles: threads would only wait on each other
     got = O.unfx_vec(st.zm).reshape(n, n)
     s1, s2 = seq(1), seq(2)
     rms = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))  # noqa: E731
