@@ -216,6 +216,8 @@ def main():
                     help="CPU baseline on one whole cycle, plus the 256^2 CPU workload in full and the batched "
                          "oracle on all cores (several minutes)")
     ap.add_argument("--window", type=int, default=None)
+    ap.add_argument("--cascade-order", type=int, default=None,
+                    help="-1 automatic, 0 synchronous, 1 colour-ordered cascade (default: the workload's)")
     ap.add_argument("--track", type=int, default=None)
     ap.add_argument("--tile-order", type=int, default=None,
                     help="SWE_TUNE_TILE_ORDER for the step kernel (default: the library's)")
@@ -255,6 +257,8 @@ def main():
         over["window"] = args.window
     if args.track is not None:
         over["track"] = bool(args.track)
+    if args.cascade_order is not None:
+        over["cascade_order"] = args.cascade_order
     world = workloads.make_world(args.workload, seed=args.seed, **over)
     p = world.params()
     tuned = apply_tuning(world, args, [wl["particles"]])
@@ -317,6 +321,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "map": "%dx%d" % (world.nx, world.ny),
                    "particles_per_cycle": N, "window": R, "layers": int(p.layers), "track": int(p.track),
+                   "cascade": "colour-ordered" if (p.cascade_order == 1 or (p.cascade_order < 0 and p.settling > 0.25))
+                   else "synchronous",
                    "periodic": int(p.periodic), "parallelism": "single GPU"},
         "map_cycles_per_s": args.steps / elapsed,
         "mass_exact": None,

@@ -457,7 +457,7 @@ int erode_global(swe_ctx* c, int64_t particles, int32_t n_cycles, int64_t nwin, 
       ra.sm_out = layers2 ? c->smv[cur ^ 1] : nullptr;
       ra.ss_out = layers2 ? c->ss[cur ^ 1] : nullptr;
       if ((r = timed_begin(c, &e0, &e1))) return r;
-      if (colour_mode(c->p)) swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, c->stream);
+      if (colour_mode(c->p)) swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, true, c->stream);
       else swe::launch_relax(ra, c->p.periodic != 0, c->p.layers, c->stream);
       HIPCHK(c, hipGetLastError());
       if ((r = timed_end(c, e0, e1, 1))) return r;
@@ -561,6 +561,18 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
   sp.tile_sx = swe::kStepShape[c->shape].sx;
   sp.tile_sy = swe::kStepShape[c->shape].sy;
 
+  // colour-ordered cascade: every window's step kernel only moves the
+  // particles (MODE 2), and its relax follows at once as one launch per
+  // colour class; no closing relax-only window
+  const bool colour = colour_mode(c->p);
+  swe::RelaxArgs ra{};
+  ra.nx = c->nx;
+  ra.ny = c->ny;
+  ra.rcp_nx = 1.0f / (float)c->nx;
+  ra.rcp_ny = 1.0f / (float)c->ny;
+  ra.maxdiff = c->p.maxdiff;
+  ra.settling = c->p.settling;
+
   // the launches of one map-cycle (kernels only: no copies, fills or host
   // synchronisation, so that the sequence can be captured into a graph)
   auto enqueue_cycle = [&]() -> int {
@@ -576,7 +588,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
       swe::launch_spawn(sp, c->stream);
       HIPCHK(c, hipGetLastError());
       // windows 0 .. nwin-1 step the particles; window nwin only relaxes
-      for (int64_t w = 0; w <= nwin; w++) {
+      for (int64_t w = 0; w <= nwin - (colour ? 1 : 0); w++) {
         const int cur = c->cur(), nxt = cur ^ 1;
         const int wp = (int)(w & 1);
         sa.zm_in = c->zmv[cur];
@@ -618,9 +630,22 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
         int rr;
         if ((rr = timed_begin(c, &e0, &e1))) return rr;
         swe::launch_step(c->shape, sa, c->p.periodic != 0, c->p.track != 0, c->p.layers,
-                         c->p.deposit_on_death != 0, w == nwin, c->stream);
+                         c->p.deposit_on_death != 0, colour ? 2 : (w == nwin ? 1 : 0), c->stream);
         HIPCHK(c, hipGetLastError());
         if ((rr = timed_end(c, e0, e1, 2))) return rr;
+        if (colour) {  // the step kernel wrote the masters with the pickups and their snapshots
+          ra.V = c->fV[nxt];
+          ra.zm_out = c->zmv[nxt];
+          ra.zs_out = c->zs[nxt];
+          ra.sm_out = layers2 ? c->smv[nxt] : nullptr;
+          ra.ss_out = layers2 ? c->ss[nxt] : nullptr;
+          if ((rr = timed_begin(c, &e0, &e1))) return rr;
+          swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, false, c->stream);
+          // a cycle ends with no events pending, as after a relax-only window
+          if (w == nwin - 1) swe::launch_zero32(c->fV[nxt], c->ncell, c->stream);
+          HIPCHK(c, hipGetLastError());
+          if ((rr = timed_end(c, e0, e1, 1))) return rr;
+        }
         c->g++;
       }
     }
@@ -645,7 +670,7 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
     key.append((const char*)&nwin, sizeof(nwin));
     key.append((const char*)&c->alloc_gen, sizeof(c->alloc_gen));
   }
-  const uint64_t windows_per_cycle = total > 0 ? (uint64_t)nwin + 1 : 0;
+  const uint64_t windows_per_cycle = total > 0 ? (uint64_t)nwin + (colour ? 0 : 1) : 0;
 
   for (int32_t cyc = 0; cyc < n_cycles; cyc++) {
     if (graph) {
@@ -1024,9 +1049,8 @@ int swe_erode_batch(swe_ctx* c, const int64_t* particles, int32_t n_cycles) {
   int32_t R;
   const int64_t nwin = windows_of(c->p, &R);
   const bool colour = colour_mode(c->p);
-  if (colour && (c->path == 2 || c->nterr > 1))
-    return fail(c, SWE_EINVAL, "the colour-ordered cascade runs on the global path of single-map contexts");
-  const bool tiled = !colour && (c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1);
+  if (colour && c->nterr > 1) return fail(c, SWE_EINVAL, "the colour-ordered cascade runs on single-map contexts");
+  const bool tiled = c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1;
   if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path (and batches) need window == 1");
   if (tiled) return erode_tiled(c, particles, n_cycles, nwin);
   return erode_global(c, particles[0], n_cycles, nwin, R);

@@ -80,8 +80,11 @@ constexpr float kSettlingMaxSync = 0.25f;
 
 void launch_march(const MarchArgs& a, bool first, hipStream_t s);
 void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
-// the window's relax as masters += pickups, then one launch per colour class
-void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
+// the window's colour-ordered relax: with `begin`, masters += pickups and
+// their snapshots first (the global march leaves the pickups in dz / ds);
+// then one launch per colour class on zm_out / zs_out (sm_out / ss_out) with
+// the events V, and the sediment layer clamped at zero
+void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, bool begin, hipStream_t s);
 void launch_zero64(int64_t* p, int64_t n, hipStream_t s);
 void launch_zero32(uint32_t* p, int64_t n, hipStream_t s);
 void launch_from_f32(const float* in, int64_t* master, float* snap, int64_t n, hipStream_t s);

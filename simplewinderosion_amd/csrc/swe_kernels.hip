@@ -305,9 +305,27 @@ __global__ void k_colour_begin(RelaxArgs a, int64_t n) {
 // One colour class (cx, cy): every cell of the class with events runs its
 // cascade against the current map (zs_out / ss_out) and adds its transfers
 // to the masters at once.  Cells of one class are at least three apart, so
-// no two threads touch the same cell: plain read-modify-writes.
+// no two threads touch the same cell: plain stores.  All loads of a cell's
+// 3x3 block (snapshot and masters) are issued together and its nine
+// masters and snapshots written once, so a class costs two memory round
+// trips.  On maps under three cells along an axis the block's cells alias
+// (the periodic wrap reaches the centre or a neighbour twice): there the
+// transfers are added to memory one after another.
+template <bool PERIODIC>
+__device__ __forceinline__ int64_t colour_neighbour(int32_t x, int32_t y, int k, int32_t nx, int32_t ny, bool& ok) {
+  int32_t qx = x + c_offx[k], qy = y + c_offy[k];
+  ok = true;
+  if (PERIODIC) {
+    qx = qx < 0 ? qx + nx : (qx >= nx ? qx - nx : qx);
+    qy = qy < 0 ? qy + ny : (qy >= ny ? qy - ny : qy);
+  } else {
+    ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
+  }
+  return (int64_t)qx * ny + qy;
+}
+
 template <bool PERIODIC, int LAYERS>
-__global__ void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, int32_t nj) {
+__global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, int32_t nj) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)ni * nj) return;
   const int32_t nx = a.nx, ny = a.ny;
@@ -316,44 +334,80 @@ __global__ void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, i
   const int64_t c = (int64_t)x * ny + y;
   const uint32_t v = a.V[c];
   if (v == 0) return;
-  int64_t nb[8];
+  int64_t zn[8], snm[8];
   float hn[8], sn[8];
   uint32_t valid = 0;
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    int32_t qx = x + c_offx[k], qy = y + c_offy[k];
-    bool ok = true;
-    if (PERIODIC) {
-      qx = qx < 0 ? qx + nx : (qx >= nx ? qx - nx : qx);
-      qy = qy < 0 ? qy + ny : (qy >= ny ? qy - ny : qy);
-    } else {
-      ok = qx >= 0 && qy >= 0 && qx < nx && qy < ny;
-    }
-    nb[k] = ok ? (int64_t)qx * ny + qy : c;
-    hn[k] = ok ? a.zs_out[nb[k]] : 0.0f;
-    sn[k] = (LAYERS == 2 && ok) ? a.ss_out[nb[k]] : 0.0f;
+    bool ok;
+    const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+    hn[k] = ok ? a.zs_out[q] : 0.0f;
+    zn[k] = ok ? a.zm_out[q] : 0;
+    sn[k] = (LAYERS == 2 && ok) ? a.ss_out[q] : 0.0f;
+    snm[k] = (LAYERS == 2 && ok) ? a.sm_out[q] : 0;
     valid |= (ok ? 1u : 0u) << k;
   }
   const float hc = a.zs_out[c];
   const float sc = LAYERS == 2 ? a.ss_out[c] : 0.0f;
+  const int64_t zc = a.zm_out[c];
+  const int64_t scm = LAYERS == 2 ? a.sm_out[c] : 0;
+  const bool alias = nx < 3 || ny < 3;
+  int64_t d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int64_t centre = 0;
   const auto sed_of = [&](int32_t k) { return sn[k]; };
+  // the neighbour index k of a transfer is data-dependent: select into the
+  // eight accumulators (an indexed register array would live in scratch)
   cascade_cell_dev<LAYERS>(hn, sed_of, hc, sc, valid, settle_factor_dev(a.settling, v), a.maxdiff,
                            [&](int32_t k, int64_t amt) {
-                             a.zm_out[nb[k]] += amt;
-                             if (LAYERS == 2) a.sm_out[nb[k]] += amt;
+#pragma unroll
+                             for (int j = 0; j < 8; j++) d[j] += k == j ? amt : 0;
                              centre -= amt;
                            });
-  a.zm_out[c] += centre;
-  if (LAYERS == 2) a.sm_out[c] += centre;
+  if (alias) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      bool ok;
+      const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+      if (!ok || d[k] == 0) continue;
+      a.zm_out[q] += d[k];
+      if (LAYERS == 2) a.sm_out[q] += d[k];
+    }
+    a.zm_out[c] += centre;
+    if (LAYERS == 2) a.sm_out[c] += centre;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      bool ok;
+      const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+      if (!ok) continue;
+      a.zs_out[q] = f32_from_fx(a.zm_out[q]);
+      if (LAYERS == 2) a.ss_out[q] = f32_from_fx(a.sm_out[q]);
+    }
+    a.zs_out[c] = f32_from_fx(a.zm_out[c]);
+    if (LAYERS == 2) a.ss_out[c] = f32_from_fx(a.sm_out[c]);
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    if (!((valid >> k) & 1u)) continue;
-    a.zs_out[nb[k]] = f32_from_fx(a.zm_out[nb[k]]);
-    if (LAYERS == 2) a.ss_out[nb[k]] = f32_from_fx(a.sm_out[nb[k]]);
+    if (d[k] == 0) continue;
+    bool ok;
+    const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+    const int64_t z = zn[k] + d[k];
+    a.zm_out[q] = z;
+    a.zs_out[q] = f32_from_fx(z);
+    if (LAYERS == 2) {
+      const int64_t sm = snm[k] + d[k];
+      a.sm_out[q] = sm;
+      a.ss_out[q] = f32_from_fx(sm);
+    }
   }
-  a.zs_out[c] = f32_from_fx(a.zm_out[c]);
-  if (LAYERS == 2) a.ss_out[c] = f32_from_fx(a.sm_out[c]);
+  if (centre != 0) {
+    a.zm_out[c] = zc + centre;
+    a.zs_out[c] = f32_from_fx(zc + centre);
+    if (LAYERS == 2) {
+      a.sm_out[c] = scm + centre;
+      a.ss_out[c] = f32_from_fx(scm + centre);
+    }
+  }
 }
 
 // the sediment layer clamped at zero after the last class
@@ -544,10 +598,12 @@ void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s) 
   }
 }
 
-void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, hipStream_t s) {
+void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, bool begin, hipStream_t s) {
   const int64_t n = (int64_t)a.nx * a.ny;
-  if (layers == 2) hipLaunchKernelGGL(k_colour_begin<2>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
-  else hipLaunchKernelGGL(k_colour_begin<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+  if (begin) {
+    if (layers == 2) hipLaunchKernelGGL(k_colour_begin<2>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+    else hipLaunchKernelGGL(k_colour_begin<1>, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+  }
   const int32_t ncx = colour_classes(a.nx, periodic), ncy = colour_classes(a.ny, periodic);
   for (int32_t cx = 0; cx < ncx; cx++)
     for (int32_t cy = 0; cy < ncy; cy++) {

@@ -140,9 +140,9 @@ struct SpawnArgs {
 };
 
 void launch_step(int shape, const StepArgs& a, bool periodic, bool track, int layers, bool deposit,
-                 bool relax_only, hipStream_t s);
+                 int mode, hipStream_t s);
 #define SWE_DECLARE_SHAPE(i)                                                                                    \
-  void launch_step_shape##i(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only, \
+  void launch_step_shape##i(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, int mode, \
                             hipStream_t s);
 SWE_DECLARE_SHAPE(0)
 SWE_DECLARE_SHAPE(1)

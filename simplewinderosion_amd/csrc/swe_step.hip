@@ -96,12 +96,12 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-void launch_step(int shape, const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,
+void launch_step(int shape, const StepArgs& a, bool periodic, bool track, int layers, bool deposit, int mode,
                  hipStream_t s) {
   switch (shape) {
 #define SWE_SHAPE_CASE(i)                                                           \
   case i:                                                                           \
-    launch_step_shape##i(a, periodic, track, layers, deposit, relax_only, s); \
+    launch_step_shape##i(a, periodic, track, layers, deposit, mode, s); \
     break;
     SWE_SHAPE_CASE(0)
     SWE_SHAPE_CASE(1)

@@ -89,8 +89,14 @@ __device__ __forceinline__ int32_t wrapt(int32_t v, int32_t n) {
 
 // the register budget keeps WPE waves per SIMD (the default shape: five, and
 // with the LDS of one workgroup under 32 KiB five workgroups share a CU)
-template <int SX, int SY, int NT, int WPE, bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, bool RELAX_ONLY>
+// MODE 0: relax of the previous window (synchronous cascade) + one fly()
+// call per particle; MODE 1: the relax only (the cycle's last window);
+// MODE 2: the fly() calls only -- the previous window was relaxed before the
+// launch by the colour-ordered cascade (one launch per colour class), so the
+// tile reads the snapshot and masters as they are.
+template <int SX, int SY, int NT, int WPE, bool PERIODIC, bool TRACK, int LAYERS, bool DEPOSIT, int MODE>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_step(StepArgs a) {
+  constexpr bool RELAX_ONLY = MODE == 1, EXT = MODE == 2;
   constexpr int kSX = SX, kSY = SY;
   constexpr int kH3X = kSX + 6, kH3Y = kSY + 6;  // snapshot halo 3
   constexpr int kH2X = kSX + 4, kH2Y = kSY + 4;  // events halo 2
@@ -206,6 +212,89 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 
   PRec rec0{};       // this thread's first arrival (loaded in the relax phase)
   uint32_t sr0 = 0;  // its sub-range
+  if constexpr (EXT) {
+    // ---------------- 1'. the relaxed map of the previous window ----------------
+    // one-halo snapshot into LDS for the particles' stencils, the tile's
+    // masters into the pickup accumulators; all loads issued first
+    constexpr int kH1Per = (kH1X * kH1Y + kStepThreads - 1) / kStepThreads;
+    float hv[kH1Per], hs1[LAYERS == 2 ? kH1Per : 1];
+    int64_t zt[kCellPer], st[LAYERS == 2 ? kCellPer : 1];
+#pragma unroll
+    for (int i = 0; i < kH1Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      int32_t gx = x0 - 1 + e / kH1Y, gy = y0 - 1 + e % kH1Y;
+      bool ok = e < kH1X * kH1Y;
+      if (PERIODIC) {
+        gx = wrap(gx, nx, rcp_nx);
+        gy = wrap(gy, ny, rcp_ny);
+      } else {
+        ok = ok && gx >= 0 && gy >= 0 && gx < nx && gy < ny;
+      }
+      const int64_t g = (int64_t)gx * ny + gy;
+      hv[i] = ok ? snap_in[g] : 0.0f;
+      if (LAYERS == 2) hs1[i] = ok ? ssnap_in[g] : 0.0f;
+    }
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      const int32_t gx = x0 + e / kSY, gy = y0 + e % kSY;
+      const bool ok = e < kCell && gx < nx && gy < ny;
+      const int64_t g = (int64_t)gx * ny + gy;
+      zt[i] = ok ? zm_in[g] : 0;
+      if (LAYERS == 2) st[i] = ok ? sm_in[g] : 0;
+    }
+    if (threadIdx.x < 9) {
+      const int32_t d = threadIdx.x, dx = d / 3 - 1, dy = d % 3 - 1;
+      int32_t sx = tx - dx, sy = ty - dy;
+      bool ok = true;
+      if (PERIODIC) {
+        sx = wrapt(sx, tiles_x);
+        sy = wrapt(sy, tiles_y);
+      } else {
+        ok = sx >= 0 && sy >= 0 && sx < tiles_x && sy < tiles_y;
+      }
+      int64_t rb = 0;
+      uint32_t rc = 0;
+      if (ok) {
+        const int64_t s9 = (int64_t)(t0 + sx * tiles_y + sy) * 9 + d;
+        rb = a.sub_base_prev[s9];
+        rc = a.sub_cnt_prev[s9];
+      }
+      s_rbase[d] = rb;
+      s_slot[d] = rc;
+    }
+    int64_t* acc_z = (int64_t*)s_work;
+#pragma unroll
+    for (int i = 0; i < kH1Per; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      if (e < kH1X * kH1Y) {
+        s_new[e] = hv[i];
+        if (LAYERS == 2) s_snew[e] = hs1[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kCellPer; i++) {
+      const int e = threadIdx.x + i * kStepThreads;
+      if (e < kCell) {
+        acc_z[e] = zt[i];
+        if (LAYERS == 2) acc_z[kCell + e] = st[i];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t r = 0;
+      for (int d = 0; d < 9; d++) {
+        s_rpre[d] = r;
+        r += s_slot[d];
+      }
+      s_rpre[9] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < s_rpre[9]) {
+      while (sr0 < 8 && threadIdx.x >= s_rpre[sr0 + 1]) sr0++;
+      rec0 = a.in[s_rbase[sr0] + (int64_t)(threadIdx.x - s_rpre[sr0])];
+    }
+  } else {
   // ---------------- 1. relax of the previous window ----------------
   {
     float* s_old = (float*)s_work;
@@ -442,6 +531,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     }
     __syncthreads();
   }
+  }  // relax (MODE 0, 1)
   {
     int64_t part = (int64_t)part_c + (int64_t)part_n;
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
@@ -492,10 +582,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     // than scalar ones, which this kernel has too few of
     k.mapscale = in_vgpr(k.mapscale);
     k.boundarylayer = in_vgpr(k.boundarylayer);
-    k.rcp_bl = in_vgpr(k.rcp_bl);
-    k.suspension = in_vgpr(k.suspension);
     k.gravity = in_vgpr(k.gravity);
-    k.abrasion = in_vgpr(k.abrasion);
+    if (!EXT) {  // in the step-only mode these stay scalar: pinning them too spills vector registers there
+      k.rcp_bl = in_vgpr(k.rcp_bl);
+      k.suspension = in_vgpr(k.suspension);
+      k.abrasion = in_vgpr(k.abrasion);
+    }
     k.ps = mk(in_vgpr(k.ps.x), in_vgpr(k.ps.y), in_vgpr(k.ps.z));
     k.nps = mk(in_vgpr(k.nps.x), in_vgpr(k.nps.y), in_vgpr(k.nps.z));
     const uint64_t ck = cycle_key(T.seed, *a.cycle);
@@ -642,6 +734,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     zm_out[g] = acc_d[e];
     v_out[g] = acc_v[e];
     if (LAYERS == 2) sm_out[g] = acc_ds[e];
+    if (EXT) {  // the snapshot the colour-ordered relax starts from
+      snap_out[g] = f32_from_fx(acc_d[e]);
+      if (LAYERS == 2) ssnap_out[g] = f32_from_fx(acc_ds[e]);
+    }
   }
   // flow tracks of the cells this window's particles visited, read here
   // rather than prefetched (16 fewer vector registers through the particle
@@ -782,32 +878,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 template <int SX, int SY, int NT, int WPE>
 struct StepLauncher {
   template <bool P, bool T, int L, bool D>
-  static void step4(const StepArgs& a, bool relax_only, hipStream_t s) {
+  static void step4(const StepArgs& a, int mode, hipStream_t s) {
     const unsigned g = (unsigned)(a.heavy_slots + a.default_slots + a.light_slots);
-    if (relax_only)
-      hipLaunchKernelGGL((k_step<SX, SY, NT, WPE, P, T, L, D, true>), dim3(g), dim3(NT), 0, s, a);
+    if (mode == 1)
+      hipLaunchKernelGGL((k_step<SX, SY, NT, WPE, P, T, L, D, 1>), dim3(g), dim3(NT), 0, s, a);
+    else if (mode == 2)
+      hipLaunchKernelGGL((k_step<SX, SY, NT, WPE, P, T, L, D, 2>), dim3(g), dim3(NT), 0, s, a);
     else
-      hipLaunchKernelGGL((k_step<SX, SY, NT, WPE, P, T, L, D, false>), dim3(g), dim3(NT), 0, s, a);
+      hipLaunchKernelGGL((k_step<SX, SY, NT, WPE, P, T, L, D, 0>), dim3(g), dim3(NT), 0, s, a);
   }
   template <bool P, bool T, int L>
-  static void step3(const StepArgs& a, bool ro, bool dep, hipStream_t s) {
+  static void step3(const StepArgs& a, int ro, bool dep, hipStream_t s) {
     if (dep) step4<P, T, L, true>(a, ro, s);
     else step4<P, T, L, false>(a, ro, s);
   }
   template <bool P, bool T>
-  static void step2(const StepArgs& a, bool ro, bool dep, int layers, hipStream_t s) {
+  static void step2(const StepArgs& a, int ro, bool dep, int layers, hipStream_t s) {
     if (layers == 2) step3<P, T, 2>(a, ro, dep, s);
     else step3<P, T, 1>(a, ro, dep, s);
   }
   template <bool P>
-  static void step1(const StepArgs& a, bool ro, bool dep, bool trk, int layers, hipStream_t s) {
+  static void step1(const StepArgs& a, int ro, bool dep, bool trk, int layers, hipStream_t s) {
     if (trk) step2<P, true>(a, ro, dep, layers, s);
     else step2<P, false>(a, ro, dep, layers, s);
   }
-  static void launch(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, bool relax_only,
+  static void launch(const StepArgs& a, bool periodic, bool track, int layers, bool deposit, int mode,
                      hipStream_t s) {
-    if (periodic) step1<true>(a, relax_only, deposit, track, layers, s);
-    else step1<false>(a, relax_only, deposit, track, layers, s);
+    if (periodic) step1<true>(a, mode, deposit, track, layers, s);
+    else step1<false>(a, mode, deposit, track, layers, s);
   }
 };
 

@@ -12,9 +12,9 @@ namespace swe {
 #define SWE_CAT2(a, b) a##b
 #define SWE_CAT(a, b) SWE_CAT2(a, b)
 void SWE_CAT(launch_step_shape, SWE_STEP_SHAPE)(const StepArgs& a, bool periodic, bool track, int layers, bool deposit,
-                                                bool relax_only, hipStream_t s) {
+                                                int mode, hipStream_t s) {
   constexpr StepShape sh = kStepShape[SWE_STEP_SHAPE];
-  StepLauncher<sh.sx, sh.sy, sh.threads, sh.waves>::launch(a, periodic, track, layers, deposit, relax_only, s);
+  StepLauncher<sh.sx, sh.sy, sh.threads, sh.waves>::launch(a, periodic, track, layers, deposit, mode, s);
 }
 
 }  // namespace swe

@@ -24,12 +24,42 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("path", [1, 2])
 @pytest.mark.parametrize("name,nx,ny,particles,cycles,opts", CASES, ids=[c[0] for c in CASES])
-def test_colour_cascade_matches_oracle(statics, name, nx, ny, particles, cycles, opts):
-    say("case colour", name)
+def test_colour_cascade_matches_oracle(statics, name, nx, ny, particles, cycles, opts, path):
+    """Both execution paths: the global march with per-class launches, and
+    the tile step kernel moving the particles (window 1) with the same
+    per-class launches after each window."""
+    if path == 2 and opts.get("window", 1) != 1:
+        pytest.skip("the tile step path runs window 1")
+    say("case colour", name, "path", path)
     Wind.suspension = 0.01
-    w, st = run_pair(nx, ny, particles, cycles, seed=7, **opts)
+    w, st = run_pair(nx, ny, particles, cycles, seed=7, path=path, **opts)
     assert O.load().or_colour_mode(st.p) == 1
+    assert_same(w, st)
+    w.close()
+
+
+def test_colour_tile_path_crowded_and_mode_switch(statics):
+    """The benchmark's density (one particle per cell, crowded tiles) on the
+    tile path with the colour-ordered cascade, then synchronous cycles on
+    the same context: the oracle's bits throughout."""
+    say("case colour crowded + switch")
+    Wind.suspension = 1e-3
+    World.settling = 0.01
+    nx, ny = 96, 160
+    rng = np.random.default_rng(12)
+    h = (rng.random((nx, ny)) * 0.5).astype(np.float32)
+    sed = np.full((nx, ny), 0.01, np.float32)
+    w, st = run_pair(nx, ny, nx * ny, 2, seed=3, sed=sed, periodic=True, layers=2, abrasion=1e-3, window=1,
+                     cascade_order=1, path=2)
+    assert_same(w, st)
+    # two more cycles with the synchronous cascade on both sides
+    w.cascade_order = 0
+    st.p.cascade_order = 0
+    w.erode(nx * ny, n_cycles=2)
+    w.synchronize()
+    st.erode(nx * ny, 2)
     assert_same(w, st)
     w.close()
 
@@ -56,13 +86,8 @@ def test_synchronous_cascade_refuses_large_settling(statics):
     World.settling = 1.0
     with pytest.raises(SweError):
         World(16, 16, cascade_order=0)
-    # batches and the tile path need the synchronous cascade
+    # batches need the synchronous cascade
     b = Batch([(32, 32), (16, 16)], periodic=True)
     with pytest.raises(SweError):
         b.erode([100, 50])
     b.close()
-    w = World(32, 32, periodic=True)
-    w.set_path(2)
-    with pytest.raises(SweError):
-        w.erode(100)
-    w.close()

@@ -63,7 +63,7 @@ def test_step_kernel_budget(metadata, shape):
     LDS for the workgroups those waves make per CU."""
     sx, sy, threads, waves = SHAPES[shape]
     steps = {k: v for k, v in metadata[shape].items() if "k_step" in k}
-    assert len(steps) == 32  # periodic x track x layers x deposit x relax-only
+    assert len(steps) == 48  # periodic x track x layers x deposit x mode (step + relax, relax only, step only)
     vgpr_max = (512 // waves) // 8 * 8
     groups_per_cu = 4 * waves // (threads // 64)
     for name, f in steps.items():
