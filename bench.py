@@ -171,6 +171,64 @@ def apply_tuning(target, args, particles):
     return out
 
 
+def bench_surrogate(args):
+    """The learned surrogate end to end on one GPU: training pairs generated
+    by the plain method (16 maps of 256^2, random winds, K = 50), a short
+    training, then a rollout of K = 50 blocks on the 1024^2 map with its 8
+    obstacles, the plain method judging every 10th block (blocks within 1.5x
+    the plain method's own run-to-run RMS accepted) and taking over after a
+    rejection.  value = map-cycles/s of the whole rollout (surrogate blocks,
+    judges and fallback blocks); the plain method's rate beside it."""
+    import math
+
+    import torch
+
+    from simplewinderosion_amd import surrogate as S
+    from simplewinderosion_amd import workloads
+
+    def log(msg):
+        print("bench: " + msg, file=sys.stderr, flush=True)
+
+    K = 50
+    states, az, t_gen = S.generate_states(args.sur_terrains, args.sur_cycles, seed=args.seed, log=log)
+    model = S.Surrogate(seed=args.seed)
+    S.fit_scales(model, states, K)
+    curve, t_train = S.train(model, states, az, K, args.sur_steps, batch=8, lr=1e-3, log=log, log_every=250)
+    fps = 3 * S.Surrogate.flops_forward(8, 256, 256)
+    del states
+    torch.cuda.empty_cache()
+    world = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
+    plain = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
+    ps = workloads.WORKLOADS["surr1024"]["params"]["pspeed"]
+    n = workloads.WORKLOADS["surr1024"]["particles"]
+    rec = S.rollout(model, world, plain, n, args.sur_blocks, math.atan2(ps[2], ps[0]), judge_every=10,
+                    tol_rms=1e-3, tol_noise=1.5, log=log)
+    world.close()
+    plain.close()
+    rate = args.sur_blocks * K / rec["t_total_s"]
+    plain_rate = K / rec["plain_seconds_per_block"]
+    judged = rec["judged"]
+    line = {"metric": "map-cycles/s", "value": rate, "unit": "map-cycles/s", "n_gpus": 1,
+            "steps": args.sur_blocks, "warmup": 0, "ms_per_step": 1e3 * rec["t_total_s"] / args.sur_blocks,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16 MFMA (fp32 accumulation); plain method f32 with int64 fixed point",
+            "data": "synthetic", "config": {"workload": "surrogate rollout 1024x1024 + 8 pyramids", "K": K,
+                                            "blocks": args.sur_blocks, "judge_every": 10,
+                                            "parallelism": "single GPU"},
+            "plain_map_cycles_per_s": plain_rate, "speedup_vs_plain": rate / plain_rate,
+            "blocks": {"surrogate": rec["surrogate_blocks"], "plain_fallback": rec["plain_blocks"],
+                       "judged": rec["judge_blocks"],
+                       "accepted": sum(1 for j in judged if j["accepted"])},
+            "judged_rms": [round(j["rms_err"], 6) for j in judged],
+            "no_change_rms": [round(j["rms_change_plain"], 6) for j in judged],
+            "noise_floor_rms": rec["noise_floor_rms"], "tolerance_rms": rec.get("tol_rms_effective"),
+            "mass_drift_fx": rec["mass_drift_fx"], "mass_exact": rec["mass_exact"],
+            "training": {"maps": args.sur_terrains, "cycles_per_map": args.sur_cycles, "steps": args.sur_steps,
+                         "batch": 8, "final_loss": curve[-1][1], "tflops": fps * args.sur_steps / t_train / 1e12,
+                         "bf16_dense_peak_tflops": 2500.0, "data_erode_seconds": t_gen}}
+    print(json.dumps(line), flush=True)
+
+
 def bench_batch(args):
     """The multi-terrain workload: 64 terrains (sides log-uniform in
     128..1024, own seed, wind azimuth uniform at 45 degrees elevation,
@@ -230,6 +288,10 @@ def main():
                     help="SWE_TUNE_SHAPE: the step kernel's compiled tile shape (0 = 16x32 cells, the default)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the library's autotuner (shapes, tile orders, graphs, heavy-first) before timing")
+    ap.add_argument("--sur-terrains", type=int, default=16, help="surrogate: training maps")
+    ap.add_argument("--sur-cycles", type=int, default=160, help="surrogate: plain cycles per training map")
+    ap.add_argument("--sur-steps", type=int, default=1500, help="surrogate: Adam steps")
+    ap.add_argument("--sur-blocks", type=int, default=20, help="surrogate: rollout blocks of K = 50 cycles")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -247,6 +309,9 @@ def main():
 
     if args.workload == "batch64":
         bench_batch(args)
+        return
+    if args.workload == "surrogate":
+        bench_surrogate(args)
         return
 
     from simplewinderosion_amd import workloads

@@ -149,3 +149,69 @@ def test_trained_network_beats_no_change(statics):
     base = float((true ** 2).mean().sqrt())
     print("held-out RMS error %.3g, no-change %.3g" % (err, base), flush=True)
     assert err < 0.9 * base
+
+
+def test_config_d_pairs_training_and_gradients_at_256(statics):
+    """The training configuration's shapes: pairs (state, state after K = 50
+    cycles) generated by the plain method on 256x256 maps of the training
+    workload (one particle per cell, random winds, obstacles on every other
+    map); Adam steps on batches of them at 256x256; and the hand-written
+    backward against float32 autograd at 256x256, every layer at its real
+    shape."""
+    print("case surrogate config D", flush=True)
+    states, az, _ = S.generate_states(2, 56, seed=5)
+    assert states.shape == (57, 2, 2, 256, 256)
+    K = 50
+    m = S.Surrogate(seed=6)
+    S.fit_scales(m, states, K)
+    assert m.K == K and m.d_scale > 0
+    curve, _ = S.train(m, states, az, K, steps=12, batch=4, lr=1e-3, log_every=1)
+    losses = [v for _, v in curve]
+    assert all(np.isfinite(losses)) and min(losses[-4:]) < losses[0], losses
+    # gradient check at 256x256
+    x0, x1 = states[:1, 0], states[K:K + 1, 0]
+    h, s = x0[:, 0].contiguous(), x0[:, 1].contiguous()
+    dh, ds = (x1[:, 0] - x0[:, 0]).contiguous(), (x1[:, 1] - x0[:, 1]).contiguous()
+    feats = m.features(h, s, [float(az[0])])
+    out, A = m.forward(feats)
+    ref, W, B = forward_ref(m, feats)
+    assert torch.allclose(out, ref, atol=0.05 * float(ref.abs().max()) + 1e-3)
+    N, H, Wd, C = out.shape
+    target = torch.stack((dh / m.d_scale, ds / m.ds_scale), dim=-1).reshape(-1, 2).contiguous()
+    grad = torch.empty_like(out)
+    part = torch.empty(((N * H * Wd * C + 255) // 256,), device="cuda")
+    S.nn.mse(out.view(-1, C), target, grad.view(-1, C), part)
+    m.backward(A, grad)
+    torch.cuda.synchronize()
+    ((ref[..., :2].reshape(-1, 2) - target) ** 2).mean().backward()
+    for name in m.gw:
+        a, b = m.gw[name], W[name].grad
+        if name == "enc0a":
+            a, b = a[:, :, :S.N_IN], b[:, :, :S.N_IN]
+        a, b = a.flatten(), b.flatten()
+        cos = float(torch.dot(a, b) / (a.norm() * b.norm() + 1e-30))
+        assert cos > 0.98, (name, cos)
+
+
+def test_config_e_rollout_blocks_at_1024(statics):
+    """The rollout configuration's shapes: K = 50 blocks on the 1024x1024 map
+    with its 8 obstacles, the plain method judging every 2nd block (with a
+    second plain run for the noise floor) and taking over after a rejected
+    judgement; the map's mass stays exact (drift 0 in fixed point)."""
+    print("case surrogate config E", flush=True)
+    m = S.Surrogate(seed=7)
+    m.K = 50
+    m.h_scale, m.s_scale, m.g_scale, m.l_scale, m.d_scale, m.ds_scale = 0.2, 0.05, 1e-3, 1e-3, 0.02, 0.02
+    world = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
+    plain = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
+    rec = S.rollout(m, world, plain, 1_000_000, blocks=3, azimuth=0.785, judge_every=2, tol_rms=-1.0)
+    assert rec["K"] == 50 and rec["judge_blocks"] == 1
+    assert rec["surrogate_blocks"] == 1 and rec["plain_blocks"] == 1  # rejected judge: fallback
+    j = rec["judged"][0]
+    assert not j["accepted"] and j["rms_change_plain"] > 0 and rec["noise_floor_rms"] > 0
+    assert rec["mass_exact"] and rec["mass_drift_fx"] == 0
+    assert world.cycle == 150
+    _, sm = world.master()
+    assert sm.min() >= 0
+    world.close()
+    plain.close()
