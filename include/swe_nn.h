@@ -16,9 +16,6 @@
 extern "C" {
 #endif
 
-/* pixels per weight-gradient partial sum */
-#define kSweNnChunk 8192
-
 /* y = conv3x3(x, w) + bias, optional ReLU; y16 (bf16) and/or y32 (float).
  * Cin % 8 == 0; BF16 MFMA with FP32 accumulation. */
 int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
@@ -29,7 +26,8 @@ int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream
 /* floats of work memory swe_nn_wgrad needs */
 int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
 /* dw = sum_p dy[p] x[p + t]^T (float [Cout][9][Cin]); db = sum_p dy[p]
- * (float [Cout], optional); deterministic. */
+ * (float [Cout], optional); deterministic (per-chunk partials in `work`,
+ * added in chunk order). */
 int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                  float* dw, float* db, float* work, int64_t work_len, void* stream);
 /* g16 = bf16(g32 * (y > 0)); y may be NULL (conversion only) */

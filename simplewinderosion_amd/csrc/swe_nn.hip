@@ -25,70 +25,126 @@ __device__ __forceinline__ bf16 f2bf(float v) { return (bf16)v; }  // round to n
 __device__ __forceinline__ int32_t wrapi(int32_t v, int32_t n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution, implicit GEMM: rows = pixels, columns = output channels,
-// reduction k = tap * Cin + ci (Cin % 8 == 0, so a lane's 8 consecutive k
-// stay within one tap).  A workgroup (4 waves) takes 64 consecutive pixels
-// of one image row; wave w owns pixels 16w .. 16w+15 and 16*NACC output
-// channels starting at blockIdx.y * 16 * NACC.  Per 16x16x32 MFMA, lane l
-// supplies A row l%16 (a pixel) and B column l%16 (an output channel), both
-// at k = 8*(l/16) .. +7, and receives D rows 4*(l/16) .. +3 of column l%16.
+// 3x3 convolution, implicit GEMM: rows = pixels (the batch's pixels
+// flattened, n-major), columns = output channels, reduction k = tap * Cin +
+// ci in steps of 32 (one tap, 32 input channels; channels at or beyond Cin
+// read as zero, so any Cin % 8 == 0 works).  A workgroup (4 waves) owns 128
+// pixels x NT output channels.  Each step's A block [128][32] (the pixels
+// shifted by the tap, periodic) and B block [NT][32] are staged in LDS,
+// double buffered with rows padded to 40 elements: every thread loads its
+// 16-byte runs of the next step into registers while the waves run the
+// current step's MFMAs, then stores them into the other buffer (one barrier
+// per step).  Waves tile the block 2 x 2 (NT >= 64) or 4 x 1; per
+// v_mfma_f32_16x16x32_bf16, lane l supplies A row l%16 and B row l%16, both
+// at k = 8*(l/16) .. +7, and holds D rows 4*(l/16) .. +3 of column l%16.
 // ---------------------------------------------------------------------------
-template <int NACC, int ACT>
+template <int NT, int ACT>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, const bf16* __restrict__ w,
                                                  const float* __restrict__ bias, bf16* __restrict__ y16,
-                                                 float* __restrict__ y32, int32_t H, int32_t W, int32_t Cin,
-                                                 int32_t Cout) {
-  const int32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int32_t segs = (W + 63) / 64;
-  const int32_t seg = blockIdx.x % segs;
-  const int32_t row = (blockIdx.x / segs) % H;
-  const int32_t img = blockIdx.x / (segs * H);
-  const int32_t co0 = blockIdx.y * 16 * NACC;
-  const int32_t px = seg * 64 + wave * 16 + (lane & 15);  // this lane's A row (pixel)
-  const int32_t kq = (lane >> 4) * 8;                     // this lane's k offset in a step
-  const int32_t K = 9 * Cin;
-  const int32_t pxc = px < W ? px : W - 1;                // a valid pixel for the loads
-  const bf16x8 zero = {};
+                                                 float* __restrict__ y32, int32_t N, int32_t H, int32_t W,
+                                                 int32_t Cin, int32_t Cout) {
+  constexpr int WM = NT >= 64 ? 2 : 4;  // waves along the pixels
+  constexpr int WN = 4 / WM;            // ... and the channels
+  constexpr int TM = 128 / WM, TN = NT / WN;
+  constexpr int BM = TM / 16, BN = TN / 16;
+  constexpr int LD = 40;
+  constexpr int BROWS = NT < 64 ? NT : 64;  // B rows a pass of 64 threads' row index covers
+  __shared__ __attribute__((aligned(16))) bf16 sA[2][128 * LD];
+  __shared__ __attribute__((aligned(16))) bf16 sB[2][NT * LD];
+  const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t wm = wave / WN, wn = wave % WN;
+  const int64_t P = (int64_t)N * H * W;
+  const int64_t p0 = (int64_t)blockIdx.x * 128;
+  const int32_t co0 = blockIdx.y * NT;
+  const int32_t cchunks = (Cin + 31) / 32, ksteps = 9 * cchunks;
+  const int32_t g = tid & 3, r0 = tid >> 2;  // loader: row r0 (+64), channel group g
 
-  f32x4 acc[NACC];
+  // the two A rows of this thread: image, row, column
+  int32_t ai[2], ay[2], ax[2];
+  bool aok[2];
 #pragma unroll
-  for (int j = 0; j < NACC; j++) acc[j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
-
-  for (int32_t k0 = 0; k0 < K; k0 += 32) {
-    const int32_t k = k0 + kq;
-    bf16x8 a = zero;
-    int32_t tap = 0, ci = 0;
-    const bool kin = k < K;
-    if (kin) {
-      tap = k / Cin;
-      ci = k - tap * Cin;
-      const int32_t yy = wrapi(row + tap / 3 - 1, H), xx = wrapi(pxc + tap % 3 - 1, W);
-      a = *(const bf16x8*)(x + (((int64_t)img * H + yy) * W + xx) * Cin + ci);
-    }
-#pragma unroll
-    for (int j = 0; j < NACC; j++) {
-      const int32_t co = co0 + 16 * j + (lane & 15);
-      bf16x8 b = zero;
-      if (kin && co < Cout) b = *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * Cin + ci);
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
-    }
+  for (int h = 0; h < 2; h++) {
+    const int64_t p = p0 + r0 + 64 * h;
+    aok[h] = p < P;
+    const int64_t pc = aok[h] ? p : 0;
+    ax[h] = (int32_t)(pc % W);
+    ay[h] = (int32_t)((pc / W) % H);
+    ai[h] = (int32_t)(pc / ((int64_t)W * H));
   }
-  // D rows 4*(lane/16) + i of this wave's 16 pixels, column lane%16
+  const bf16x8 zero = {};
+  bf16x8 ra[2], rb[2];
+  auto load = [&](int32_t s) __attribute__((always_inline)) {
+    const int32_t tap = s / cchunks, ci = (s - tap * cchunks) * 32 + 8 * g;
+    const int32_t dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const bool cok = ci < Cin;
 #pragma unroll
-  for (int j = 0; j < NACC; j++) {
-    const int32_t co = co0 + 16 * j + (lane & 15);
-    if (co >= Cout) continue;
-    const float b = bias ? bias[co] : 0.0f;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int32_t p = seg * 64 + wave * 16 + (lane >> 4) * 4 + i;
-      if (p >= W) continue;
-      float v = acc[j][i] + b;
-      if (ACT == 1) v = v > 0.0f ? v : 0.0f;
-      const int64_t o = (((int64_t)img * H + row) * W + p) * Cout + co;
-      if (y16) y16[o] = f2bf(v);
-      if (y32) y32[o] = v;
+    for (int h = 0; h < 2; h++) {
+      ra[h] = zero;
+      if (aok[h] && cok) {
+        const int32_t yy = wrapi(ay[h] + dy, H), xx = wrapi(ax[h] + dx, W);
+        ra[h] = *(const bf16x8*)(x + (((int64_t)ai[h] * H + yy) * W + xx) * Cin + ci);
+      }
     }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      rb[h] = zero;
+      const int32_t row = r0 + 64 * h;
+      const int32_t co = co0 + row;
+      if (row < NT && (h == 0 || NT > 64) && co < Cout && cok) rb[h] = *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * Cin + ci);
+    }
+  };
+  auto store = [&](int32_t buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) *(bf16x8*)&sA[buf][(r0 + 64 * h) * LD + 8 * g] = ra[h];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int32_t row = r0 + 64 * h;
+      if (row < NT && (h == 0 || NT > 64)) *(bf16x8*)&sB[buf][row * LD + 8 * g] = rb[h];
+    }
+  };
+  (void)BROWS;
+
+  f32x4 acc[BM][BN];
+#pragma unroll
+  for (int i = 0; i < BM; i++)
+#pragma unroll
+    for (int j = 0; j < BN; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+
+  load(0);
+  store(0);
+  __syncthreads();
+  const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
+  for (int32_t s = 0; s < ksteps; s++) {
+    const int32_t buf = s & 1;
+    if (s + 1 < ksteps) load(s + 1);
+    bf16x8 a[BM], b[BN];
+#pragma unroll
+    for (int i = 0; i < BM; i++) a[i] = *(const bf16x8*)&sA[buf][(wm * TM + 16 * i + rl) * LD + kq];
+#pragma unroll
+    for (int j = 0; j < BN; j++) b[j] = *(const bf16x8*)&sB[buf][(wn * TN + 16 * j + rl) * LD + kq];
+#pragma unroll
+    for (int i = 0; i < BM; i++)
+#pragma unroll
+      for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (s + 1 < ksteps) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < BN; j++) {
+    const int32_t co = co0 + wn * TN + 16 * j + rl;
+    if (co >= Cout) continue;
+    const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < BM; i++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int64_t p = p0 + wm * TM + 16 * i + 4 * (lane >> 4) + e;
+        if (p >= P) continue;
+        float v = acc[i][j][e] + bv;
+        if (ACT == 1) v = v > 0.0f ? v : 0.0f;
+        if (y16) y16[p * Cout + co] = f2bf(v);
+        if (y32) y32[p * Cout + co] = v;
+      }
   }
 }
 
@@ -102,110 +158,137 @@ __global__ void k_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int32_
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[co][t][ci] = sum_p dY[p][co] * X[p shifted by t][ci].
-// Rows = output channels, columns = input channels, reduction over pixels.
-// Workgroup (4 waves) = one tap, a 32 (co) x 32 (ci) block, and a chunk of
-// `chunk` pixels of the flattened batch.  Per step of 32 pixels, half the
-// threads load dY and half the shifted X as 16-byte runs of 8 channels
-// (coalesced NHWC rows) and store them channel-major into LDS; each wave
-// then reads its MFMA operands -- 8 consecutive pixels of one channel -- as
-// 16-byte LDS reads and accumulates its 16x16 sub-block (w / 2, w % 2).
-// Partial sums per chunk go to `part`, reduced by k_reduce_parts
-// (deterministic order).  Cin % 8 == 0 and Cout % 8 == 0.
+// Weight and bias gradients: dW[co][t][ci] = sum_p dY[p][co] X[p + s_t][ci],
+// db[co] = sum_p dY[p][co].  Rows = output channels, columns = input
+// channels of one tap, reduction over pixels.  A workgroup (4 waves) owns a
+// 64 (co) x 64 (ci) block of one tap and a chunk of the flattened batch's
+// pixels; per step of 32 pixels every thread loads one 16-byte run of dY
+// (8 channels of a pixel) and one of the tap-shifted X, and stores them
+// channel-major into LDS (double buffered: the next step's runs are loaded
+// while the waves run the current MFMAs); each wave accumulates a 32 x 32
+// sub-block.  The workgroups of tap 4 (no shift) and the first ci block
+// also sum their dY runs for the bias.  Each chunk writes its partial
+// block to `part` ([chunk][Cout * 9 * Cin + Cout]); k_reduce_parts adds the
+// chunks in order, so the result is deterministic.  Cin % 8 == Cout % 8 == 0.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dy,
                                                float* __restrict__ part, int32_t N, int32_t H, int32_t W,
                                                int32_t Cin, int32_t Cout, int32_t chunk) {
-  __shared__ __attribute__((aligned(16))) bf16 s_dy[32][40];  // [co][pixel], rows padded against bank conflicts
-  __shared__ __attribute__((aligned(16))) bf16 s_x[32][40];   // [ci][pixel]
+  constexpr int LD = 40;
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[2][64 * LD];  // [co][pixel]
+  __shared__ __attribute__((aligned(16))) bf16 s_x[2][64 * LD];   // [ci][pixel]
+  __shared__ float s_b[32][65];                                    // bias partials [pixel lane][co]
   const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int32_t cib = (Cin + 31) / 32;
-  int32_t b = blockIdx.y;
-  const int32_t t = b % 9;
-  b /= 9;
-  const int32_t ciB = (b % cib) * 32, coB = (b / cib) * 32;
-  const int32_t npix = N * H * W;  // < 2^31 (checked by the launcher)
-  const int32_t p0 = (int32_t)blockIdx.x * chunk;
-  const int32_t pend = p0 + chunk < npix ? p0 + chunk : npix;
-  const int32_t dyo = t / 3 - 1, dxo = t % 3 - 1;
-  // loader role: threads 0..127 load dY, 128..255 load X; (pixel q, group g)
-  const bool ldy = tid < 128;
-  const int32_t q = (tid & 127) >> 2, g = tid & 3;
-  const int32_t cl = (ldy ? coB : ciB) + 8 * g;  // first channel of this thread's run
-  const bool cl_ok = cl < (ldy ? Cout : Cin);
-  // consumer role
-  const int32_t co_l = (wave >> 1) * 16 + (lane & 15), ci_l = (wave & 1) * 16 + (lane & 15);
-  const int32_t kq = (lane >> 4) * 8;
-  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-  for (int32_t q0 = p0; q0 < pend; q0 += 32) {
-    const int32_t p = q0 + q;
-    bf16x8 v = {};
-    if (p < pend && cl_ok) {
-      if (ldy) {
-        v = *(const bf16x8*)(dy + (int64_t)p * Cout + cl);
-      } else {
-        const int32_t xx = p % W, r = p / W;
-        const int32_t yy = r % H, im = r / H;
-        const int32_t sy = wrapi(yy + dyo, H), sx = wrapi(xx + dxo, W);
-        v = *(const bf16x8*)(x + (((int64_t)im * H + sy) * W + sx) * Cin + cl);
-      }
-    }
-    bf16(*dst)[40] = ldy ? s_dy : s_x;
-#pragma unroll
-    for (int e = 0; e < 8; e++) dst[8 * g + e][q] = v[e];
-    __syncthreads();
-    const bf16x8 a = *(const bf16x8*)&s_dy[co_l][kq];
-    const bf16x8 bb = *(const bf16x8*)&s_x[ci_l][kq];
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, acc, 0, 0, 0);
-    __syncthreads();
-  }
-  // D rows = coB + 16*(wave/2) + 4*(lane/16) + i, column = ciB + 16*(wave%2) + lane%16
-  const int32_t ci = ciB + ci_l;
-  if (ci >= Cin) return;
-  const int64_t nw = (int64_t)Cout * 9 * Cin;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int32_t c = coB + (wave >> 1) * 16 + (lane >> 4) * 4 + i;
-    if (c < Cout) part[(int64_t)blockIdx.x * nw + ((int64_t)c * 9 + t) * Cin + ci] = acc[i];
-  }
-}
-
-__global__ void k_reduce_parts(const float* __restrict__ part, float* __restrict__ out, int64_t n, int32_t nparts) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.0f;
-  for (int32_t k = 0; k < nparts; k++) s += part[(int64_t)k * n + i];
-  out[i] = s;
-}
-
-// bias gradient partials: part[chunk][c] = sum over the chunk's pixels;
-// the 256 threads form 256 / C rows of C channels, each row summing every
-// (256 / C)-th pixel, then the rows are added in LDS (fixed order)
-__global__ __launch_bounds__(256) void k_bgrad(const bf16* __restrict__ dy, float* __restrict__ part, int64_t npix,
-                                               int32_t C, int32_t chunk) {
-  __shared__ float s_sum[256];
+  const int32_t cit = (Cin + 63) / 64;
+  int32_t bq = blockIdx.y;
+  const int32_t t = bq % 9;
+  bq /= 9;
+  const int32_t ciB = (bq % cit) * 64, coB = (bq / cit) * 64;
+  const bool do_bias = t == 4 && ciB == 0;
+  const int64_t npix = (int64_t)N * H * W;
   const int64_t p0 = (int64_t)blockIdx.x * chunk;
   const int64_t pend = p0 + chunk < npix ? p0 + chunk : npix;
-  if (C > 256) {  // one thread per channel, several channels per thread
-    for (int32_t c = threadIdx.x; c < C; c += 256) {
-      float s = 0.0f;
-      for (int64_t p = p0; p < pend; p++) s += bf2f(dy[p * C + c]);
-      part[(int64_t)blockIdx.x * C + c] = s;
+  const int32_t dyo = t / 3 - 1, dxo = t % 3 - 1;
+  // loader: pixel q of the step, channel group gg (8 channels)
+  const int32_t q = tid >> 3, gg = tid & 7;
+  const int32_t co_ld = coB + 8 * gg, ci_ld = ciB + 8 * gg;
+  const bool co_ok = co_ld < Cout, ci_ok = ci_ld < Cin;
+  const bf16x8 zero = {};
+  bf16x8 rdy, rx;
+  float bsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto load = [&](int64_t q0) __attribute__((always_inline)) {
+    const int64_t p = q0 + q;
+    rdy = zero;
+    rx = zero;
+    if (p < pend) {
+      if (co_ok) rdy = *(const bf16x8*)(dy + p * Cout + co_ld);
+      if (ci_ok) {
+        const int32_t xx = (int32_t)(p % W);
+        const int64_t r = p / W;
+        const int32_t yy = (int32_t)(r % H);
+        const int64_t im = r / H;
+        const int32_t sy = wrapi(yy + dyo, H), sx = wrapi(xx + dxo, W);
+        rx = *(const bf16x8*)(x + ((im * H + sy) * W + sx) * Cin + ci_ld);
+      }
     }
-    return;
-  }
-  const int32_t rows = 256 / C;
-  const int32_t c = threadIdx.x % C, r = threadIdx.x / C;
-  float s = 0.0f;
-  if (r < rows)
-    for (int64_t p = p0 + r; p < pend; p += rows) s += bf2f(dy[p * C + c]);
-  s_sum[threadIdx.x] = s;
+  };
+  auto store = [&](int32_t buf) __attribute__((always_inline)) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      s_dy[buf][(8 * gg + e) * LD + q] = rdy[e];
+      s_x[buf][(8 * gg + e) * LD + q] = rx[e];
+    }
+    if (do_bias) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) bsum[e] += bf2f(rdy[e]);
+    }
+  };
+  // consumer: wave (wave / 2, wave % 2) owns co rows 32 * (wave / 2) .., ci 32 * (wave % 2) ..
+  const int32_t wr = 32 * (wave >> 1), wc = 32 * (wave & 1);
+  const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+  load(p0);
+  store(0);
   __syncthreads();
-  if (r == 0) {
-    float tot = 0.0f;
-    for (int32_t k = 0; k < rows; k++) tot += s_sum[k * C + c];
-    part[(int64_t)blockIdx.x * C + c] = tot;
+  int32_t buf = 0;
+  for (int64_t q0 = p0; q0 < pend; q0 += 32) {
+    const bool more = q0 + 32 < pend;
+    if (more) load(q0 + 32);
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) a[i] = *(const bf16x8*)&s_dy[buf][(wr + 16 * i + rl) * LD + kq];
+#pragma unroll
+    for (int j = 0; j < 2; j++) b[j] = *(const bf16x8*)&s_x[buf][(wc + 16 * j + rl) * LD + kq];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
+  const int64_t nw = (int64_t)Cout * 9 * Cin;
+  float* out = part + (int64_t)blockIdx.x * (nw + Cout);
+  // D rows (co) = wr + 16 i + 4 (lane / 16) + e, column (ci) = wc + 16 j + lane % 16
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const int32_t ci = ciB + wc + 16 * j + rl;
+    if (ci >= Cin) continue;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int32_t co = coB + wr + 16 * i + 4 * (lane >> 4) + e;
+        if (co < Cout) out[((int64_t)co * 9 + t) * Cin + ci] = acc[i][j][e];
+      }
+  }
+  if (do_bias) {  // the 32 pixel lanes' sums per channel, added in a fixed order
+#pragma unroll
+    for (int e = 0; e < 8; e++) s_b[q][8 * gg + e] = bsum[e];
+    __syncthreads();
+    if (tid < 64) {
+      float sb = 0.0f;
+      for (int k = 0; k < 32; k++) sb += s_b[k][tid];
+      if (coB + tid < Cout) out[nw + coB + tid] = sb;
+    }
+  }
+}
+
+// dw[i] (i < nw) and db[i - nw] (nw <= i < nw + Cout, when db is given):
+// the chunks' partials added in chunk order
+__global__ void k_reduce_parts(const float* __restrict__ part, float* __restrict__ dw, float* __restrict__ db,
+                               int64_t nw, int32_t Cout, int32_t nparts) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t stride = nw + Cout;
+  if (i >= (db ? stride : nw)) return;
+  float s = 0.0f;
+  for (int32_t k = 0; k < nparts; k++) s += part[(int64_t)k * stride + i];
+  if (i < nw) dw[i] = s;
+  else db[i - nw] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -419,20 +502,26 @@ extern "C" {
 int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
                    int32_t Cout, int32_t relu, void* y16, float* y32, void* stream) {
   if (!x || !w || N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (!y16 && !y32)) return -1;
-  const int32_t nacc = Cout >= 64 ? 4 : (Cout >= 32 ? 2 : 1);
-  const dim3 grid((unsigned)(((W + 63) / 64) * H * N), (unsigned)((Cout + 16 * nacc - 1) / (16 * nacc)));
+  const int64_t P = (int64_t)N * H * W;
+  const int64_t mt = (P + 127) / 128;
+  if (mt > 0x7fffffffLL) return -1;
+  // output channels per workgroup: the layer's (up to 64), 128 where that
+  // still leaves two workgroups per CU of the chip's 256
+  const int32_t nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
+  const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt));
   const bf16* xx = (const bf16*)x;
   const bf16* ww = (const bf16*)w;
   bf16* yy = (bf16*)y16;
-#define SWENN_CONV(NA, AC) \
-  hipLaunchKernelGGL((k_conv3x3<NA, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, yy, y32, H, W, Cin, Cout)
-  if (nacc == 4) {
-    if (relu) SWENN_CONV(4, 1); else SWENN_CONV(4, 0);
-  } else if (nacc == 2) {
-    if (relu) SWENN_CONV(2, 1); else SWENN_CONV(2, 0);
-  } else {
-    if (relu) SWENN_CONV(1, 1); else SWENN_CONV(1, 0);
-  }
+#define SWENN_CONV(NT_, AC)                                                                                   \
+  hipLaunchKernelGGL((k_conv3x3<NT_, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, yy, y32, N, H, W, Cin, \
+                     Cout)
+#define SWENN_CONV_NT(NT_) \
+  if (relu) SWENN_CONV(NT_, 1); else SWENN_CONV(NT_, 0)
+  if (nt == 16) { SWENN_CONV_NT(16); }
+  else if (nt == 32) { SWENN_CONV_NT(32); }
+  else if (nt == 64) { SWENN_CONV_NT(64); }
+  else { SWENN_CONV_NT(128); }
+#undef SWENN_CONV_NT
 #undef SWENN_CONV
   SWENN_LAUNCH_CHECK();
 }
@@ -444,35 +533,38 @@ int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream
   SWENN_LAUNCH_CHECK();
 }
 
+// pixels per workgroup of the weight gradient: enough chunks for ~1024
+// workgroups, at least 256 pixels, a multiple of 32
+static int64_t wgrad_chunk(int64_t npix, int32_t Cin, int32_t Cout) {
+  const int64_t tiles = (int64_t)((Cout + 63) / 64) * 9 * ((Cin + 63) / 64);
+  int64_t c = (npix * tiles + 1023) / 1024;
+  c = (c + 31) / 32 * 32;
+  return c < 256 ? 256 : c;
+}
+
 int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
   const int64_t npix = (int64_t)N * H * W;
-  const int64_t chunks = (npix + kSweNnChunk - 1) / kSweNnChunk;
-  const int64_t nw = (int64_t)Cout * 9 * Cin;
-  return chunks * (nw > Cout ? nw : Cout);
+  const int64_t chunk = wgrad_chunk(npix, Cin, Cout);
+  const int64_t chunks = (npix + chunk - 1) / chunk;
+  return chunks * ((int64_t)Cout * 9 * Cin + Cout);
 }
 
 int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                  float* dw, float* db, float* work, int64_t work_len, void* stream) {
   if (!x || !dy || !dw || !work) return -1;
   if (Cin < 8 || Cout < 8 || Cin % 8 || Cout % 8) return -1;
-  if ((int64_t)N * H * W >= ((int64_t)1 << 31) - kSweNnChunk) return -1;
   if (work_len < swe_nn_wgrad_work(N, H, W, Cin, Cout)) return -1;
   const int64_t npix = (int64_t)N * H * W;
-  const int32_t chunks = (int32_t)((npix + kSweNnChunk - 1) / kSweNnChunk);
+  const int64_t chunk = wgrad_chunk(npix, Cin, Cout);
+  const int64_t chunks = (npix + chunk - 1) / chunk;
+  if (chunks > 0x7fffffffLL || chunk > 0x7fffffffLL) return -1;
   const int64_t nw = (int64_t)Cout * 9 * Cin;
-  const dim3 grid((unsigned)chunks, (unsigned)(9 * ((Cin + 31) / 32) * ((Cout + 31) / 32)));
+  const dim3 grid((unsigned)chunks, (unsigned)(9 * ((Cin + 63) / 64) * ((Cout + 63) / 64)));
   hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, S(stream), (const bf16*)x, (const bf16*)dy, work, N, H, W, Cin,
-                     Cout, kSweNnChunk);
-  hipLaunchKernelGGL(k_reduce_parts, dim3(G(nw)), dim3(256), 0, S(stream), work, dw, nw, chunks);
-  if (db) {
-    // smaller pixel chunks for the bias: chunks * (kSweNnChunk / kBiasChunk)
-    // partials of Cout floats fit the work size (9 * Cin >= 16)
-    constexpr int32_t kBiasChunk = kSweNnChunk / 16;
-    const int32_t bchunks = (int32_t)((npix + kBiasChunk - 1) / kBiasChunk);
-    hipLaunchKernelGGL(k_bgrad, dim3((unsigned)bchunks), dim3(256), 0, S(stream), (const bf16*)dy, work, npix, Cout,
-                       kBiasChunk);
-    hipLaunchKernelGGL(k_reduce_parts, dim3(G(Cout)), dim3(256), 0, S(stream), work, db, (int64_t)Cout, bchunks);
-  }
+                     Cout, (int32_t)chunk);
+  // one pass adds the chunks' weight and bias partials (contiguous per chunk)
+  hipLaunchKernelGGL(k_reduce_parts, dim3(G(db ? nw + Cout : nw)), dim3(256), 0, S(stream), work, dw, db, nw, Cout,
+                     (int32_t)chunks);
   SWENN_LAUNCH_CHECK();
 }
 

@@ -39,6 +39,8 @@ def rnd(shape, dtype=torch.bfloat16, seed=0, scale=1.0):
     (1, 8, 64, 8, 16, False),     # one accumulator, K = 72 (a partial 32-step)
     (1, 6, 128, 32, 80, True),    # four accumulators, a partial output-channel block
     (3, 4, 16, 64, 128, False),   # narrow rows
+    (1, 128, 520, 32, 136, True),  # 128-channel workgroups (many pixel blocks), a partial channel block
+    (2, 9, 13, 40, 24, False),     # pixel blocks across rows and images, a partial 32-channel step
 ])
 def test_conv3x3_forward(N, H, W, Cin, Cout, relu):
     print("case conv", N, H, W, Cin, Cout, relu, flush=True)
@@ -56,10 +58,13 @@ def test_conv3x3_forward(N, H, W, Cin, Cout, relu):
     assert torch.equal(y16, y32.to(torch.bfloat16))
 
 
-@pytest.mark.parametrize("Cin,Cout", [(16, 32), (24, 384)])
-def test_conv3x3_gradients(Cin, Cout):
-    print("case conv grads", Cin, Cout, flush=True)
-    N, H, W = 2, 10, 70
+@pytest.mark.parametrize("Cin,Cout,N,H,W", [(16, 32, 2, 10, 70), (24, 384, 2, 10, 70), (64, 72, 2, 60, 70),
+                                             (128, 8, 1, 33, 40)])
+def test_conv3x3_gradients(Cin, Cout, N, H, W):
+    """Data gradient (forward on flipped weights) and the weight and bias
+    gradients, over several pixel chunks with a partial last one and
+    partial 64-channel blocks."""
+    print("case conv grads", Cin, Cout, N, H, W, flush=True)
     x = rnd((N, H, W, Cin), seed=4)
     w = rnd((Cout, 9, Cin), seed=5, scale=0.2)
     dy = rnd((N, H, W, Cout), seed=6)
