@@ -173,8 +173,9 @@ def apply_tuning(target, args, particles):
 
 def bench_surrogate(args):
     """The learned surrogate end to end on one GPU: training pairs generated
-    by the plain method (16 maps of 256^2, random winds, K = 50), a short
-    training, then a rollout of K = 50 blocks on the 1024^2 map with its 8
+    by the plain method (64 maps of 256^2 x 400 cycles, random winds, K =
+    50), training under the map's symmetries, then a rollout of K = 50
+    blocks on the 1024^2 map with its 8
     obstacles, the plain method judging every 10th block (blocks within 1.5x
     the plain method's own run-to-run RMS accepted) and taking over after a
     rejection.  value = map-cycles/s of the whole rollout (surrogate blocks,
@@ -288,10 +289,10 @@ def main():
                     help="SWE_TUNE_SHAPE: the step kernel's compiled tile shape (0 = 16x32 cells, the default)")
     ap.add_argument("--autotune", action="store_true",
                     help="run the library's autotuner (shapes, tile orders, graphs, heavy-first) before timing")
-    ap.add_argument("--sur-terrains", type=int, default=16, help="surrogate: training maps")
-    ap.add_argument("--sur-cycles", type=int, default=160, help="surrogate: plain cycles per training map")
-    ap.add_argument("--sur-steps", type=int, default=1500, help="surrogate: Adam steps")
-    ap.add_argument("--sur-blocks", type=int, default=20, help="surrogate: rollout blocks of K = 50 cycles")
+    ap.add_argument("--sur-terrains", type=int, default=64, help="surrogate: training maps")
+    ap.add_argument("--sur-cycles", type=int, default=400, help="surrogate: plain cycles per training map")
+    ap.add_argument("--sur-steps", type=int, default=12000, help="surrogate: Adam steps")
+    ap.add_argument("--sur-blocks", type=int, default=40, help="surrogate: rollout blocks of K = 50 cycles")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -312,22 +312,66 @@ def fit_scales(model, states, K):
     model.K = K
 
 
-def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_every=100):
+def symmetry(x, az, k):
+    """One of the eight symmetries of the square periodic map applied to maps
+    x [..., X, Y] and wind azimuths az (wind (cos az, sin az) along (X, Y)):
+    bit 0 swaps the axes (az -> pi/2 - az), bit 1 reverses X (az -> pi -
+    az), bit 2 reverses Y (az -> -az).  The erosion is isotropic on a
+    periodic map -- spawn, the wind's effect and the cascade treat the two
+    axes alike -- so a transformed pair is a pair of the same process."""
+    torch = _torch()
+    az = np.asarray(az, np.float64)
+    if k & 1:
+        x = x.transpose(-1, -2)
+        az = 0.5 * math.pi - az
+    if k & 2:
+        x = torch.flip(x, dims=(-2,))
+        az = math.pi - az
+    if k & 4:
+        x = torch.flip(x, dims=(-1,))
+        az = -az
+    return x.contiguous(), az
+
+
+def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_every=100, augment=True,
+          pushforward=0.0, pushforward_from=0.5):
     """Adam on random (terrain, t) pairs of `states` with a cosine-decayed
-    learning rate; returns the loss curve and the time of the steps."""
+    learning rate; with `augment`, each batch under a random one of the
+    map's eight symmetries (square maps).  With `pushforward` p > 0, after
+    the first `pushforward_from` fraction of the steps a batch is, with
+    probability p, a pushforward pair: its input is the network's own
+    prediction K cycles on from a plain state (the state a rollout reaches
+    after a surrogate block) and its target the plain state 2K cycles on, so
+    that the network learns to correct its own errors.  Returns the loss
+    curve and the time of the steps."""
     torch = _torch()
     T, n = states.shape[0] - K, states.shape[1]
+    T2 = states.shape[0] - 2 * K
     g = torch.Generator(device="cpu").manual_seed(seed)
-    azt = torch.as_tensor(az, dtype=torch.float32)
+    az = np.asarray(az, np.float64)
+    augment = augment and states.shape[-1] == states.shape[-2]
     curve = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for st in range(steps):
-        t = torch.randint(0, T, (batch,), generator=g)
+        push = (pushforward > 0 and T2 > 0 and st >= pushforward_from * steps
+                and float(torch.rand((1,), generator=g)) < pushforward)
+        t = torch.randint(0, T2 if push else T, (batch,), generator=g)
         k = torch.randint(0, n, (batch,), generator=g)
-        x0, x1 = states[t, k], states[t + K, k]
+        a = az[k.numpy()]
+        if push:
+            xs = states[t, k]
+            dh, ds = model.predict(xs[:, 0].contiguous(), xs[:, 1].contiguous(), a.astype(np.float32))
+            x0 = torch.stack((xs[:, 0] + dh, (xs[:, 1] + ds).clamp_min(0.0)), dim=1)
+            x1 = states[t + 2 * K, k]
+        else:
+            x0, x1 = states[t, k], states[t + K, k]
+        if augment:
+            sym = int(torch.randint(0, 8, (1,), generator=g))
+            pair, a = symmetry(torch.stack((x0, x1)), a, sym)
+            x0, x1 = pair[0], pair[1]
         rate = lr * 0.5 * (1.0 + math.cos(math.pi * st / max(1, steps)))
-        loss = model.train_step(x0[:, 0].contiguous(), x0[:, 1].contiguous(), azt[k],
+        loss = model.train_step(x0[:, 0].contiguous(), x0[:, 1].contiguous(), a.astype(np.float32),
                                 (x1[:, 0] - x0[:, 0]).contiguous(), (x1[:, 1] - x0[:, 1]).contiguous(), rate)
         if st % log_every == 0 or st == steps - 1:
             curve.append((st, float(loss)))
@@ -457,6 +501,9 @@ def main():
     ap.add_argument("--tol-noise", type=float, default=None,
                     help="accept blocks within this multiple of the plain method's own run-to-run RMS")
     ap.add_argument("--rollout", default="surr1024", help="comma-separated workloads to roll out on")
+    ap.add_argument("--no-augment", action="store_true", help="train without the map's symmetries")
+    ap.add_argument("--pushforward", type=float, default=0.0,
+                    help="fraction of the later batches that start from the network's own prediction")
     ap.add_argument("--save", default=None)
     ap.add_argument("--load", default=None, help="skip data and training: use a saved model")
     ap.add_argument("--out", default=None)
@@ -478,7 +525,8 @@ def main():
         log("data: %d pairs, %.1f s of erosion" % (pairs, t_gen))
         model = Surrogate(seed=1)
         fit_scales(model, states, args.K)
-        curve, t_train = train(model, states, az, args.K, args.steps, batch=args.batch, lr=args.lr, log=log)
+        curve, t_train = train(model, states, az, args.K, args.steps, batch=args.batch, lr=args.lr, log=log,
+                               augment=not args.no_augment, pushforward=args.pushforward)
         fps = 3 * Surrogate.flops_forward(args.batch, 256, 256)
         res["train"] = {"params": model.n_params(), "loss_curve": curve, "seconds": t_train,
                         "flops_per_step": fps, "tflops": fps * args.steps / t_train / 1e12,
