@@ -109,6 +109,7 @@ def cpu_baseline(workload, seed, target_s, full=False):
                     "sample": "%d particles (%d particle-steps) of one %dx%d cycle, sequential glibc-rand order, "
                               "%.1f s on core %d" % (n, steps, w["nx"], w["ny"], dt, pin.core)}
         steps, dt = _seq_run(O, p, w, h, s, w["particles"], 1, seed)
+        print("bench: CPU full cycle done, %.1f s" % dt, file=sys.stderr, flush=True)
         out = {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
                "sample": "one full %dx%d cycle: %d particles, %d particle-steps, sequential glibc-rand order, "
                          "%.1f s on core %d" % (w["nx"], w["ny"], w["particles"], steps, dt, pin.core)}
@@ -118,6 +119,7 @@ def cpu_baseline(workload, seed, target_s, full=False):
         steps_a, dt_a = _seq_run(O, pa, a, ha, sa, a["particles"], a["cycles"], seed)
         out["cpu256_full"] = {"value": steps_a / dt_a, "seconds": dt_a, "particle_steps": steps_a, "cores": 1,
                               "sample": "256x256, %d cycles x %d particles" % (a["cycles"], a["particles"])}
+        print("bench: CPU 256x256 workload done, %.1f s" % dt_a, file=sys.stderr, flush=True)
     # the batched oracle (OpenMP over particles) on all cores, one cycle
     st = O.OracleState(w["nx"], w["ny"], h, sediment=s, params=p)
     t0 = time.perf_counter()
@@ -127,6 +129,24 @@ def cpu_baseline(workload, seed, target_s, full=False):
                              "seconds": dt_b, "cores": cores_all,
                              "sample": "one full %dx%d cycle, batched-synchronous method" % (w["nx"], w["ny"])}
     return out
+
+
+def calibrate_snap(side):
+    """A second kernel of known traffic for the counter scale: the snapshot
+    conversion of a side x side map's int64 master (k_snap: 8 bytes read and
+    4 written per cell), run three times by uploading the master."""
+    import numpy as np
+
+    from simplewinderosion_amd import World
+
+    w = World(side, side, periodic=True)
+    m = np.zeros(side * side, np.int64)
+    for _ in range(3):
+        w.upload_master(m)
+    w.synchronize()
+    w.close()
+    print(json.dumps({"snap_cells": side * side, "snap_bytes_read": 8 * side * side,
+                      "snap_bytes_written": 4 * side * side, "launches": 3}), flush=True)
 
 
 def calibrate_copy(mib):
@@ -297,9 +317,14 @@ def main():
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--calibrate-snap", type=int, default=0, metavar="SIDE",
+                    help="only run the snapshot conversion of a SIDE x SIDE map (counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.calibrate_snap:
+        calibrate_snap(args.calibrate_snap)
         return
 
     if args.gpus != 1:
