@@ -16,7 +16,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "swe_oracle.c")
 PINS = ["tests/test_oracle_fly_pins.py", "tests/test_oracle_march.py", "tests/test_oracle_geometry.py",
-        "tests/test_oracle_primitives.py"]
+        "tests/test_oracle_primitives.py", "tests/test_oracle_colour.py"]
 
 MUTANTS = {
     # wind.hpp:66 -- gravity with the wrong sign
@@ -38,6 +38,12 @@ MUTANTS = {
                   "const float lift = (1.0f + collision) * len3(sp);"),
     # world.hpp:139 -- the sequential cascade gives only half to the neighbour
     "seq_cascade_half": ("      c->height -= t; nc->height += t;", "      c->height -= t; nc->height += 0.5f * t;"),
+    # colour-ordered cascade: the classes walked y class major instead of x class major
+    "colour_class_order": ("  for (int32_t cx = 0; cx < ncx; cx++)\n    for (int32_t cy = 0; cy < ncy; cy++) {",
+                           "  for (int32_t cy = 0; cy < ncy; cy++)\n    for (int32_t cx = 0; cx < ncx; cx++) {"),
+    # colour-ordered cascade: later classes read the neighbours' heights as they were
+    "colour_stale_neighbours": ("          for (int32_t k = 0; k < cnt; k++) refresh(p, nbc[k], zm, sm, S, SS);",
+                                "          (void)nbc;"),
     # wind.hpp:111 -- the sequential pickup added to the cell
     "seq_pickup_sign": ("    cell->height -= p->suspension * diff;\n    if (p->layers == 2) {",
                         "    cell->height += p->suspension * diff;\n    if (p->layers == 2) {"),
