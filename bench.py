@@ -227,7 +227,8 @@ def bench_surrogate(args):
     world.close()
     plain.close()
     rate = args.sur_blocks * K / rec["t_total_s"]
-    plain_rate = K / rec["plain_seconds_per_block"]
+    # the plain method's rate comes from the judged blocks (none in a run shorter than judge_every)
+    plain_rate = K / rec["plain_seconds_per_block"] if rec["plain_seconds_per_block"] > 0 else None
     judged = rec["judged"]
     line = {"metric": "map-cycles/s", "value": rate, "unit": "map-cycles/s", "n_gpus": 1,
             "steps": args.sur_blocks, "warmup": 0, "ms_per_step": 1e3 * rec["t_total_s"] / args.sur_blocks,
@@ -236,7 +237,7 @@ def bench_surrogate(args):
             "data": "synthetic", "config": {"workload": "surrogate rollout 1024x1024 + 8 pyramids", "K": K,
                                             "blocks": args.sur_blocks, "judge_every": 10,
                                             "parallelism": "single GPU"},
-            "plain_map_cycles_per_s": plain_rate, "speedup_vs_plain": rate / plain_rate,
+            "plain_map_cycles_per_s": plain_rate, "speedup_vs_plain": rate / plain_rate if plain_rate else None,
             "blocks": {"surrogate": rec["surrogate_blocks"], "plain_fallback": rec["plain_blocks"],
                        "judged": rec["judge_blocks"],
                        "accepted": sum(1 for j in judged if j["accepted"])},

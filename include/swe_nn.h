@@ -20,14 +20,20 @@ extern "C" {
  * Cin % 8 == 0; BF16 MFMA with FP32 accumulation. */
 int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
                    int32_t Cout, int32_t relu, void* y16, float* y32, void* stream);
+/* y16 = bf16(conv3x3(x, w) + bias), zero where mask <= 0 (mask [N][H][W][Cout]
+ * bf16, optional; bias optional): a data gradient through the ReLU of the
+ * layer that produced mask, in one pass.  The same bits as swe_nn_conv3x3
+ * into y32 followed by swe_nn_relu_grad. */
+int swe_nn_conv3x3_masked(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w,
+                          const float* bias, int32_t Cout, const void* mask, void* y16, void* stream);
 /* wt[ci][t][co] = w[co][8-t][ci]: the weights whose forward convolution of
  * dy is the data gradient dx (Cin and Cout swap). */
 int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream);
 /* floats of work memory swe_nn_wgrad needs */
 int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
 /* dw = sum_p dy[p] x[p + t]^T (float [Cout][9][Cin]); db = sum_p dy[p]
- * (float [Cout], optional); deterministic (per-chunk partials in `work`,
- * added in chunk order). */
+ * (float [Cout], optional); Cin % 8 == Cout % 8 == 0; deterministic
+ * (per-chunk partials in `work`, 16-byte aligned, added in a fixed order). */
 int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                  float* dw, float* db, float* work, int64_t work_len, void* stream);
 /* g16 = bf16(g32 * (y > 0)); y may be NULL (conversion only) */
@@ -40,6 +46,18 @@ int swe_nn_pool2(const void* x, void* y, int32_t N, int32_t H, int32_t W, int32_
 int swe_nn_pool2_grad(const void* dy, void* dx, int32_t N, int32_t H, int32_t W, int32_t C, void* stream);
 int swe_nn_up2(const void* x, void* y, int32_t N, int32_t h, int32_t w, int32_t C, void* stream);
 int swe_nn_up2_grad(const void* dy, void* dx, int32_t N, int32_t h, int32_t w, int32_t C, void* stream);
+/* Fused skip-connection passes (C % 8 == 0), the same bits as the unfused
+ * calls: out = up2(x) + skip (swe_nn_up2, swe_nn_add); dx = pool2_grad(dy)
+ * + skip, zero where mask <= 0 (swe_nn_pool2_grad, swe_nn_add,
+ * swe_nn_relu_mask; mask optional); dx = up2_grad(dy), zero where mask <= 0
+ * (swe_nn_up2_grad, swe_nn_relu_mask; mask optional).  h, w: the low
+ * resolution; H, W: the high one. */
+int swe_nn_up2_add(const void* x, const void* skip, void* out, int32_t N, int32_t h, int32_t w, int32_t C,
+                   void* stream);
+int swe_nn_pool2_grad_add(const void* dy, const void* skip, const void* mask, void* dx, int32_t N, int32_t H,
+                          int32_t W, int32_t C, void* stream);
+int swe_nn_up2_grad_mask(const void* dy, const void* mask, void* dx, int32_t N, int32_t h, int32_t w, int32_t C,
+                         void* stream);
 /* mean squared error over the first `used` of C channels (target [npix][used]);
  * grad [npix][C]; part[ceil(npix*C/256)] partial loss sums */
 int swe_nn_mse(const float* pred, const float* target, float* grad, float* part, int64_t npix, int32_t C,

@@ -38,11 +38,13 @@ __device__ __forceinline__ int32_t wrapi(int32_t v, int32_t n) { return v < 0 ? 
 // v_mfma_f32_16x16x32_bf16, lane l supplies A row l%16 and B row l%16, both
 // at k = 8*(l/16) .. +7, and holds D rows 4*(l/16) .. +3 of column l%16.
 // ---------------------------------------------------------------------------
+// ACT 0: none, 1: ReLU, 2: zero where mask <= 0 (the ReLU gradient of a
+// data gradient through the layer below, mask = that layer's output).
 template <int NT, int ACT>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, const bf16* __restrict__ w,
-                                                 const float* __restrict__ bias, bf16* __restrict__ y16,
-                                                 float* __restrict__ y32, int32_t N, int32_t H, int32_t W,
-                                                 int32_t Cin, int32_t Cout) {
+                                                 const float* __restrict__ bias, const bf16* __restrict__ mask,
+                                                 bf16* __restrict__ y16, float* __restrict__ y32, int32_t N,
+                                                 int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
   constexpr int WM = NT >= 64 ? 2 : 4;  // waves along the pixels
   constexpr int WN = 4 / WM;            // ... and the channels
   constexpr int TM = 128 / WM, TN = NT / WN;
@@ -142,6 +144,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
         if (p >= P) continue;
         float v = acc[i][j][e] + bv;
         if (ACT == 1) v = v > 0.0f ? v : 0.0f;
+        if (ACT == 2) v = bf2f(mask[p * Cout + co]) > 0.0f ? v : 0.0f;
         if (y16) y16[p * Cout + co] = f2bf(v);
         if (y32) y32[p * Cout + co] = v;
       }
@@ -169,7 +172,7 @@ __global__ void k_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int32_
 // sub-block.  The workgroups of tap 4 (no shift) and the first ci block
 // also sum their dY runs for the bias.  Each chunk writes its partial
 // block to `part` ([chunk][Cout * 9 * Cin + Cout]); k_reduce_parts adds the
-// chunks in order, so the result is deterministic.  Cin % 8 == Cout % 8 == 0.
+// chunks in a fixed order, so the result is deterministic.  Cin % 8 == Cout % 8 == 0.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, const bf16* __restrict__ dy,
                                                float* __restrict__ part, int32_t N, int32_t H, int32_t W,
@@ -278,17 +281,238 @@ __global__ __launch_bounds__(256) void k_wgrad(const bf16* __restrict__ x, const
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weight and bias gradients over row strips (W % 32 == 0): a workgroup owns a
+// CB (co) x IB (ci) block of all nine taps and walks R rows of one 32-pixel
+// column strip of one image.  Per row y it loads the dY row (32 pixels x CB)
+// and the X row y + 1 (34 pixels, x0 - 1 .. x0 + 32, periodic); X rows y - 1,
+// y, y + 1 stay in a four-row LDS ring, so every X and dY value is loaded
+// once per strip and serves all nine taps.  LDS rows are channel-major
+// (pixels contiguous, the MFMA's k), 40 elements a row, with the 8-pixel
+// blocks of columns 0..31 permuted per row (block ^ (row / 8) % 4) so that
+// the transposing two-byte stores of eight channel rows fall in different
+// banks.  The tap shift dx = -1, 0, +1 is a B operand read at column
+// offset 0, 1, 2: one aligned 16-byte read plus the next dword, shifted by
+// one element with v_alignbit for the odd offset.  Waves tile the block
+// 2 x 2; each holds 9 x (CB/32) x (IB/32) accumulators.  The workgroups of
+// the first ci block also sum the dY rows for the bias.  Partials go to
+// `part` ([chunk][Cout * 9 * Cin + Cout]) like k_wgrad's.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t wg_col(int32_t row, int32_t c) { return c < 32 ? c ^ (8 * ((row >> 3) & 3)) : c; }
+
+template <int CB, int IB>
+__global__ __launch_bounds__(256) void k_wgrad_rows(const bf16* __restrict__ x, const bf16* __restrict__ dy,
+                                                    float* __restrict__ part, int32_t N, int32_t H, int32_t W,
+                                                    int32_t Cin, int32_t Cout, int32_t R) {
+  constexpr int LD = 40;
+  constexpr int TI = CB / 32, TJ = IB / 32;  // 16 x 16 tiles per wave along co and ci
+  constexpr int DG = CB / 8, XG = IB / 8;    // 16-byte channel groups of a pixel
+  constexpr int XITEMS = 34 * XG;            // X row loads (pixel, group)
+  __shared__ __attribute__((aligned(16))) bf16 s_x[4][IB * LD];
+  __shared__ __attribute__((aligned(16))) bf16 s_dy[2][CB * LD];
+  __shared__ float s_b[32][CB + 1];
+  const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t segs = W / 32, cps = (H + R - 1) / R;
+  const int32_t chunk = blockIdx.x;
+  const int32_t rc = chunk % cps, sg = (chunk / cps) % segs, n = chunk / (cps * segs);
+  const int32_t nci = (Cin + IB - 1) / IB;
+  const int32_t ciB = (int32_t)(blockIdx.y % nci) * IB, coB = (int32_t)(blockIdx.y / nci) * CB;
+  const int32_t y0 = rc * R, rows = R < H - y0 ? R : H - y0, x0 = sg * 32;
+  const bool do_bias = ciB == 0;
+  const bf16x8 zero = {};
+
+  // dY loads: thread -> (pixel dpx, group dg), groups fastest (coalesced)
+  const bool dy_item = tid < 32 * DG;
+  const int32_t dpx = tid / DG, dg = tid % DG;
+  const bool dy_ok = dy_item && coB + 8 * dg < Cout;
+  const int64_t dy_col = (int64_t)(x0 + dpx) * Cout + coB + 8 * dg;
+  // X loads: items tid and tid + 256 -> (column c, group g)
+  int32_t xc[2], xg[2];
+  bool xok[2], xitem[2];
+  int64_t xcol[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int32_t u = tid + 256 * h;
+    xitem[h] = u < XITEMS;
+    xc[h] = u / XG;
+    xg[h] = u % XG;
+    xok[h] = xitem[h] && ciB + 8 * xg[h] < Cin;
+    xcol[h] = (int64_t)wrapi(x0 + xc[h] - 1, W) * Cin + ciB + 8 * xg[h];
+  }
+  const int64_t img = (int64_t)n * H;
+
+  struct Regs {
+    bf16x8 d, xa, xb;
+  };
+  // the loads of X row k (image row y0 - 1 + k) and dY row y
+  auto load = [&](int32_t k, int32_t y, bool with_dy) __attribute__((always_inline)) {
+    Regs r;
+    const int64_t xrow = (img + wrapi(y0 - 1 + k, H)) * W * Cin;
+    r.xa = xok[0] ? *(const bf16x8*)(x + xrow + xcol[0]) : zero;
+    r.xb = xok[1] ? *(const bf16x8*)(x + xrow + xcol[1]) : zero;
+    r.d = (with_dy && dy_ok) ? *(const bf16x8*)(dy + (img + y) * W * Cout + dy_col) : zero;
+    return r;
+  };
+  float bsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  auto store_x = [&](const Regs& r, int32_t slot) __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      if (!xitem[h]) continue;
+      const bf16x8 v = h == 0 ? r.xa : r.xb;
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const int32_t row = 8 * xg[h] + e;
+        s_x[slot][row * LD + wg_col(row, xc[h])] = v[e];
+      }
+    }
+  };
+  auto store_dy = [&](const Regs& r, int32_t slot) __attribute__((always_inline)) {
+    if (!dy_item) return;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int32_t row = 8 * dg + e;
+      s_dy[slot][row * LD + wg_col(row, dpx)] = r.d[e];
+    }
+    if (do_bias) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) bsum[e] += bf2f(r.d[e]);
+    }
+  };
+
+  const int32_t wr = (CB / 2) * (wave >> 1), wc = (IB / 2) * (wave & 1);
+  const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
+  f32x4 acc[9][TI][TJ];
+#pragma unroll
+  for (int t = 0; t < 9; t++)
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+      for (int j = 0; j < TJ; j++) acc[t][i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
+
+  // prologue: X rows y0 - 1, y0, y0 + 1 and dY row y0, all loads in flight together
+  {
+    const Regs r0 = load(0, y0, true), r1 = load(1, y0, false), r2 = load(2, y0, false);
+    store_x(r0, 0);
+    store_x(r1, 1);
+    store_x(r2, 2);
+    store_dy(r0, 0);
+  }
+  __syncthreads();
+  for (int32_t s = 0; s < rows; s++) {
+    const bool more = s + 1 < rows;
+    Regs nx;
+    if (more) nx = load(s + 3, y0 + s + 1, true);
+    const int32_t ds = s & 1;
+    bf16x8 a[TI];
+#pragma unroll
+    for (int i = 0; i < TI; i++) {
+      const int32_t row = wr + 16 * i + rl;
+      a[i] = *(const bf16x8*)&s_dy[ds][row * LD + wg_col(row, kq)];
+    }
+#pragma unroll
+    for (int dyi = 0; dyi < 3; dyi++) {
+      const bf16* xr = s_x[(s + dyi) & 3];
+#pragma unroll
+      for (int j = 0; j < TJ; j++) {
+        const int32_t row = wc + 16 * j + rl;
+        const u32x4 v0 = *(const u32x4*)&xr[row * LD + wg_col(row, kq)];
+        const uint32_t v1 = *(const uint32_t*)&xr[row * LD + wg_col(row, kq + 8)];
+        const u32x4 vm = v0;  // columns kq .. kq + 7: pixel x + dx for dx = -1
+        const u32x4 vz = {__builtin_amdgcn_alignbit(v0[1], v0[0], 16), __builtin_amdgcn_alignbit(v0[2], v0[1], 16),
+                          __builtin_amdgcn_alignbit(v0[3], v0[2], 16), __builtin_amdgcn_alignbit(v1, v0[3], 16)};
+        const u32x4 vp = {v0[1], v0[2], v0[3], v1};
+        const bf16x8 bm = __builtin_bit_cast(bf16x8, vm), bz = __builtin_bit_cast(bf16x8, vz),
+                     bp = __builtin_bit_cast(bf16x8, vp);
+#pragma unroll
+        for (int i = 0; i < TI; i++) {
+          acc[3 * dyi + 0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bm, acc[3 * dyi + 0][i][j], 0, 0, 0);
+          acc[3 * dyi + 1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bz, acc[3 * dyi + 1][i][j], 0, 0, 0);
+          acc[3 * dyi + 2][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bp, acc[3 * dyi + 2][i][j], 0, 0, 0);
+        }
+      }
+    }
+    if (more) {
+      store_x(nx, (s + 3) & 3);
+      store_dy(nx, ds ^ 1);
+    }
+    __syncthreads();
+  }
+
+  const int64_t nw = (int64_t)Cout * 9 * Cin;
+  float* out = part + (int64_t)chunk * (nw + Cout);
+  // D rows (co) = wr + 16 i + 4 (lane / 16) + e, column (ci) = wc + 16 j + lane % 16
+#pragma unroll
+  for (int j = 0; j < TJ; j++) {
+    const int32_t ci = ciB + wc + 16 * j + rl;
+    if (ci >= Cin) continue;
+#pragma unroll
+    for (int i = 0; i < TI; i++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int32_t co = coB + wr + 16 * i + 4 * (lane >> 4) + e;
+        if (co >= Cout) continue;
+#pragma unroll
+        for (int t = 0; t < 9; t++) out[((int64_t)co * 9 + t) * Cin + ci] = acc[t][i][j][e];
+      }
+  }
+  if (do_bias) {  // the 32 pixel columns' sums per channel, added in a fixed order
+    if (dy_item) {
+#pragma unroll
+      for (int e = 0; e < 8; e++) s_b[dpx][8 * dg + e] = bsum[e];
+    }
+    __syncthreads();
+    if (tid < CB) {
+      float sb = 0.0f;
+      for (int k = 0; k < 32; k++) sb += s_b[k][tid];
+      if (coB + tid < Cout) out[nw + coB + tid] = sb;
+    }
+  }
+}
+
 // dw[i] (i < nw) and db[i - nw] (nw <= i < nw + Cout, when db is given):
-// the chunks' partials added in chunk order
-__global__ void k_reduce_parts(const float* __restrict__ part, float* __restrict__ dw, float* __restrict__ db,
-                               int64_t nw, int32_t Cout, int32_t nparts) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+// the chunks' partials added in a fixed order.  A workgroup takes 64
+// consecutive entries; its 16 groups of 16 threads read them as float4 runs
+// from every 16th chunk (group g: chunks g, g + 16, ...), and the groups'
+// sums are added in group order.  Rows are nw + Cout floats, a multiple of
+// 8 (Cin % 8 == Cout % 8 == 0), so the runs are aligned.
+__global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, float* __restrict__ dw,
+                                                      float* __restrict__ db, int64_t nw, int32_t Cout,
+                                                      int32_t nparts) {
+  __shared__ float s_sum[16][65];
+  const int32_t tid = threadIdx.x, c4 = tid & 15, g = tid >> 4;
   const int64_t stride = nw + Cout;
-  if (i >= (db ? stride : nw)) return;
-  float s = 0.0f;
-  for (int32_t k = 0; k < nparts; k++) s += part[(int64_t)k * stride + i];
-  if (i < nw) dw[i] = s;
-  else db[i - nw] = s;
+  const int64_t n = db ? stride : nw;
+  const int64_t i0 = (int64_t)blockIdx.x * 64 + 4 * c4;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (i0 < n) {
+    const float* p = part + i0;
+    int32_t k = g;
+    // four chunks' loads in flight per step; the adds stay in chunk order
+    for (; k + 48 < nparts; k += 64) {
+      const f32x4 v0 = *(const f32x4*)(p + (int64_t)k * stride);
+      const f32x4 v1 = *(const f32x4*)(p + (int64_t)(k + 16) * stride);
+      const f32x4 v2 = *(const f32x4*)(p + (int64_t)(k + 32) * stride);
+      const f32x4 v3 = *(const f32x4*)(p + (int64_t)(k + 48) * stride);
+      acc += v0;
+      acc += v1;
+      acc += v2;
+      acc += v3;
+    }
+    for (; k < nparts; k += 16) acc += *(const f32x4*)(p + (int64_t)k * stride);
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) s_sum[g][4 * c4 + e] = acc[e];
+  __syncthreads();
+  if (tid < 64) {
+    const int64_t i = (int64_t)blockIdx.x * 64 + tid;
+    if (i < n) {
+      float s = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 16; q++) s += s_sum[q][tid];
+      if (i < nw) dw[i] = s;
+      else db[i - nw] = s;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -369,6 +593,104 @@ __global__ void k_up2_grad(const bf16* __restrict__ dy, bf16* __restrict__ dx, i
   const int64_t b = ((im * 2 * h + 2 * yy) * W + 2 * xx) * C + c;
   const float s = bf2f(dy[b]) + bf2f(dy[b + C]) + bf2f(dy[b + (int64_t)W * C]) + bf2f(dy[b + (int64_t)W * C + C]);
   dx[i] = f2bf(s);
+}
+
+// ---------------------------------------------------------------------------
+// Fused passes of the skip connections, 8 channels (16 bytes) a thread
+// (C % 8 == 0); each rounds to bf16 where the unfused sequence of the
+// kernels above does, so the results are the same bits.
+// ---------------------------------------------------------------------------
+// out = bf16(up2(x) + skip): x [N][h][w][C] upsampled (nearest) and added
+// to skip [N][2h][2w][C] (k_up2 then k_add)
+__global__ void k_up2_add(const bf16* __restrict__ x, const bf16* __restrict__ skip, bf16* __restrict__ out,
+                          int32_t N, int32_t h, int32_t w, int32_t C) {
+  const int32_t C8 = C / 8, H = 2 * h, W = 2 * w;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)N * H * W * C8) return;
+  const int32_t c = 8 * (int32_t)(i % C8);
+  const int64_t q = i / C8;
+  const int32_t xx = (int32_t)(q % W), yy = (int32_t)((q / W) % H);
+  const int64_t im = q / ((int64_t)W * H);
+  const bf16x8 a = *(const bf16x8*)(x + ((im * h + yy / 2) * w + xx / 2) * C + c);
+  const bf16x8 b = *(const bf16x8*)(skip + q * C + c);
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; e++) r[e] = f2bf(bf2f(a[e]) + bf2f(b[e]));
+  *(bf16x8*)(out + q * C + c) = r;
+}
+
+// dx = bf16(bf16(dy / 4 upsampled) + skip), zero where mask <= 0 (mask
+// optional): the pool's gradient plus the skip connection's, through the
+// ReLU of the pooled layer (k_pool2_grad, k_add, k_relu_mask)
+__global__ void k_pool2_grad_add(const bf16* __restrict__ dy, const bf16* __restrict__ skip,
+                                 const bf16* __restrict__ mask, bf16* __restrict__ dx, int32_t N, int32_t H,
+                                 int32_t W, int32_t C) {
+  const int32_t C8 = C / 8;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)N * H * W * C8) return;
+  const int32_t c = 8 * (int32_t)(i % C8);
+  const int64_t q = i / C8;
+  const int32_t xx = (int32_t)(q % W), yy = (int32_t)((q / W) % H);
+  const int64_t im = q / ((int64_t)W * H);
+  const bf16x8 d = *(const bf16x8*)(dy + ((im * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c);
+  const bf16x8 sk = *(const bf16x8*)(skip + q * C + c);
+  bf16x8 m = {};
+  if (mask) m = *(const bf16x8*)(mask + q * C + c);
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    bf16 v = f2bf(bf2f(f2bf(0.25f * bf2f(d[e]))) + bf2f(sk[e]));
+    if (mask && !(bf2f(m[e]) > 0.0f)) v = f2bf(0.0f);
+    r[e] = v;
+  }
+  *(bf16x8*)(dx + q * C + c) = r;
+}
+
+// dx = bf16(sum of dy over each 2x2 block), zero where mask <= 0 (mask
+// optional): up2's gradient through the ReLU of the layer below
+// (k_up2_grad, k_relu_mask)
+__global__ void k_up2_grad_mask(const bf16* __restrict__ dy, const bf16* __restrict__ mask, bf16* __restrict__ dx,
+                                int32_t N, int32_t h, int32_t w, int32_t C) {
+  const int32_t C8 = C / 8;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)N * h * w * C8) return;
+  const int32_t c = 8 * (int32_t)(i % C8);
+  const int64_t q = i / C8;
+  const int32_t xx = (int32_t)(q % w), yy = (int32_t)((q / w) % h);
+  const int64_t im = q / ((int64_t)w * h);
+  const int32_t W = 2 * w;
+  const int64_t b = ((im * 2 * h + 2 * yy) * W + 2 * xx) * C + c;
+  const bf16x8 d0 = *(const bf16x8*)(dy + b), d1 = *(const bf16x8*)(dy + b + C);
+  const bf16x8 d2 = *(const bf16x8*)(dy + b + (int64_t)W * C), d3 = *(const bf16x8*)(dy + b + (int64_t)W * C + C);
+  bf16x8 m = {};
+  if (mask) m = *(const bf16x8*)(mask + q * C + c);
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; e++) {
+    bf16 v = f2bf(bf2f(d0[e]) + bf2f(d1[e]) + bf2f(d2[e]) + bf2f(d3[e]));
+    if (mask && !(bf2f(m[e]) > 0.0f)) v = f2bf(0.0f);
+    r[e] = v;
+  }
+  *(bf16x8*)(dx + q * C + c) = r;
+}
+
+// 2x2 average pool, 8 channels a thread (k_pool2's arithmetic)
+__global__ void k_pool2_v8(const bf16* __restrict__ x, bf16* __restrict__ y, int32_t N, int32_t H, int32_t W,
+                           int32_t C) {
+  const int32_t C8 = C / 8, h = H / 2, w = W / 2;
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)N * h * w * C8) return;
+  const int32_t c = 8 * (int32_t)(i % C8);
+  const int64_t q = i / C8;
+  const int32_t xx = (int32_t)(q % w), yy = (int32_t)((q / w) % h);
+  const int64_t im = q / ((int64_t)w * h);
+  const int64_t b = ((im * H + 2 * yy) * W + 2 * xx) * C + c;
+  const bf16x8 a0 = *(const bf16x8*)(x + b), a1 = *(const bf16x8*)(x + b + C);
+  const bf16x8 a2 = *(const bf16x8*)(x + b + (int64_t)W * C), a3 = *(const bf16x8*)(x + b + (int64_t)W * C + C);
+  bf16x8 r;
+#pragma unroll
+  for (int e = 0; e < 8; e++) r[e] = f2bf(0.25f * (bf2f(a0[e]) + bf2f(a1[e]) + bf2f(a2[e]) + bf2f(a3[e])));
+  *(bf16x8*)(y + q * C + c) = r;
 }
 
 // loss = sum over pixels and the first `used` of C channels of
@@ -499,24 +821,27 @@ static inline unsigned G(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 extern "C" {
 
-int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
-                   int32_t Cout, int32_t relu, void* y16, float* y32, void* stream) {
+// output channels per workgroup: the layer's (up to 64), 128 where that
+// still leaves two workgroups per CU of the chip's 256
+static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
+                       int32_t Cout, int act, const void* mask, void* y16, float* y32, void* stream) {
   if (!x || !w || N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (!y16 && !y32)) return -1;
   const int64_t P = (int64_t)N * H * W;
   const int64_t mt = (P + 127) / 128;
   if (mt > 0x7fffffffLL) return -1;
-  // output channels per workgroup: the layer's (up to 64), 128 where that
-  // still leaves two workgroups per CU of the chip's 256
   const int32_t nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
   const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt));
   const bf16* xx = (const bf16*)x;
   const bf16* ww = (const bf16*)w;
+  const bf16* mm = (const bf16*)mask;
   bf16* yy = (bf16*)y16;
-#define SWENN_CONV(NT_, AC)                                                                                   \
-  hipLaunchKernelGGL((k_conv3x3<NT_, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, yy, y32, N, H, W, Cin, \
+#define SWENN_CONV(NT_, AC)                                                                                       \
+  hipLaunchKernelGGL((k_conv3x3<NT_, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, mm, yy, y32, N, H, W, Cin, \
                      Cout)
-#define SWENN_CONV_NT(NT_) \
-  if (relu) SWENN_CONV(NT_, 1); else SWENN_CONV(NT_, 0)
+#define SWENN_CONV_NT(NT_)            \
+  if (act == 1) SWENN_CONV(NT_, 1);  \
+  else if (act == 2) SWENN_CONV(NT_, 2); \
+  else SWENN_CONV(NT_, 0)
   if (nt == 16) { SWENN_CONV_NT(16); }
   else if (nt == 32) { SWENN_CONV_NT(32); }
   else if (nt == 64) { SWENN_CONV_NT(64); }
@@ -526,6 +851,17 @@ int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, 
   SWENN_LAUNCH_CHECK();
 }
 
+int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
+                   int32_t Cout, int32_t relu, void* y16, float* y32, void* stream) {
+  return conv_launch(x, N, H, W, Cin, w, bias, Cout, relu ? 1 : 0, nullptr, y16, y32, stream);
+}
+
+int swe_nn_conv3x3_masked(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w,
+                          const float* bias, int32_t Cout, const void* mask, void* y16, void* stream) {
+  if (!y16) return -1;
+  return conv_launch(x, N, H, W, Cin, w, bias, Cout, mask ? 2 : 0, mask, y16, nullptr, stream);
+}
+
 int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream) {
   if (!w || !wt) return -1;
   hipLaunchKernelGGL(k_flip, dim3(G((int64_t)Cin * 9 * Cout)), dim3(256), 0, S(stream), (const bf16*)w, (bf16*)wt,
@@ -533,38 +869,87 @@ int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream
   SWENN_LAUNCH_CHECK();
 }
 
-// pixels per workgroup of the weight gradient: enough chunks for ~1024
-// workgroups, at least 256 pixels, a multiple of 32
-static int64_t wgrad_chunk(int64_t npix, int32_t Cin, int32_t Cout) {
-  const int64_t tiles = (int64_t)((Cout + 63) / 64) * 9 * ((Cin + 63) / 64);
-  int64_t c = (npix * tiles + 1023) / 1024;
-  c = (c + 31) / 32 * 32;
-  return c < 256 ? 256 : c;
+// How a weight gradient is split.  Maps whose rows are whole 32-pixel
+// strips take k_wgrad_rows: CB x IB channel blocks (32; 64 where the layer
+// has that many channels when SWENN_WGRAD_BLOCK is 64), each strip cut into
+// chunks of R rows, enough chunks for ~SWENN_WGRAD_WGS workgroups but at
+// most ceil(H / 8) a strip.  Measured on the surrogate's training layers
+// (MI355X, batch 8 at 256^2): 32 x 32 blocks and 1024 workgroups are the
+// fastest (64 x 64 blocks hold 356 registers a lane, one wave per SIMD).  Others take k_wgrad: 64 x 64 blocks of
+// one tap over chunks of `chunk` pixels (~1024 workgroups, >= 256 pixels, a
+// multiple of 32).  Either way the partials are [chunks][Cout*9*Cin + Cout].
+#ifndef SWENN_WGRAD_BLOCK
+#define SWENN_WGRAD_BLOCK 32
+#endif
+#ifndef SWENN_WGRAD_WGS
+#define SWENN_WGRAD_WGS 1024
+#endif
+#ifndef SWENN_WGRAD_ROWS
+#define SWENN_WGRAD_ROWS 1
+#endif
+struct WgradPlan {
+  bool rows;
+  int32_t cb, ib, R;
+  int64_t chunk, chunks, blocks;
+};
+
+static WgradPlan wgrad_plan(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
+  WgradPlan p{};
+  const int64_t npix = (int64_t)N * H * W;
+  if (SWENN_WGRAD_ROWS && W % 32 == 0) {
+    p.rows = true;
+    p.cb = Cout >= 64 && SWENN_WGRAD_BLOCK >= 64 ? 64 : 32;
+    p.ib = Cin >= 64 && SWENN_WGRAD_BLOCK >= 64 ? 64 : 32;
+    p.blocks = (int64_t)((Cout + p.cb - 1) / p.cb) * ((Cin + p.ib - 1) / p.ib);
+    const int64_t strips = (int64_t)N * (W / 32);
+    int64_t cps = (SWENN_WGRAD_WGS + strips * p.blocks - 1) / (strips * p.blocks);
+    const int64_t cmax = (H + 7) / 8;
+    if (cps > cmax) cps = cmax;
+    if (cps < 1) cps = 1;
+    p.R = (int32_t)((H + cps - 1) / cps);
+    p.chunks = strips * ((H + p.R - 1) / p.R);
+  } else {
+    p.blocks = (int64_t)((Cout + 63) / 64) * 9 * ((Cin + 63) / 64);
+    int64_t c = (npix * p.blocks + 1023) / 1024;
+    c = (c + 31) / 32 * 32;
+    p.chunk = c < 256 ? 256 : c;
+    p.chunks = (npix + p.chunk - 1) / p.chunk;
+  }
+  return p;
 }
 
 int64_t swe_nn_wgrad_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
-  const int64_t npix = (int64_t)N * H * W;
-  const int64_t chunk = wgrad_chunk(npix, Cin, Cout);
-  const int64_t chunks = (npix + chunk - 1) / chunk;
-  return chunks * ((int64_t)Cout * 9 * Cin + Cout);
+  if (N < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return 0;
+  return wgrad_plan(N, H, W, Cin, Cout).chunks * ((int64_t)Cout * 9 * Cin + Cout);
 }
 
 int swe_nn_wgrad(const void* x, const void* dy, int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout,
                  float* dw, float* db, float* work, int64_t work_len, void* stream) {
-  if (!x || !dy || !dw || !work) return -1;
-  if (Cin < 8 || Cout < 8 || Cin % 8 || Cout % 8) return -1;
+  if (!x || !dy || !dw || !work || (uintptr_t)work % 16) return -1;
+  if (N < 1 || H < 1 || W < 1 || Cin < 8 || Cout < 8 || Cin % 8 || Cout % 8) return -1;
   if (work_len < swe_nn_wgrad_work(N, H, W, Cin, Cout)) return -1;
-  const int64_t npix = (int64_t)N * H * W;
-  const int64_t chunk = wgrad_chunk(npix, Cin, Cout);
-  const int64_t chunks = (npix + chunk - 1) / chunk;
-  if (chunks > 0x7fffffffLL || chunk > 0x7fffffffLL) return -1;
+  const WgradPlan p = wgrad_plan(N, H, W, Cin, Cout);
+  if (p.chunks > 0x7fffffffLL || p.chunk > 0x7fffffffLL || p.blocks > 65535) return -1;
   const int64_t nw = (int64_t)Cout * 9 * Cin;
-  const dim3 grid((unsigned)chunks, (unsigned)(9 * ((Cin + 63) / 64) * ((Cout + 63) / 64)));
-  hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, S(stream), (const bf16*)x, (const bf16*)dy, work, N, H, W, Cin,
-                     Cout, (int32_t)chunk);
+  const dim3 grid((unsigned)p.chunks, (unsigned)p.blocks);
+  const bf16* xx = (const bf16*)x;
+  const bf16* dd = (const bf16*)dy;
+  if (p.rows) {
+#define SWENN_ROWS_KERNEL(CB_, IB_)                                                                         \
+  hipLaunchKernelGGL((k_wgrad_rows<CB_, IB_>), grid, dim3(256), 0, S(stream), xx, dd, work, N, H, W, Cin, Cout, \
+                     p.R)
+    if (p.cb == 64 && p.ib == 64) SWENN_ROWS_KERNEL(64, 64);
+    else if (p.cb == 64) SWENN_ROWS_KERNEL(64, 32);
+    else if (p.ib == 64) SWENN_ROWS_KERNEL(32, 64);
+    else SWENN_ROWS_KERNEL(32, 32);
+#undef SWENN_ROWS_KERNEL
+  } else {
+    hipLaunchKernelGGL(k_wgrad, grid, dim3(256), 0, S(stream), xx, dd, work, N, H, W, Cin, Cout, (int32_t)p.chunk);
+  }
   // one pass adds the chunks' weight and bias partials (contiguous per chunk)
-  hipLaunchKernelGGL(k_reduce_parts, dim3(G(db ? nw + Cout : nw)), dim3(256), 0, S(stream), work, dw, db, nw, Cout,
-                     (int32_t)chunks);
+  const int64_t nred = db ? nw + Cout : nw;
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((nred + 63) / 64)), dim3(256), 0, S(stream), work, dw, db, nw,
+                     Cout, (int32_t)p.chunks);
   SWENN_LAUNCH_CHECK();
 }
 
@@ -586,9 +971,37 @@ int swe_nn_add(const void* a, const void* b, void* out, int64_t n, void* stream)
 }
 
 int swe_nn_pool2(const void* x, void* y, int32_t N, int32_t H, int32_t W, int32_t C, void* stream) {
-  if (H % 2 || W % 2) return -1;
-  hipLaunchKernelGGL(k_pool2, dim3(G((int64_t)N * (H / 2) * (W / 2) * C)), dim3(256), 0, S(stream), (const bf16*)x,
-                     (bf16*)y, N, H, W, C);
+  if (!x || !y || H % 2 || W % 2) return -1;
+  if (C % 8 == 0)
+    hipLaunchKernelGGL(k_pool2_v8, dim3(G((int64_t)N * (H / 2) * (W / 2) * (C / 8))), dim3(256), 0, S(stream),
+                       (const bf16*)x, (bf16*)y, N, H, W, C);
+  else
+    hipLaunchKernelGGL(k_pool2, dim3(G((int64_t)N * (H / 2) * (W / 2) * C)), dim3(256), 0, S(stream), (const bf16*)x,
+                       (bf16*)y, N, H, W, C);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_up2_add(const void* x, const void* skip, void* out, int32_t N, int32_t h, int32_t w, int32_t C,
+                   void* stream) {
+  if (!x || !skip || !out || N < 1 || h < 1 || w < 1 || C < 8 || C % 8) return -1;
+  hipLaunchKernelGGL(k_up2_add, dim3(G((int64_t)N * 4 * h * w * (C / 8))), dim3(256), 0, S(stream), (const bf16*)x,
+                     (const bf16*)skip, (bf16*)out, N, h, w, C);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_pool2_grad_add(const void* dy, const void* skip, const void* mask, void* dx, int32_t N, int32_t H,
+                          int32_t W, int32_t C, void* stream) {
+  if (!dy || !skip || !dx || N < 1 || H < 2 || W < 2 || H % 2 || W % 2 || C < 8 || C % 8) return -1;
+  hipLaunchKernelGGL(k_pool2_grad_add, dim3(G((int64_t)N * H * W * (C / 8))), dim3(256), 0, S(stream),
+                     (const bf16*)dy, (const bf16*)skip, (const bf16*)mask, (bf16*)dx, N, H, W, C);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_up2_grad_mask(const void* dy, const void* mask, void* dx, int32_t N, int32_t h, int32_t w, int32_t C,
+                         void* stream) {
+  if (!dy || !dx || N < 1 || h < 1 || w < 1 || C < 8 || C % 8) return -1;
+  hipLaunchKernelGGL(k_up2_grad_mask, dim3(G((int64_t)N * h * w * (C / 8))), dim3(256), 0, S(stream),
+                     (const bf16*)dy, (const bf16*)mask, (bf16*)dx, N, h, w, C);
   SWENN_LAUNCH_CHECK();
 }
 

@@ -15,6 +15,7 @@ _I64 = ctypes.c_int64
 _F = ctypes.c_float
 SIGNATURES = {
     "swe_nn_conv3x3": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _I32, _P, _P, _P]),
+    "swe_nn_conv3x3_masked": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _P, _P, _P]),
     "swe_nn_flip": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "swe_nn_wgrad_work": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "swe_nn_wgrad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
@@ -25,6 +26,9 @@ SIGNATURES = {
     "swe_nn_pool2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_up2": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_up2_grad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_up2_add": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_pool2_grad_add": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
+    "swe_nn_up2_grad_mask": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
     "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
     "swe_nn_features": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _F, _F, _P, _P]),
@@ -93,6 +97,25 @@ def conv3x3(x, w, bias=None, relu=False, out16=None, out32=None):
     return out16 if out16 is not None else out32
 
 
+def conv3x3_masked(x, w, mask, out16, bias=None):
+    """out16 [N,H,W,Cout] bf16 = bf16(conv3x3(x, w) + bias), zero where mask
+    [N,H,W,Cout] bf16 <= 0 (mask optional): a data gradient through a ReLU."""
+    import torch
+
+    _dense(x, torch.bfloat16, "x")
+    _dense(w, torch.bfloat16, "w")
+    _dense(out16, torch.bfloat16, "out16")
+    N, H, W, Cin = x.shape
+    Cout = w.shape[0]
+    if w.shape[1] != 9 or w.shape[2] != Cin or tuple(out16.shape) != (N, H, W, Cout):
+        raise ValueError("weights must be [Cout, 9, Cin] and out16 [N, H, W, Cout]")
+    if mask is not None and (mask.shape != out16.shape or mask.dtype != torch.bfloat16):
+        raise ValueError("mask must be a bf16 tensor shaped like out16")
+    _ck(load().swe_nn_conv3x3_masked(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, _p(mask), _p(out16), _stream()),
+        "swe_nn_conv3x3_masked")
+    return out16
+
+
 def flip(w):
     """[Cout,9,Cin] -> [Cin,9,Cout] weights of the data gradient."""
     import torch
@@ -153,6 +176,28 @@ def up2(x, y):
 def up2_grad(dy, dx):
     N, h, w, C = dx.shape
     _ck(load().swe_nn_up2_grad(_p(dy), _p(dx), N, h, w, C, _stream()), "swe_nn_up2_grad")
+    return dx
+
+
+def up2_add(x, skip, out):
+    """out [N,2h,2w,C] = up2(x [N,h,w,C]) + skip (bf16)."""
+    N, h, w, C = x.shape
+    _ck(load().swe_nn_up2_add(_p(x), _p(skip), _p(out), N, h, w, C, _stream()), "swe_nn_up2_add")
+    return out
+
+
+def pool2_grad_add(dy, skip, mask, dx):
+    """dx [N,H,W,C] = pool2_grad(dy) + skip, zero where mask <= 0 (mask optional)."""
+    N, H, W, C = dx.shape
+    _ck(load().swe_nn_pool2_grad_add(_p(dy), _p(skip), _p(mask), _p(dx), N, H, W, C, _stream()),
+        "swe_nn_pool2_grad_add")
+    return dx
+
+
+def up2_grad_mask(dy, mask, dx):
+    """dx [N,h,w,C] = up2_grad(dy), zero where mask <= 0 (mask optional)."""
+    N, h, w, C = dx.shape
+    _ck(load().swe_nn_up2_grad_mask(_p(dy), _p(mask), _p(dx), N, h, w, C, _stream()), "swe_nn_up2_grad_mask")
     return dx
 
 

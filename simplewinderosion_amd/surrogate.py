@@ -172,7 +172,7 @@ class Surrogate:
         b = conv("midb", conv("mida", x))
         for l in range(L - 2, -1, -1):
             hl, wl = H >> l, W >> l
-            c = nn.add(nn.up2(b, self._buf((N, hl, wl, CH[l]))), A["enc%db" % l], self._buf((N, hl, wl, CH[l])))
+            c = nn.up2_add(b, A["enc%db" % l], self._buf((N, hl, wl, CH[l])))
             A["cat%d" % l] = c
             b = conv("dec%db" % l, conv("dec%da" % l, c), out32=(l == 0))
         return b, A
@@ -184,42 +184,38 @@ class Surrogate:
         N, H, W, _ = dout.shape
         L = len(CH)
 
-        def layer(name, x_in, g16, need_dx=True):
+        def layer(name, x_in, g16, mask=None, need_dx=True):
             """g16: bf16 gradient by this layer's pre-activation output;
-            returns the float gradient by its input."""
+            returns the bf16 gradient by its input, zero where mask (the
+            ReLU output that is this layer's input) is not positive."""
             self._work = nn.wgrad(x_in, g16, self.gw[name], self.gb[name], self._work)
             if not need_dx:
                 return None
-            dx = self._buf(x_in.shape[:3] + (self.w16[name].shape[2],), torch.float32)
-            nn.conv3x3(g16, nn.flip(self.w16[name]), None, out32=dx)
-            return dx
-
-        def to16(g32, y=None):
-            return nn.relu_grad(g32, y, self._buf(g32.shape))
+            dx = self._buf(x_in.shape[:3] + (self.w16[name].shape[2],))
+            return nn.conv3x3_masked(g16, nn.flip(self.w16[name]), mask, dx)
 
         skip = {}
-        g = to16(dout)  # the last layer is linear
+        g = nn.relu_grad(dout, None, self._buf(dout.shape))  # the last layer is linear
         for l in range(L - 1):
             # dec_l_b (its input is dec_l_a's ReLU output), dec_l_a (input cat_l)
-            g = to16(layer("dec%db" % l, A["dec%da" % l], g), A["dec%da" % l])
-            dcat = to16(layer("dec%da" % l, A["cat%d" % l], g))
+            g = layer("dec%db" % l, A["dec%da" % l], g, mask=A["dec%da" % l])
+            dcat = layer("dec%da" % l, A["cat%d" % l], g)
             skip[l] = dcat
             hl, wl = H >> (l + 1), W >> (l + 1)
             below = "midb" if l == L - 2 else "dec%db" % (l + 1)
-            g = nn.relu_mask(nn.up2_grad(dcat, self._buf((N, hl, wl, CH[l]))), A[below])
+            g = nn.up2_grad_mask(dcat, A[below], self._buf((N, hl, wl, CH[l])))
         # bottom
-        g = to16(layer("midb", A["mida"], g), A["mida"])
-        dpool = to16(layer("mida", A["pool%d" % (L - 2)], g))
+        g = layer("midb", A["mida"], g, mask=A["mida"])
+        dpool = layer("mida", A["pool%d" % (L - 2)], g)
         for l in range(L - 2, -1, -1):
             hl, wl = H >> l, W >> l
-            ds = nn.add(nn.pool2_grad(dpool, self._buf((N, hl, wl, CH[l]))), skip[l], self._buf((N, hl, wl, CH[l])))
-            nn.relu_mask(ds, A["enc%db" % l])
-            g = to16(layer("enc%db" % l, A["enc%da" % l], ds), A["enc%da" % l])
+            ds = nn.pool2_grad_add(dpool, skip[l], A["enc%db" % l], self._buf((N, hl, wl, CH[l])))
+            g = layer("enc%db" % l, A["enc%da" % l], ds, mask=A["enc%da" % l])
             x_in = A["x0"] if l == 0 else A["pool%d" % (l - 1)]
             if l == 0:
                 layer("enc0a", x_in, g, need_dx=False)
             else:
-                dpool = to16(layer("enc%da" % l, x_in, g))
+                dpool = layer("enc%da" % l, x_in, g)
 
     def adam(self, lr):
         self.step += 1

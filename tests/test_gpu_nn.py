@@ -85,6 +85,86 @@ def test_conv3x3_gradients(Cin, Cout, N, H, W):
     assert torch.allclose(db.cpu().double(), br.grad, atol=2e-3, rtol=1e-4)
 
 
+@pytest.mark.parametrize("N,H,W,Cin,Cout", [
+    (1, 3, 32, 8, 16),      # 32 x 32 blocks, one chunk, channels padded to the block
+    (2, 9, 64, 32, 32),     # two strips a row, several chunks a strip, a short last chunk
+    (1, 20, 32, 64, 72),    # 64 x 64 blocks, a partial output-channel block
+    (1, 1, 32, 16, 8),      # one row: the ring's three rows are the same row
+    (1, 17, 96, 40, 64),    # 64 (co) x 32 (ci) blocks, three strips a row
+    (2, 2, 32, 72, 24),     # 32 (co) x 64 (ci) blocks, a partial input-channel block
+    (8, 64, 64, 128, 128),  # a training layer's shape (four 64 x 64 blocks)
+])
+def test_wgrad_row_strips(N, H, W, Cin, Cout):
+    """The weight and bias gradients of maps whose rows are whole 32-pixel
+    strips (all nine taps of a channel block per workgroup, rows streamed
+    through an LDS ring): against float64 autograd, and the same bits on a
+    second call."""
+    print("case wgrad rows", N, H, W, Cin, Cout, flush=True)
+    x = rnd((N, H, W, Cin), seed=7)
+    dy = rnd((N, H, W, Cout), seed=8)
+    w = rnd((Cout, 9, Cin), seed=9, scale=0.2)
+    dw = torch.empty((Cout, 9, Cin), device=DEV)
+    db = torch.empty((Cout,), device=DEV)
+    work = nn.wgrad(x, dy, dw, db)
+    dw2 = torch.empty_like(dw)
+    db2 = torch.empty_like(db)
+    nn.wgrad(x, dy, dw2, db2, work)
+    torch.cuda.synchronize()
+    assert torch.equal(dw, dw2) and torch.equal(db, db2)
+    xr = x.cpu().double().requires_grad_(True)
+    wr = w.cpu().double().requires_grad_(True)
+    br = torch.zeros(Cout, dtype=torch.float64, requires_grad=True)
+    ref_conv_grad(xr, wr, br).backward(dy.cpu().double())
+    tol = 1e-5 * (N * H * W) ** 0.5 + 1e-3
+    assert torch.allclose(dw.cpu().double(), wr.grad, atol=tol, rtol=1e-4)
+    assert torch.allclose(db.cpu().double(), br.grad, atol=tol, rtol=1e-4)
+
+
+@pytest.mark.parametrize("Cin,Cout,N,H,W", [(16, 32, 2, 10, 70), (64, 8, 1, 33, 40), (32, 64, 2, 16, 32)])
+def test_masked_conv_matches_unfused(Cin, Cout, N, H, W):
+    """The data gradient through a ReLU in one pass: the same bits as the
+    float convolution followed by the ReLU gradient's conversion."""
+    print("case masked conv", Cin, Cout, N, H, W, flush=True)
+    x = rnd((N, H, W, Cin), seed=10)
+    w = rnd((Cout, 9, Cin), seed=11, scale=0.2)
+    mask = rnd((N, H, W, Cout), seed=12)
+    y32 = torch.empty((N, H, W, Cout), device=DEV)
+    nn.conv3x3(x, w, None, out32=y32)
+    for m in (mask, None):
+        want = nn.relu_grad(y32, m, torch.empty((N, H, W, Cout), device=DEV, dtype=torch.bfloat16))
+        got = nn.conv3x3_masked(x, w, m, torch.empty((N, H, W, Cout), device=DEV, dtype=torch.bfloat16))
+        torch.cuda.synchronize()
+        assert torch.equal(got, want)
+
+
+def test_fused_skip_passes_match_unfused():
+    """up2 + add, pool2_grad + add + ReLU mask and up2_grad + ReLU mask in
+    one pass each: the same bits as the separate kernels."""
+    N, h, w, C = 2, 6, 10, 24
+    bf = torch.bfloat16
+
+    def empty(*shape):
+        return torch.empty(shape, device=DEV, dtype=bf)
+
+    lo = rnd((N, h, w, C), seed=13)
+    hi = rnd((N, 2 * h, 2 * w, C), seed=14)
+    hi2 = rnd((N, 2 * h, 2 * w, C), seed=15)
+    mlo = rnd((N, h, w, C), seed=16)
+    want = nn.add(nn.up2(lo, empty(N, 2 * h, 2 * w, C)), hi, empty(N, 2 * h, 2 * w, C))
+    assert torch.equal(nn.up2_add(lo, hi, empty(N, 2 * h, 2 * w, C)), want)
+    for m in (hi2, None):
+        want = nn.add(nn.pool2_grad(lo, empty(N, 2 * h, 2 * w, C)), hi, empty(N, 2 * h, 2 * w, C))
+        if m is not None:
+            nn.relu_mask(want, m)
+        assert torch.equal(nn.pool2_grad_add(lo, hi, m, empty(N, 2 * h, 2 * w, C)), want)
+    for m in (mlo, None):
+        want = nn.up2_grad(hi, empty(N, h, w, C))
+        if m is not None:
+            nn.relu_mask(want, m)
+        assert torch.equal(nn.up2_grad_mask(hi, m, empty(N, h, w, C)), want)
+    torch.cuda.synchronize()
+
+
 def ref_conv_grad(x, w, b):
     Cout, _, Cin = w.shape
     wc = w.view(Cout, 3, 3, Cin).permute(0, 3, 1, 2)
