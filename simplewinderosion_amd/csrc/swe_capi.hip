@@ -106,6 +106,10 @@ struct swe_ctx {
   std::vector<char> tseed_set, tps_set;
   int32_t ntiles = 0;
   swe::Terrain* dterr = nullptr;
+  // colour classes of a batch's terrains (swe::colour_prefix), periodic
+  // table then bounded, on the device and the host
+  int64_t* dcolour = nullptr;
+  std::vector<int64_t> hcolour;
   int32_t* dtile_terrain = nullptr;
 
   // tile-owned step path
@@ -640,7 +644,14 @@ int erode_tiled(swe_ctx* c, const int64_t* particles, int32_t n_cycles, int64_t 
           ra.sm_out = layers2 ? c->smv[nxt] : nullptr;
           ra.ss_out = layers2 ? c->ss[nxt] : nullptr;
           if ((rr = timed_begin(c, &e0, &e1))) return rr;
-          swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, false, c->stream);
+          if (c->nterr > 1) {
+            const int64_t per = (int64_t)swe::kColourAxisMax * swe::kColourAxisMax * (c->nterr + 1);
+            const int64_t off = c->p.periodic ? 0 : per;
+            swe::launch_relax_colour_batch(ra, c->dterr, c->dcolour + off, c->hcolour.data() + off, c->nterr,
+                                           c->ncell, c->p.periodic != 0, c->p.layers, c->stream);
+          } else {
+            swe::launch_relax_colour(ra, c->p.periodic != 0, c->p.layers, false, c->stream);
+          }
           // a cycle ends with no events pending, as after a relax-only window
           if (w == nwin - 1) swe::launch_zero32(c->fV[nxt], c->ncell, c->stream);
           HIPCHK(c, hipGetLastError());
@@ -780,6 +791,7 @@ int swe_destroy(swe_ctx* c) {
   dfree(c->dstat);
   dfree(c->dcycle);
   dfree(c->dterr);
+  dfree(c->dcolour);
   dfree(c->dtile_terrain);
   for (int k = 0; k < 2; k++)
     if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
@@ -836,6 +848,18 @@ int swe_create_batch(swe_ctx** out, int32_t n_terrains, const int32_t* nx, const
       (r = ensure_layers(c))) {
     swe_destroy(c);
     return r;
+  }
+  if (n_terrains > 1) {
+    const int64_t per = (int64_t)swe::kColourAxisMax * swe::kColourAxisMax * (n_terrains + 1);
+    c->hcolour.assign((size_t)(2 * per), 0);
+    swe::colour_prefix(nx, ny, n_terrains, true, c->hcolour.data());
+    swe::colour_prefix(nx, ny, n_terrains, false, c->hcolour.data() + per);
+    if ((r = dalloc(c, &c->dcolour, 2 * per)) ||
+        hipMemcpy(c->dcolour, c->hcolour.data(), sizeof(int64_t) * (size_t)(2 * per), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      swe_destroy(c);
+      return r ? r : SWE_EHIP;
+    }
   }
   *out = c;
   return 0;
@@ -1048,8 +1072,6 @@ int swe_erode_batch(swe_ctx* c, const int64_t* particles, int32_t n_cycles) {
   if ((r = ensure_layers(c))) return r;
   int32_t R;
   const int64_t nwin = windows_of(c->p, &R);
-  const bool colour = colour_mode(c->p);
-  if (colour && c->nterr > 1) return fail(c, SWE_EINVAL, "the colour-ordered cascade runs on single-map contexts");
   const bool tiled = c->path == 2 || (c->path == 0 && R == 1) || c->nterr > 1;
   if (tiled && R != 1) return fail(c, SWE_EINVAL, "the tile-owned step path (and batches) need window == 1");
   if (tiled) return erode_tiled(c, particles, n_cycles, nwin);

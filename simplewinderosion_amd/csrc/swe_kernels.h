@@ -74,6 +74,7 @@ __host__ __device__ inline int32_t colour_cell(int32_t n, bool periodic, int32_t
   (void)periodic;
   return col < 3 ? col + 3 * i : n - n % 3 + (col - 3);
 }
+constexpr int32_t kColourAxisMax = 5;  // classes along an axis: 3 + n % 3 at most
 // Settling factors above this use the colour-ordered cascade when
 // cascade_order is automatic; the synchronous cascade rejects them.
 constexpr float kSettlingMaxSync = 0.25f;
@@ -85,6 +86,17 @@ void launch_relax(const RelaxArgs& a, bool periodic, int layers, hipStream_t s);
 // then one launch per colour class on zm_out / zs_out (sm_out / ss_out) with
 // the events V, and the sediment layer clamped at zero
 void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, bool begin, hipStream_t s);
+// the same relax over all terrains of a batch (masters already hold the
+// window's pickups): class (cx, cy) of every terrain in one launch, classes
+// x-major over a 5 x 5 grid (each terrain sees its own classes in its own
+// order), then the sediment clamp over all ncell cells.  pre: the
+// colour_prefix table on the device and on the host.
+struct Terrain;
+void launch_relax_colour_batch(const RelaxArgs& a, const Terrain* terr, const int64_t* pre_dev,
+                               const int64_t* pre_host, int32_t nterr, int64_t ncell, bool periodic, int layers,
+                               hipStream_t s);
+// pre[(cx * 5 + cy) * (nterr + 1) + k]: cells of class (cx, cy) on terrains 0 .. k-1
+void colour_prefix(const int32_t* nx, const int32_t* ny, int32_t nterr, bool periodic, int64_t* pre);
 void launch_zero64(int64_t* p, int64_t n, hipStream_t s);
 void launch_zero32(uint32_t* p, int64_t n, hipStream_t s);
 void launch_from_f32(const float* in, int64_t* master, float* snap, int64_t n, hipStream_t s);

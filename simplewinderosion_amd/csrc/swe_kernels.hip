@@ -21,6 +21,7 @@
 //              P1 float4 (speed.x, speed.y, speed.z, age bits),
 //              ledger int64 (fixed-point load), lastc int32.
 #include "swe_kernels.h"
+#include "swe_step.h"
 
 #include "swe_cascade.hpp"
 #include "swe_fly.hpp"
@@ -324,14 +325,12 @@ __device__ __forceinline__ int64_t colour_neighbour(int32_t x, int32_t y, int k,
   return (int64_t)qx * ny + qy;
 }
 
+// cell (x, y) of a map of nx x ny cells starting at cell `base` of the
+// context's arrays
 template <bool PERIODIC, int LAYERS>
-__global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, int32_t nj) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)ni * nj) return;
-  const int32_t nx = a.nx, ny = a.ny;
-  const int32_t x = colour_cell(nx, PERIODIC, cx, (int32_t)(t / nj));
-  const int32_t y = colour_cell(ny, PERIODIC, cy, (int32_t)(t % nj));
-  const int64_t c = (int64_t)x * ny + y;
+__device__ __forceinline__ void colour_relax_cell(const RelaxArgs& a, int64_t base, int32_t nx, int32_t ny, int32_t x,
+                                                  int32_t y) {
+  const int64_t c = base + (int64_t)x * ny + y;
   const uint32_t v = a.V[c];
   if (v == 0) return;
   int64_t zn[8], snm[8];
@@ -340,7 +339,7 @@ __global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, in
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     bool ok;
-    const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+    const int64_t q = base + colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
     hn[k] = ok ? a.zs_out[q] : 0.0f;
     zn[k] = ok ? a.zm_out[q] : 0;
     sn[k] = (LAYERS == 2 && ok) ? a.ss_out[q] : 0.0f;
@@ -367,7 +366,7 @@ __global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, in
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       bool ok;
-      const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+      const int64_t q = base + colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
       if (!ok || d[k] == 0) continue;
       a.zm_out[q] += d[k];
       if (LAYERS == 2) a.sm_out[q] += d[k];
@@ -377,7 +376,7 @@ __global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, in
 #pragma unroll
     for (int k = 0; k < 8; k++) {
       bool ok;
-      const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+      const int64_t q = base + colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
       if (!ok) continue;
       a.zs_out[q] = f32_from_fx(a.zm_out[q]);
       if (LAYERS == 2) a.ss_out[q] = f32_from_fx(a.sm_out[q]);
@@ -390,7 +389,7 @@ __global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, in
   for (int k = 0; k < 8; k++) {
     if (d[k] == 0) continue;
     bool ok;
-    const int64_t q = colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
+    const int64_t q = base + colour_neighbour<PERIODIC>(x, y, k, nx, ny, ok);
     const int64_t z = zn[k] + d[k];
     a.zm_out[q] = z;
     a.zs_out[q] = f32_from_fx(z);
@@ -408,6 +407,40 @@ __global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, in
       a.ss_out[c] = f32_from_fx(scm + centre);
     }
   }
+}
+
+template <bool PERIODIC, int LAYERS>
+__global__ __launch_bounds__(256) void k_colour_pass(RelaxArgs a, int32_t cx, int32_t cy, int32_t ni, int32_t nj) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ni * nj) return;
+  const int32_t x = colour_cell(a.nx, PERIODIC, cx, (int32_t)(t / nj));
+  const int32_t y = colour_cell(a.ny, PERIODIC, cy, (int32_t)(t % nj));
+  colour_relax_cell<PERIODIC, LAYERS>(a, 0, a.nx, a.ny, x, y);
+}
+
+// The same class (cx, cy) on every terrain of a batch at once (terrains
+// share no cell, so their classes run side by side; a terrain without the
+// class has none of the launch's cells).  pre[k] = the class's cells on
+// terrains 0 .. k-1; thread t takes the terrain with pre[k] <= t < pre[k+1].
+template <bool PERIODIC, int LAYERS>
+__global__ __launch_bounds__(256) void k_colour_pass_batch(RelaxArgs a, const Terrain* __restrict__ terr,
+                                                           const int64_t* __restrict__ pre, int32_t nterr, int32_t cx,
+                                                           int32_t cy) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= pre[nterr]) return;
+  int32_t lo = 0, hi = nterr - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int64_t base = terr[lo].cell0;
+  const int32_t nx = terr[lo].nx, ny = terr[lo].ny;
+  const int64_t u = t - pre[lo];
+  const int32_t nj = colour_count(ny, PERIODIC, cy);
+  const int32_t x = colour_cell(nx, PERIODIC, cx, (int32_t)(u / nj));
+  const int32_t y = colour_cell(ny, PERIODIC, cy, (int32_t)(u % nj));
+  colour_relax_cell<PERIODIC, LAYERS>(a, base, nx, ny, x, y);
 }
 
 // the sediment layer clamped at zero after the last class
@@ -620,6 +653,41 @@ void launch_relax_colour(const RelaxArgs& a, bool periodic, int layers, bool beg
       }
     }
   if (layers == 2) hipLaunchKernelGGL(k_colour_end, dim3(grid_for(n, 256)), dim3(256), 0, s, a, n);
+}
+
+void launch_relax_colour_batch(const RelaxArgs& a, const Terrain* terr, const int64_t* pre_dev,
+                               const int64_t* pre_host, int32_t nterr, int64_t ncell, bool periodic, int layers,
+                               hipStream_t s) {
+  for (int32_t cx = 0; cx < kColourAxisMax; cx++)
+    for (int32_t cy = 0; cy < kColourAxisMax; cy++) {
+      const int32_t cls = cx * kColourAxisMax + cy;
+      const int64_t m = pre_host[(int64_t)cls * (nterr + 1) + nterr];
+      if (m == 0) continue;
+      const int64_t* pre = pre_dev + (int64_t)cls * (nterr + 1);
+      const dim3 g((unsigned)((m + 255) / 256));
+      if (periodic) {
+        if (layers == 2) hipLaunchKernelGGL((k_colour_pass_batch<true, 2>), g, dim3(256), 0, s, a, terr, pre, nterr, cx, cy);
+        else hipLaunchKernelGGL((k_colour_pass_batch<true, 1>), g, dim3(256), 0, s, a, terr, pre, nterr, cx, cy);
+      } else {
+        if (layers == 2) hipLaunchKernelGGL((k_colour_pass_batch<false, 2>), g, dim3(256), 0, s, a, terr, pre, nterr, cx, cy);
+        else hipLaunchKernelGGL((k_colour_pass_batch<false, 1>), g, dim3(256), 0, s, a, terr, pre, nterr, cx, cy);
+      }
+    }
+  if (layers == 2) hipLaunchKernelGGL(k_colour_end, dim3(grid_for(ncell, 256)), dim3(256), 0, s, a, ncell);
+}
+
+void colour_prefix(const int32_t* nx, const int32_t* ny, int32_t nterr, bool periodic, int64_t* pre) {
+  for (int32_t cx = 0; cx < kColourAxisMax; cx++)
+    for (int32_t cy = 0; cy < kColourAxisMax; cy++) {
+      int64_t* p = pre + (int64_t)(cx * kColourAxisMax + cy) * (nterr + 1);
+      p[0] = 0;
+      for (int32_t k = 0; k < nterr; k++) {
+        const int64_t n = (cx < colour_classes(nx[k], periodic) && cy < colour_classes(ny[k], periodic))
+                              ? (int64_t)colour_count(nx[k], periodic, cx) * colour_count(ny[k], periodic, cy)
+                              : 0;
+        p[k + 1] = p[k] + n;
+      }
+    }
 }
 
 void launch_zero64(int64_t* p, int64_t n, hipStream_t s) {

@@ -303,14 +303,15 @@ class Batch:
     seed and prevailing wind."""
 
     def __init__(self, sizes, device=0, periodic=True, track=True, layers=1, abrasion=0.0,
-                 deposit_on_death=False, mapscale=80.0, spawn=0):
+                 deposit_on_death=False, mapscale=80.0, spawn=0, cascade_order=-1):
         self.lib = capi.load()
         self.sizes = [(int(a), int(b)) for a, b in sizes]
         n = len(self.sizes)
         nx = (ctypes.c_int32 * n)(*[a for a, _ in self.sizes])
         ny = (ctypes.c_int32 * n)(*[b for _, b in self.sizes])
         self.opts = dict(abrasion=abrasion, mapscale=mapscale, window=1, periodic=periodic, track=track,
-                         layers=layers, deposit_on_death=deposit_on_death, spawn=spawn)
+                         layers=layers, deposit_on_death=deposit_on_death, spawn=spawn,
+                         cascade_order=cascade_order)
         self.ctx = ctypes.c_void_p()
         rc = self.lib.swe_create_batch(ctypes.byref(self.ctx), n, nx, ny, int(device))
         if rc != 0:
@@ -432,7 +433,7 @@ class Batch:
         _statics_from(p)
         self.opts = dict(abrasion=p.abrasion, mapscale=p.mapscale, window=1, periodic=bool(p.periodic),
                          track=bool(p.track), layers=p.layers, deposit_on_death=bool(p.deposit_on_death),
-                         spawn=p.spawn)
+                         spawn=p.spawn, cascade_order=p.cascade_order)
 
     def export(self, k, path, fmt="npz"):
         """World.export for terrain k."""

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from simplewinderosion_amd import Batch, World, Wind
+from simplewinderosion_amd import Batch, World, Wind, glm_normalize
 from simplewinderosion_amd.world import SweError
 
 from test_gpu_parity import assert_same, run_pair, say
@@ -86,8 +86,51 @@ def test_synchronous_cascade_refuses_large_settling(statics):
     World.settling = 1.0
     with pytest.raises(SweError):
         World(16, 16, cascade_order=0)
-    # batches need the synchronous cascade
-    b = Batch([(32, 32), (16, 16)], periodic=True)
     with pytest.raises(SweError):
-        b.erode([100, 50])
+        Batch([(32, 32), (16, 16)], periodic=True, cascade_order=0)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_colour_cascade_batch_equals_single_runs(statics, periodic):
+    """A batch at the reference's settling (1.0, automatic colour order):
+    each class runs on every terrain in one launch.  Every terrain equals
+    the same terrain eroded alone (sides of every residue mod 3, so the
+    terrains have different class grids, and one 2-cell-wide terrain whose
+    neighbourhoods alias), and one terrain equals the oracle."""
+    sizes = [(31, 29), (40, 17), (33, 64), (2, 7), (10, 10)]
+    say("case colour batch", periodic, sizes)
+    Wind.suspension = 1e-3
+    World.settling = 1.0
+    layers = 2 if periodic else 1
+    rng = np.random.default_rng(5)
+    winds = [glm_normalize((np.cos(a), 0.2, np.sin(a))) for a in rng.uniform(0, 6.28, len(sizes))]
+    seeds = [101, 202, 303, 404, 505]
+    heights = [(rng.random(sz) * 0.4).astype(np.float32) for sz in sizes]
+    seds = [np.full(sz, 0.004, np.float32) for sz in sizes]
+    parts = [int(0.8 * a * b) for a, b in sizes]
+    b = Batch(sizes, periodic=periodic, layers=layers, abrasion=1e-3 if layers == 2 else 0.0)
+    assert b.params().cascade_order == -1
+    for k in range(len(sizes)):
+        b.set_terrain(k, seeds[k], winds[k])
+        b.upload(k, heights[k], seds[k] if layers == 2 else None)
+    b.erode(parts, n_cycles=2)
+    for k, (nx, ny) in enumerate(sizes):
+        World.SEED = seeds[k]
+        Wind.pspeed = winds[k]
+        w = World(nx, ny, periodic=periodic, layers=layers, abrasion=1e-3 if layers == 2 else 0.0)
+        w.upload(heights[k], seds[k] if layers == 2 else None)
+        w.erode(parts[k], n_cycles=2)
+        zb, sb = b.master(k)
+        zw, sw = w.master()
+        assert np.array_equal(zb, zw), k
+        if layers == 2:
+            assert np.array_equal(sb, sw), k
+        assert b.audit(k)["lost"] == w.audit()["lost"], k
+        if k == 0:
+            st = O.OracleState(nx, ny, heights[k], sediment=seds[k] if layers == 2 else None,
+                               params=O.params_from(w.params()))
+            assert O.load().or_colour_mode(st.p) == 1
+            st.erode(parts[k], 2)
+            assert np.array_equal(zw.ravel(), st.zm)
+        w.close()
     b.close()
