@@ -65,6 +65,24 @@ int swe_nn_mse(const float* pred, const float* target, float* grad, float* part,
 /* Adam on float master weights w (+ bf16 copy w16, optional); step >= 1 */
 int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                 float eps, int32_t step, void* stream);
+/* One segment of swe_nn_adam_multi: n float master values w (elements off ..
+ * off + n - 1 of the concatenation), their gradient g and moments m, v; the
+ * bf16 copy w16 (optional); for convolution weights [cout][9][cin], the
+ * flipped bf16 copy wt16 [cin][9][cout] of swe_nn_flip (optional). */
+typedef struct {
+  float* w;
+  void* w16;
+  void* wt16;
+  const float* g;
+  float* m;
+  float* v;
+  int64_t off, n;
+  int32_t cin, cout;
+} swe_nn_adam_seg;
+/* Adam (swe_nn_adam's arithmetic) over every segment in one launch; seg is
+ * a device array of nseg segments in order of off, total = sum of n. */
+int swe_nn_adam_multi(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, float lr, float b1, float b2,
+                      float eps, int32_t step, void* stream);
 /* Surrogate input features x [N][H][W][8] bf16 from float maps h, s [N][H][W]
  * (s may be NULL), per-map mean heights and wind azimuths [N]: (h - mean) /
  * h_scale, s / s_scale, cos az, sin az, periodic central differences along

@@ -12,6 +12,8 @@
 
 #include "../../include/swe_nn.h"
 
+static_assert(sizeof(swe_nn_adam_seg) == 72, "swe_nn_adam_seg: six pointers, two int64, two int32");
+
 namespace swenn {
 
 typedef __bf16 bf16;
@@ -148,6 +150,139 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
         if (y16) y16[p * Cout + co] = f2bf(v);
         if (y32) y32[p * Cout + co] = v;
       }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution over row strips (W % 32 == 0, Cin <= CI, CI = 32 or 64):
+// a workgroup owns 32 output channels and walks R rows of one 32-pixel
+// column strip of one image.  Its weights [32][9][CI] stay in LDS for the
+// whole strip; input rows y - 1, y, y + 1 (34 pixels with the periodic
+// halo, all input channels) sit in a four-row LDS ring, the next row loaded
+// while the current one is computed, so every input value is read from
+// memory once per strip and serves all nine taps.  NHWC rows are the MFMA's
+// A layout (lane: one pixel, 8 consecutive channels) and the weights' its B
+// layout, so both are copied as 16-byte runs; the tap's shift is the ring
+// row and the pixel row within it.  Waves tile the 32 x 32 output block of a
+// row 2 x 2 (16 pixels x 16 channels each) and write it before the next row.
+// Epilogue as k_conv3x3 (bias; ACT 0 none, 1 ReLU, 2 mask).
+// ---------------------------------------------------------------------------
+template <int CI, int ACT>
+__global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, const bf16* __restrict__ w,
+                                                   const float* __restrict__ bias, const bf16* __restrict__ mask,
+                                                   bf16* __restrict__ y16, float* __restrict__ y32, int32_t N,
+                                                   int32_t H, int32_t W, int32_t Cin, int32_t Cout, int32_t R) {
+  constexpr int LDX = CI + 8;          // ring row stride (elements): 16-byte aligned, shifted banks
+  constexpr int LDW = 9 * CI + 8;      // weight row stride
+  constexpr int XG = CI / 8;           // 16-byte groups of a pixel
+  constexpr int XITEMS = 34 * XG;      // loads of one input row
+  constexpr int XPER = (XITEMS + 255) / 256;
+  __shared__ __attribute__((aligned(16))) bf16 s_w[32 * LDW];
+  __shared__ __attribute__((aligned(16))) bf16 s_x[4][34 * LDX];
+  const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int32_t segs = W / 32, cps = (H + R - 1) / R;
+  const int32_t chunk = blockIdx.x;
+  const int32_t rc = chunk % cps, sg = (chunk / cps) % segs, n = chunk / (cps * segs);
+  const int32_t co0 = blockIdx.y * 32;
+  const int32_t y0 = rc * R, rows = R < H - y0 ? R : H - y0, x0 = sg * 32;
+  const int64_t img = (int64_t)n * H;
+  const bf16x8 zero = {};
+
+  // weights of the 32 output channels: [co][tap][ci], channels >= Cin zero
+  for (int32_t u = tid; u < 32 * 9 * XG; u += 256) {
+    const int32_t g = u % XG, t = (u / XG) % 9, col = u / (9 * XG);
+    const int32_t co = co0 + col, ci = 8 * g;
+    bf16x8 v = zero;
+    if (co < Cout && ci < Cin) v = *(const bf16x8*)(w + ((int64_t)co * 9 + t) * Cin + ci);
+    *(bf16x8*)&s_w[col * LDW + t * CI + ci] = v;
+  }
+  // input row loads: items (pixel column c = 0 .. 33, group g)
+  int32_t xc[XPER], xg[XPER];
+  bool xitem[XPER], xok[XPER];
+  int64_t xcol[XPER];
+#pragma unroll
+  for (int h = 0; h < XPER; h++) {
+    const int32_t u = tid + 256 * h;
+    xitem[h] = u < XITEMS;
+    xc[h] = u / XG;
+    xg[h] = u % XG;
+    xok[h] = xitem[h] && 8 * xg[h] < Cin;
+    xcol[h] = (int64_t)wrapi(x0 + xc[h] - 1, W) * Cin + 8 * xg[h];
+  }
+  struct Row {
+    bf16x8 v[XPER];
+  };
+  auto load = [&](int32_t k) __attribute__((always_inline)) {  // image row y0 - 1 + k
+    Row r;
+    const int64_t xrow = (img + wrapi(y0 - 1 + k, H)) * W * Cin;
+#pragma unroll
+    for (int h = 0; h < XPER; h++) r.v[h] = xok[h] ? *(const bf16x8*)(x + xrow + xcol[h]) : zero;
+    return r;
+  };
+  auto store = [&](const Row& r, int32_t slot) __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < XPER; h++)
+      if (xitem[h]) *(bf16x8*)&s_x[slot][xc[h] * LDX + 8 * xg[h]] = r.v[h];
+  };
+  // ring rows k = 0, 1, 2 in LDS; row k = 3 in registers (two rows ahead
+  // of the first step's need)
+  Row ra, rb;
+  {
+    const Row r0 = load(0), r1 = load(1), r2 = load(2);
+    if (rows > 1) ra = load(3);
+    store(r0, 0);
+    store(r1, 1);
+    store(r2, 2);
+  }
+  __syncthreads();
+
+  const int32_t pt = wave & 1, ct = wave >> 1;  // 16 pixels x 16 channels of the 32 x 32 block
+  const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
+  const int32_t co_b = 16 * ct + rl;            // B row (output channel within the block)
+  const int32_t co = co0 + co_b;
+  const float bv = (bias && co < Cout) ? bias[co] : 0.0f;
+  // step s: LDS holds rows s .. s + 2, `held` row s + 3 (needed from step
+  // s + 1 on); row s + 4 is loaded into `next`, and `held` is stored into
+  // the slot of row s - 1, free since the barrier that ended step s - 1
+  auto step = [&](int32_t s, Row& held, Row& next) __attribute__((always_inline)) {
+    if (s + 2 < rows) next = load(s + 4);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int dyi = 0; dyi < 3; dyi++) {
+      const bf16* xr = s_x[(s + dyi) & 3];
+#pragma unroll
+      for (int dxi = 0; dxi < 3; dxi++) {
+        const int32_t t = 3 * dyi + dxi;
+        // A: pixel 16 pt + rl of the row, shifted by dx (column + dxi)
+        const bf16* ap = xr + (16 * pt + rl + dxi) * LDX + kq;
+        const bf16* bp = s_w + co_b * LDW + t * CI + kq;
+#pragma unroll
+        for (int kc = 0; kc < CI / 32; kc++) {
+          const bf16x8 a = *(const bf16x8*)(ap + 32 * kc);
+          const bf16x8 b = *(const bf16x8*)(bp + 32 * kc);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+        }
+      }
+    }
+    // D rows (pixels) 4 (lane / 16) + e of the wave's 16, column = channel co
+    if (co < Cout) {
+      const int64_t prow = (img + y0 + s) * W + x0 + 16 * pt;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int64_t p = prow + 4 * (lane >> 4) + e;
+        float v = acc[e] + bv;
+        if (ACT == 1) v = v > 0.0f ? v : 0.0f;
+        if (ACT == 2) v = bf2f(mask[p * Cout + co]) > 0.0f ? v : 0.0f;
+        if (y16) y16[p * Cout + co] = f2bf(v);
+        if (y32) y32[p * Cout + co] = v;
+      }
+    }
+    if (s + 1 < rows) store(held, (s + 3) & 3);
+    __syncthreads();
+  };
+  for (int32_t s = 0; s < rows; s += 2) {
+    step(s, ra, rb);
+    if (s + 1 < rows) step(s + 1, rb, ra);
   }
 }
 
@@ -735,6 +870,37 @@ __global__ void k_adam(float* __restrict__ w, bf16* __restrict__ w16, const floa
   if (w16) w16[i] = f2bf(wi);
 }
 
+// Adam over many tensors in one launch: element i of the concatenation lies
+// in the segment k with seg[k].off <= i < seg[k].off + seg[k].n (binary
+// search over the nseg offsets).  The arithmetic is k_adam's; a segment
+// with wt16 is a convolution's [Cout][9][Cin] weights whose flipped bf16
+// copy (k_flip's layout) is written beside w16.
+__global__ void k_adam_multi(const swe_nn_adam_seg* __restrict__ seg, int32_t nseg, int64_t total, float lr,
+                             float b1, float b2, float eps, float c1, float c2) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= total) return;
+  int32_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (seg[mid].off <= i) lo = mid;
+    else hi = mid - 1;
+  }
+  const swe_nn_adam_seg& sg = seg[lo];
+  const int64_t j = i - sg.off;
+  const float gi = sg.g[j];
+  const float mi = b1 * sg.m[j] + (1.0f - b1) * gi;
+  const float vi = b2 * sg.v[j] + (1.0f - b2) * gi * gi;
+  sg.m[j] = mi;
+  sg.v[j] = vi;
+  const float wi = sg.w[j] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  sg.w[j] = wi;
+  if (sg.w16) ((bf16*)sg.w16)[j] = f2bf(wi);
+  if (sg.wt16) {
+    const int32_t ci = (int32_t)(j % sg.cin), t = (int32_t)((j / sg.cin) % 9), co = (int32_t)(j / (9 * (int64_t)sg.cin));
+    ((bf16*)sg.wt16)[((int64_t)ci * 9 + (8 - t)) * sg.cout + co] = f2bf(wi);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Input features of the surrogate, computed in float from the float maps and
 // rounded to bf16 only at the end (the map's slopes are far below the bf16
@@ -821,20 +987,61 @@ static inline unsigned G(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 extern "C" {
 
-// output channels per workgroup: the layer's (up to 64), 128 where that
-// still leaves two workgroups per CU of the chip's 256
+// Maps whose rows are whole 32-pixel strips take k_conv_rows for layers of
+// at most 32 input channels, and of at most 64 when the output has at most
+// 2^22 values (measured on the surrogate's layers, MI355X: 32-channel layers
+// at 256^2 45 -> 30 us; with 64 input channels the 57 KB of LDS leave two
+// workgroups per CU and every 32-channel output block re-reads the rows, so
+// a 64 -> 64 layer at 128^2 is faster on k_conv3x3): 32 output channels a
+// workgroup, each strip cut into chunks of R rows, enough chunks for
+// ~SWENN_CONV_WGS workgroups but at most ceil(H / SWENN_CONV_MINROWS) a
+// strip.  Others take k_conv3x3, whose output channels per workgroup are the
+// layer's (up to 64), 128 where that still leaves two workgroups per CU of
+// the chip's 256.
+#ifndef SWENN_CONV_ROWS
+#define SWENN_CONV_ROWS 1
+#endif
+#ifndef SWENN_CONV_WGS
+#define SWENN_CONV_WGS 2048
+#endif
+#ifndef SWENN_CONV_MINROWS
+#define SWENN_CONV_MINROWS 8
+#endif
 static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
                        int32_t Cout, int act, const void* mask, void* y16, float* y32, void* stream) {
   if (!x || !w || N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (!y16 && !y32)) return -1;
   const int64_t P = (int64_t)N * H * W;
-  const int64_t mt = (P + 127) / 128;
-  if (mt > 0x7fffffffLL) return -1;
-  const int32_t nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
-  const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt));
   const bf16* xx = (const bf16*)x;
   const bf16* ww = (const bf16*)w;
   const bf16* mm = (const bf16*)mask;
   bf16* yy = (bf16*)y16;
+  if (SWENN_CONV_ROWS && W % 32 == 0 && (Cin <= 32 || (Cin <= 64 && (int64_t)Cout * P <= ((int64_t)1 << 22)))) {
+    const int64_t strips = (int64_t)N * (W / 32), cob = (Cout + 31) / 32;
+    int64_t cps = (SWENN_CONV_WGS + strips * cob - 1) / (strips * cob);
+    const int64_t cmax = (H + SWENN_CONV_MINROWS - 1) / SWENN_CONV_MINROWS;
+    if (cps > cmax) cps = cmax;
+    if (cps < 1) cps = 1;
+    const int32_t R = (int32_t)((H + cps - 1) / cps);
+    const int64_t chunks = strips * ((H + R - 1) / R);
+    if (chunks > 0x7fffffffLL || cob > 65535) return -1;
+    const dim3 g((unsigned)chunks, (unsigned)cob);
+#define SWENN_CONV_R(CI_, AC)                                                                                   \
+  hipLaunchKernelGGL((k_conv_rows<CI_, AC>), g, dim3(256), 0, S(stream), xx, ww, bias, mm, yy, y32, N, H, W, Cin, \
+                     Cout, R)
+#define SWENN_CONV_R_CI(CI_)                \
+  if (act == 1) SWENN_CONV_R(CI_, 1);      \
+  else if (act == 2) SWENN_CONV_R(CI_, 2); \
+  else SWENN_CONV_R(CI_, 0)
+    if (Cin <= 32) { SWENN_CONV_R_CI(32); }
+    else { SWENN_CONV_R_CI(64); }
+#undef SWENN_CONV_R_CI
+#undef SWENN_CONV_R
+    SWENN_LAUNCH_CHECK();
+  }
+  const int64_t mt = (P + 127) / 128;
+  if (mt > 0x7fffffffLL) return -1;
+  const int32_t nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
+  const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt));
 #define SWENN_CONV(NT_, AC)                                                                                       \
   hipLaunchKernelGGL((k_conv3x3<NT_, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, mm, yy, y32, N, H, W, Cin, \
                      Cout)
@@ -1037,6 +1244,15 @@ int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t
   if (step < 1) return -1;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(G(n)), dim3(256), 0, S(stream), w, (bf16*)w16, g, m, v, n, lr, b1, b2, eps, c1, c2);
+  SWENN_LAUNCH_CHECK();
+}
+
+int swe_nn_adam_multi(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, float lr, float b1, float b2,
+                      float eps, int32_t step, void* stream) {
+  if (!seg || nseg < 1 || total < 0 || step < 1) return -1;
+  if (total == 0) return 0;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam_multi, dim3(G(total)), dim3(256), 0, S(stream), seg, nseg, total, lr, b1, b2, eps, c1, c2);
   SWENN_LAUNCH_CHECK();
 }
 

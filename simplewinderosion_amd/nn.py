@@ -31,11 +31,18 @@ SIGNATURES = {
     "swe_nn_up2_grad_mask": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
     "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
+    "swe_nn_adam_multi": (ctypes.c_int, [_P, _I32, _I64, _F, _F, _F, _F, _I32, _P]),
     "swe_nn_features": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _F, _F, _P, _P]),
     "swe_nn_project_mass": (ctypes.c_int, [_P, _I64, _F, _I64, _P, _P, _P]),
 }
 
 _lib = None
+
+
+class AdamSeg(ctypes.Structure):
+    """swe_nn_adam_seg (include/swe_nn.h)."""
+    _fields_ = [("w", _P), ("w16", _P), ("wt16", _P), ("g", _P), ("m", _P), ("v", _P), ("off", _I64), ("n", _I64),
+                ("cin", _I32), ("cout", _I32)]
 
 
 def load():
@@ -212,6 +219,33 @@ def mse(pred, target, grad, part):
 def adam(w, w16, g, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     _ck(load().swe_nn_adam(_p(w), _p(w16), _p(g), _p(m), _p(v), w.numel(), lr, b1, b2, eps, int(step), _stream()),
         "swe_nn_adam")
+
+
+def adam_table(segs, device):
+    """Device table of swe_nn_adam_multi segments from (w, w16, wt16, g, m, v,
+    cin, cout) tuples of device tensors (w16, wt16 may be None); valid while
+    those tensors live.  Returns (table, nseg, total)."""
+    import torch
+
+    arr = (AdamSeg * len(segs))()
+    off = 0
+    for k, (w, w16, wt16, g, m, v, cin, cout) in enumerate(segs):
+        n = w.numel()
+        if g.numel() != n or m.numel() != n or v.numel() != n:
+            raise ValueError("segment %d: w, g, m, v must have the same size" % k)
+        if wt16 is not None and (wt16.numel() != n or cin * 9 * cout != n):
+            raise ValueError("segment %d: wt16 needs [cout, 9, cin] weights" % k)
+        arr[k] = AdamSeg(w.data_ptr(), None if w16 is None else w16.data_ptr(),
+                         None if wt16 is None else wt16.data_ptr(), g.data_ptr(), m.data_ptr(), v.data_ptr(),
+                         off, n, int(cin), int(cout))
+        off += n
+    raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+    return raw.to(device), len(segs), off
+
+
+def adam_multi(table, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    buf, nseg, total = table
+    _ck(load().swe_nn_adam_multi(_p(buf), nseg, total, lr, b1, b2, eps, int(step), _stream()), "swe_nn_adam_multi")
 
 
 def project_mass(d, scale, target, master, work):

@@ -67,7 +67,7 @@ class Surrogate:
         torch = _torch()
         self.dev = torch.device(device)
         g = torch.Generator(device="cpu").manual_seed(seed)
-        self.w32, self.w16, self.b32 = {}, {}, {}
+        self.w32, self.w16, self.wt16, self.b32 = {}, {}, {}, {}
         self.gw, self.gb, self.m, self.v = {}, {}, {}, {}
         for name, cin, cout, relu in LAYERS:
             std = math.sqrt(2.0 / (9 * cin)) if relu else 0.1 * math.sqrt(1.0 / (9 * cin))
@@ -76,6 +76,7 @@ class Surrogate:
                 w[:, :, N_IN:] = 0.0  # the padding channels carry no data
             self.w32[name] = w.contiguous()
             self.w16[name] = w.to(torch.bfloat16).contiguous()
+            self.wt16[name] = nn.flip(self.w16[name])  # the data gradient's weights, kept by Adam
             self.b32[name] = torch.zeros(cout, device=self.dev)
             self.gw[name] = torch.zeros_like(self.w32[name])
             self.gb[name] = torch.zeros_like(self.b32[name])
@@ -92,6 +93,7 @@ class Surrogate:
         self.K = 50
         self.step = 0
         self._work = None
+        self._adam = None  # swe_nn_adam_multi table over every parameter tensor
 
     # -- bookkeeping ----------------------------------------------------------
     def n_params(self):
@@ -114,7 +116,9 @@ class Surrogate:
         for n in self.w32:
             self.w32[n] = torch.from_numpy(np.asarray(d["w." + n], np.float32)).to(self.dev).contiguous()
             self.w16[n] = self.w32[n].to(torch.bfloat16).contiguous()
+            self.wt16[n] = nn.flip(self.w16[n])
             self.b32[n] = torch.from_numpy(np.asarray(d["b." + n], np.float32)).to(self.dev).contiguous()
+        self._adam = None
         (self.h_scale, self.s_scale, self.g_scale, self.l_scale, self.d_scale, self.ds_scale,
          k) = (float(x) for x in d["scales"])
         self.K = int(k)
@@ -192,7 +196,7 @@ class Surrogate:
             if not need_dx:
                 return None
             dx = self._buf(x_in.shape[:3] + (self.w16[name].shape[2],))
-            return nn.conv3x3_masked(g16, nn.flip(self.w16[name]), mask, dx)
+            return nn.conv3x3_masked(g16, self.wt16[name], mask, dx)
 
         skip = {}
         g = nn.relu_grad(dout, None, self._buf(dout.shape))  # the last layer is linear
@@ -218,10 +222,18 @@ class Surrogate:
                 dpool = layer("enc%da" % l, x_in, g)
 
     def adam(self, lr):
+        """One Adam step of every weight and bias in one launch (which also
+        keeps the bf16 weights and their flipped copies)."""
         self.step += 1
-        for name in self.w32:
-            nn.adam(self.w32[name], self.w16[name], self.gw[name], self.m[name], self.v[name], self.step, lr=lr)
-            nn.adam(self.b32[name], None, self.gb[name], self.m[name + ".b"], self.v[name + ".b"], self.step, lr=lr)
+        if self._adam is None:
+            segs = []
+            for name, cin, cout, _ in LAYERS:
+                segs.append((self.w32[name], self.w16[name], self.wt16[name], self.gw[name], self.m[name],
+                             self.v[name], cin, cout))
+                segs.append((self.b32[name], None, None, self.gb[name], self.m[name + ".b"], self.v[name + ".b"],
+                             0, 0))
+            self._adam = nn.adam_table(segs, self.dev)
+        nn.adam_multi(self._adam, self.step, lr=lr)
 
     def train_step(self, h, s, az, dh, ds, lr):
         """One Adam step on a batch: heights h, sediment s [N,H,W], wind

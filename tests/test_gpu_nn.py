@@ -41,6 +41,13 @@ def rnd(shape, dtype=torch.bfloat16, seed=0, scale=1.0):
     (3, 4, 16, 64, 128, False),   # narrow rows
     (1, 128, 520, 32, 136, True),  # 128-channel workgroups (many pixel blocks), a partial channel block
     (2, 9, 13, 40, 24, False),     # pixel blocks across rows and images, a partial 32-channel step
+    # row strips (W % 32 == 0, Cin <= 64): weights in LDS, input rows streamed through a ring
+    (2, 9, 64, 32, 32, True),      # two strips a row, several chunks a strip
+    (1, 7, 32, 40, 72, False),     # 64-channel ring, partial output-channel block
+    (1, 1, 32, 16, 8, False),      # one row
+    (1, 2, 64, 64, 24, True),      # a chunk of two rows
+    (1, 11, 96, 24, 40, True),     # three strips a row, odd rows per chunk
+    (8, 64, 64, 64, 64, True),     # a training layer's shape
 ])
 def test_conv3x3_forward(N, H, W, Cin, Cout, relu):
     print("case conv", N, H, W, Cin, Cout, relu, flush=True)
@@ -228,6 +235,40 @@ def test_mse_and_adam():
     wr = w0 - 1e-2 * (mr / 0.1) / (torch.sqrt(vr / 0.001) + 1e-8)
     assert torch.allclose(w, wr, atol=1e-6, rtol=1e-5)
     assert torch.equal(w16, w.to(torch.bfloat16))
+
+
+def test_adam_multi_matches_per_tensor_adam_and_flip():
+    """One launch over several tensors: the same bits as swe_nn_adam per
+    tensor, and the flipped copies as swe_nn_flip of the new bf16 weights."""
+    shapes = [(24, 9, 16), (40,), (8, 9, 32), (3,)]
+    ts = []
+    for k, sh in enumerate(shapes):
+        w = rnd(sh, torch.float32, seed=20 + k)
+        g = rnd(sh, torch.float32, seed=30 + k, scale=0.1)
+        m = rnd(sh, torch.float32, seed=40 + k, scale=0.01)
+        v = rnd(sh, torch.float32, seed=50 + k, scale=0.01).abs()
+        ts.append((w, g, m, v))
+    ref = [tuple(t.clone() for t in x) for x in ts]
+    segs = []
+    outs = []
+    for (w, g, m, v), sh in zip(ts, shapes):
+        conv = len(sh) == 3
+        w16 = torch.empty(sh, device=DEV, dtype=torch.bfloat16) if conv else None
+        wt16 = torch.empty((sh[2], 9, sh[0]), device=DEV, dtype=torch.bfloat16) if conv else None
+        outs.append((w16, wt16))
+        segs.append((w, w16, wt16, g, m, v, sh[2] if conv else 0, sh[0] if conv else 0))
+    table = nn.adam_table(segs, DEV)
+    for step in (1, 2):
+        nn.adam_multi(table, step, lr=1e-2)
+        for (w, g, m, v), sh in zip(ref, shapes):
+            w16 = torch.empty(sh, device=DEV, dtype=torch.bfloat16) if len(sh) == 3 else None
+            nn.adam(w, w16, g, m, v, step=step, lr=1e-2)
+    torch.cuda.synchronize()
+    for (w, g, m, v), (rw, rg, rm, rv), (w16, wt16) in zip(ts, ref, outs):
+        assert torch.equal(w, rw) and torch.equal(m, rm) and torch.equal(v, rv)
+        if w16 is not None:
+            assert torch.equal(w16, w.to(torch.bfloat16))
+            assert torch.equal(wt16, nn.flip(w16))
 
 
 @pytest.mark.parametrize("target", [0, 12345, -987654321])
