@@ -197,8 +197,9 @@ def bench_surrogate(args):
     50), training under the map's symmetries, then a rollout of K = 50
     blocks on the 1024^2 map with its 8
     obstacles, the plain method judging every 10th block (blocks within 1.5x
-    the plain method's own run-to-run RMS accepted) and taking over after a
-    rejection.  value = map-cycles/s of the whole rollout (surrogate blocks,
+    the plain method's own run-to-run RMS from the same state accepted; that
+    RMS measured at every judged state by a second plain run) and taking
+    over after a rejection.  value = map-cycles/s of the whole rollout (surrogate blocks,
     judges and fallback blocks); the plain method's rate beside it."""
     import math
 
@@ -214,7 +215,8 @@ def bench_surrogate(args):
     states, az, t_gen = S.generate_states(args.sur_terrains, args.sur_cycles, seed=args.seed, log=log)
     model = S.Surrogate(seed=args.seed)
     S.fit_scales(model, states, K)
-    curve, t_train = S.train(model, states, az, K, args.sur_steps, batch=8, lr=1e-3, log=log, log_every=250)
+    curve, t_train = S.train(model, states, az, K, args.sur_steps, batch=8, lr=1e-3, log=log, log_every=250,
+                             pushforward=args.sur_pushforward, pushforward_blocks=args.sur_pushforward_blocks)
     fps = 3 * S.Surrogate.flops_forward(8, 256, 256)
     del states
     torch.cuda.empty_cache()
@@ -223,7 +225,7 @@ def bench_surrogate(args):
     ps = workloads.WORKLOADS["surr1024"]["params"]["pspeed"]
     n = workloads.WORKLOADS["surr1024"]["particles"]
     rec = S.rollout(model, world, plain, n, args.sur_blocks, math.atan2(ps[2], ps[0]), judge_every=10,
-                    tol_rms=1e-3, tol_noise=1.5, log=log)
+                    tol_rms=1e-3, tol_noise=1.5, noise_every_judge=not args.sur_first_floor, log=log)
     world.close()
     plain.close()
     rate = args.sur_blocks * K / rec["t_total_s"]
@@ -243,10 +245,14 @@ def bench_surrogate(args):
                        "accepted": sum(1 for j in judged if j["accepted"])},
             "judged_rms": [round(j["rms_err"], 6) for j in judged],
             "no_change_rms": [round(j["rms_change_plain"], 6) for j in judged],
-            "noise_floor_rms": rec["noise_floor_rms"], "tolerance_rms": rec.get("tol_rms_effective"),
+            "noise_floor_rms": [round(j["noise_floor_rms"], 6) if "noise_floor_rms" in j else None for j in judged],
+            "tolerance_rms": [round(j["tol_rms"], 6) for j in judged],
+            "noise_floor_at": "each judged state" if not args.sur_first_floor else "the first judged state",
             "mass_drift_fx": rec["mass_drift_fx"], "mass_exact": rec["mass_exact"],
             "training": {"maps": args.sur_terrains, "cycles_per_map": args.sur_cycles, "steps": args.sur_steps,
-                         "batch": 8, "final_loss": curve[-1][1], "tflops": fps * args.sur_steps / t_train / 1e12,
+                         "batch": 8, "pushforward": args.sur_pushforward,
+                         "pushforward_blocks": args.sur_pushforward_blocks,
+                         "final_loss": curve[-1][1], "tflops": fps * args.sur_steps / t_train / 1e12,
                          "bf16_dense_peak_tflops": 2500.0, "data_erode_seconds": t_gen}}
     print(json.dumps(line), flush=True)
 
@@ -314,6 +320,13 @@ def main():
     ap.add_argument("--sur-cycles", type=int, default=400, help="surrogate: plain cycles per training map")
     ap.add_argument("--sur-steps", type=int, default=12000, help="surrogate: Adam steps")
     ap.add_argument("--sur-blocks", type=int, default=40, help="surrogate: rollout blocks of K = 50 cycles")
+    ap.add_argument("--sur-first-floor", action="store_true",
+                    help="surrogate: measure the plain method's run-to-run RMS at the first judged state only "
+                         "(the earlier protocol) instead of at every judged state")
+    ap.add_argument("--sur-pushforward", type=float, default=0.0,
+                    help="surrogate: share of the later training batches started from the network's own rollout")
+    ap.add_argument("--sur-pushforward-blocks", type=int, default=1,
+                    help="surrogate: longest own rollout (in blocks of K cycles) a pushforward batch starts from")
     ap.add_argument("--timing-stride", type=int, default=16,
                     help="HIP events around every n-th kernel launch in the timed region")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

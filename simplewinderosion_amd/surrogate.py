@@ -342,19 +342,20 @@ def symmetry(x, az, k):
 
 
 def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_every=100, augment=True,
-          pushforward=0.0, pushforward_from=0.5):
+          pushforward=0.0, pushforward_from=0.5, pushforward_blocks=1):
     """Adam on random (terrain, t) pairs of `states` with a cosine-decayed
     learning rate; with `augment`, each batch under a random one of the
     map's eight symmetries (square maps).  With `pushforward` p > 0, after
     the first `pushforward_from` fraction of the steps a batch is, with
-    probability p, a pushforward pair: its input is the network's own
-    prediction K cycles on from a plain state (the state a rollout reaches
-    after a surrogate block) and its target the plain state 2K cycles on, so
-    that the network learns to correct its own errors.  Returns the loss
-    curve and the time of the steps."""
+    probability p, a pushforward pair: its input is the state the network
+    reaches by itself in b blocks of K cycles from a plain state (b uniform
+    in 1 .. pushforward_blocks, as many as the data allows) -- the state a
+    rollout reaches after b surrogate blocks -- and its target the plain
+    state (b + 1) K cycles on, so that the network learns to correct its own
+    drift.  Returns the loss curve and the time of the steps."""
     torch = _torch()
     T, n = states.shape[0] - K, states.shape[1]
-    T2 = states.shape[0] - 2 * K
+    T2 = states.shape[0] - 2 * K  # pushforward needs two blocks of data after t
     g = torch.Generator(device="cpu").manual_seed(seed)
     az = np.asarray(az, np.float64)
     augment = augment and states.shape[-1] == states.shape[-2]
@@ -364,14 +365,21 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
     for st in range(steps):
         push = (pushforward > 0 and T2 > 0 and st >= pushforward_from * steps
                 and float(torch.rand((1,), generator=g)) < pushforward)
-        t = torch.randint(0, T2 if push else T, (batch,), generator=g)
+        nb = 1
+        if push and pushforward_blocks > 1:
+            nbmax = min(int(pushforward_blocks), (states.shape[0] - 1) // K - 1)
+            nb = int(torch.randint(1, nbmax + 1, (1,), generator=g))
+        t = torch.randint(0, (states.shape[0] - (nb + 1) * K) if push else T, (batch,), generator=g)
         k = torch.randint(0, n, (batch,), generator=g)
         a = az[k.numpy()]
         if push:
             xs = states[t, k]
-            dh, ds = model.predict(xs[:, 0].contiguous(), xs[:, 1].contiguous(), a.astype(np.float32))
-            x0 = torch.stack((xs[:, 0] + dh, (xs[:, 1] + ds).clamp_min(0.0)), dim=1)
-            x1 = states[t + 2 * K, k]
+            h, sd = xs[:, 0].contiguous(), xs[:, 1].contiguous()
+            for _ in range(nb):
+                dh, ds = model.predict(h, sd, a.astype(np.float32))
+                h, sd = (h + dh).contiguous(), (sd + ds).clamp_min(0.0).contiguous()
+            x0 = torch.stack((h, sd), dim=1)
+            x1 = states[t + (nb + 1) * K, k]
         else:
             x0, x1 = states[t, k], states[t + K, k]
         if augment:
@@ -413,7 +421,7 @@ def surrogate_block(model, zm, sm, h, s, az, work):
 
 
 def rollout(model, world, plain, particles, blocks, azimuth, judge_every=10, tol_rms=1e-3, noise_floor=True,
-            tol_noise=None, log=None):
+            tol_noise=None, noise_every_judge=False, log=None):
     """Advance `world` (two layers, deposit on death) by `blocks` blocks of
     model.K cycles.  A block is the surrogate's (mass-projected) prediction
     while the last judge passed; every `judge_every`-th block the plain
@@ -423,11 +431,14 @@ def rollout(model, world, plain, particles, blocks, azimuth, judge_every=10, tol
     the surrogate or the plain method.  With noise_floor, the first judged
     block also runs a second plain realisation (other random streams) to give
     the RMS difference of two plain runs; with tol_noise, the tolerance
-    becomes max(tol_rms, tol_noise x that noise floor)."""
+    becomes max(tol_rms, tol_noise x that noise floor).  With
+    noise_every_judge, every judged block measures the floor at its own
+    state (one more plain block per judge) instead of reusing the first."""
     torch = _torch()
     K = model.K
     work = torch.empty(256, device="cuda", dtype=torch.int64)
     log = log or (lambda *_: None)
+    tol_rms0 = tol_rms
     rec = {"blocks": blocks, "K": K, "tol_rms": tol_rms, "judged": [], "surrogate_blocks": 0, "plain_blocks": 0,
            "judge_blocks": 0, "t_surrogate_s": 0.0, "t_plain_s": 0.0, "noise_floor_rms": None}
     zm0, _ = world.master()
@@ -461,12 +472,14 @@ def rollout(model, world, plain, particles, blocks, azimuth, judge_every=10, tol
             rec["judge_blocks"] += 1
             m = {"block": blk, "rms_err": rms((zs - zp) * scale), "rms_change_plain": rms((zp - zm) * scale),
                  "rms_change_surrogate": rms((zs - zm) * scale)}
-            if (noise_floor or tol_noise) and rec["noise_floor_rms"] is None:
+            if (noise_floor or tol_noise) and (rec["noise_floor_rms"] is None or noise_every_judge):
                 zp2, _ = plain_block(zm, sm, cyc + 1_000_003)  # other random streams
                 rec["noise_floor_rms"] = rms((zp2 - zp) * scale)
+                m["noise_floor_rms"] = rec["noise_floor_rms"]
                 if tol_noise:
-                    tol_rms = max(tol_rms, tol_noise * rec["noise_floor_rms"])
+                    tol_rms = max(tol_rms0, tol_noise * rec["noise_floor_rms"])
                 rec["tol_rms_effective"] = tol_rms
+            m["tol_rms"] = tol_rms
             m["accepted"] = m["rms_err"] <= tol_rms
             use_sur = m["accepted"]
             rec["judged"].append(m)
