@@ -34,16 +34,21 @@ struct Terrain {
   float ps[3], nps[3];  // pspeed and normalize(pspeed)
 };
 
-// One particle record: three 16-byte words, so a record moves with three
-// vector loads or stores at immediate offsets from one address
-struct alignas(16) PRec {
-  float4 q0;       // pos.xyz, sediment
-  float4 q1;       // speed.xyz, age bits
+// One particle record: 40 bytes (two 16-byte words and the ledger), moved
+// with three vector loads or stores at immediate offsets from one address
+// (8-byte aligned; global loads and stores of 16 bytes need only 4).  The
+// number of calls a particle has made (its age) is the window index: every
+// live particle makes one call per window from window 0.  A particle whose
+// position left a bounded map (or stopped being finite) is stored with the
+// bits of its last cell in q0.x and a NaN in q0.z: its next call finds no
+// cell, and it dies (and deposits) at that cell.
+typedef float f4a8 __attribute__((ext_vector_type(4), aligned(8)));
+struct PRec {
+  f4a8 q0;         // pos.xyz, sediment (off the map: last cell's bits, -, NaN, sediment)
+  f4a8 q1;         // speed.xyz, particle index bits (RNG stream)
   int64_t ledger;  // fixed-point load
-  int32_t lastc;   // last pre-move cell
-  uint32_t pid;    // particle index (RNG stream)
 };
-static_assert(sizeof(PRec) == 48, "particle record is three 16-byte words");
+static_assert(sizeof(PRec) == 40 && alignof(PRec) == 8, "particle record: 40 bytes");
 
 struct StepArgs {
   // map, window parity in -> out.  A master holds the previous window's

@@ -89,8 +89,7 @@ __global__ __launch_bounds__(256) void k_spawn_place(SpawnArgs a) {
   Particle w;
   spawn(particle_key(cycle_key(T.seed, *a.cycle), (uint64_t)(i - T.part0)), T.nx, T.ny, a.spawn_mode, w);
   const int64_t o = a.sub_base[(int64_t)a.tile[i] * 9 + 4] + a.rank[i];
-  a.out[o] = PRec{make_float4(w.px, w.py, w.pz, w.sed), make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age)), 0, 0,
-                  (uint32_t)i};
+  a.out[o] = PRec{f4a8{w.px, w.py, w.pz, w.sed}, f4a8{w.sx, w.sy, w.sz, u2f((uint32_t)i)}, 0};
 }
 
 // ---------------------------------------------------------------------------

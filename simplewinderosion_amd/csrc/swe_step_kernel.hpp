@@ -600,25 +600,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     auto load = [](const PRec* in, int64_t j) __attribute__((always_inline)) { return in[j]; };
     // a particle more than one tile away goes to the far list; the tile of
     // cell (fx, fy) counts it among its next window's arrivals
-    auto send_far = [&](float4 o0, float4 o1, int64_t ledger, int32_t lastc, uint32_t pid, int32_t fx,
-                        int32_t fy) __attribute__((always_inline)) {
+    auto send_far = [&](f4a8 o0, f4a8 o1, int64_t ledger, int32_t fx, int32_t fy) __attribute__((always_inline)) {
       const int32_t dest = t0 + (fx / kSX) * tiles_y + fy / kSY;
       atomicAdd(&a.nin_next[dest], 1u);
       atomicAdd(&a.csum_next[dest / kChunk], 1u);
       const uint32_t fslot = atomicAdd(a.far_cnt_out, 1u);
-      a.far_out[fslot] = PRec{o0, o1, ledger, lastc, pid};
+      a.far_out[fslot] = PRec{o0, o1, ledger};
     };
     // step one record and write it out
     auto process = [&](const Rec& r) __attribute__((always_inline)) {
       Particle w;
-      const float4 q0 = r.q0;
-      const float4 q1 = r.q1;
+      const f4a8 q0 = r.q0;
+      const f4a8 q1 = r.q1;
       w.px = q0.x; w.py = q0.y; w.pz = q0.z; w.sed = q0.w;
       w.sx = q1.x; w.sy = q1.y; w.sz = q1.z;
-      w.age = (int32_t)f2u(q1.w);
+      w.age = a.window;
       w.ledger = r.ledger;
-      w.lastc = r.lastc;
-      const uint32_t pid = r.pid;
+      w.lastc = 0;
+      const uint32_t pid = f2u(q1.w);
       const uint64_t pk = particle_key(ck, (uint64_t)((int64_t)pid - p0));
       int32_t cx, cy;
       const bool ok = cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);
@@ -627,7 +626,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       if (ok && (cx < x0 || cy < y0 || cx >= x0 + kSX || cy >= y0 + kSY)) {
         // routing always places a particle in the tile of its cell; keep
         // LDS indices in range whatever happens by forwarding a stray one
-        send_far(q0, q1, w.ledger, w.lastc, pid, cx, cy);
+        send_far(q0, q1, w.ledger, cx, cy);
         return;
       }
       if (ok) {
@@ -640,7 +639,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
       }
       if (dead) {
-        const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)w.lastc;
+        // died in this call at its cell, or found off the map: the cell it left
+        const int64_t c = dead_cell >= 0 ? dead_cell : (int64_t)f2u(q0.x);
         if (DEPOSIT) {
           const int32_t lx = (int32_t)(c / ny) - x0, ly = (int32_t)(c % ny) - y0;
           if (w.ledger != 0 && lx >= 0 && ly >= 0 && lx < kSX && ly < kSY) {
@@ -659,7 +659,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       // tile of its last cell and dies there at its next call
       int32_t ncx = 0, ncy = 0;
       int32_t code = 4;
+      bool off = true;
       if (cell_of<PERIODIC>(w.px, w.pz, k, ncx, ncy)) {
+        off = false;
         int32_t dtx = ncx / kSX - tx, dty = ncy / kSY - ty;
         if (PERIODIC) {
           if (tiles_x > 2) {
@@ -673,16 +675,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         }
         code = (dtx < -1 || dtx > 1 || dty < -1 || dty > 1) ? 255 : (dtx + 1) * 3 + (dty + 1);
       }
-      const float4 o0 = make_float4(w.px, w.py, w.pz, w.sed);
-      const float4 o1 = make_float4(w.sx, w.sy, w.sz, u2f((uint32_t)w.age));
+      const f4a8 o0 = off ? f4a8{u2f((uint32_t)w.lastc), w.py, __builtin_nanf(""), w.sed}
+                          : f4a8{w.px, w.py, w.pz, w.sed};
+      const f4a8 o1 = {w.sx, w.sy, w.sz, u2f(pid)};
       if (code == 255) {
         far_here++;
-        send_far(o0, o1, w.ledger, w.lastc, pid, ncx, ncy);
+        send_far(o0, o1, w.ledger, ncx, ncy);
         return;
       }
       // sub-range `code` of this tile's output region holds up to nin_u records
       const int64_t o = out_base + (int64_t)code * nin_u + atomicAdd(&s_slot[code], 1u);
-      a.out[o] = PRec{o0, o1, w.ledger, w.lastc, pid};
+      a.out[o] = PRec{o0, o1, w.ledger};
     };
 
     // the arrivals, e = 0 .. n_in - 1 over the nine sub-ranges in order; each
@@ -704,16 +707,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
     }
     for (int64_t j = threadIdx.x; j < (int64_t)far_n; j += kStepThreads) {
-      const int64_t fp = (int64_t)a.far_in[j].pid;
+      const int64_t fp = (int64_t)f2u(a.far_in[j].q1.w);
       if (fp < p0 || fp >= p0 + T.npart) continue;  // another terrain's particle
       int32_t cx, cy;
-      const float4 q0 = a.far_in[j].q0;
+      const f4a8 q0 = a.far_in[j].q0;
       int32_t ctx, cty;
       if (cell_of<PERIODIC>(q0.x, q0.z, k, cx, cy)) {
         ctx = cx / kSX;
         cty = cy / kSY;
       } else {
-        const int32_t lc = a.far_in[j].lastc;
+        const int32_t lc = (int32_t)f2u(q0.x);
         ctx = (lc / ny) / kSX;
         cty = (lc % ny) / kSY;
       }
