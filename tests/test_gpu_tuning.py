@@ -51,15 +51,17 @@ def test_autotune_keeps_state_and_results(statics, tmp_path):
     b.close()
 
 
+@pytest.mark.parametrize("cascade_order", [0, 1])
 @pytest.mark.parametrize("periodic", [True, False])
-def test_every_tile_shape_gives_the_same_bits(statics, periodic):
+def test_every_tile_shape_gives_the_same_bits(statics, periodic, cascade_order):
     """The step kernel's compiled tile shapes (SWE_TUNE_SHAPE 0..4) on one
     workload with crowded and light tiles, a bounded or periodic map whose
-    sides are multiples of no shape: identical masters, fields and counters,
-    and the oracle's."""
+    sides are multiples of no shape, with the synchronous cascade (the step
+    kernel's own relax) and the colour-ordered one (its step-only mode):
+    identical masters, fields and counters, and the oracle's."""
     import oracle as O
 
-    print("case shapes periodic=%s" % periodic, flush=True)
+    print("case shapes periodic=%s cascade_order=%d" % (periodic, cascade_order), flush=True)
     Wind.suspension = 1e-3
     World.settling = 0.02
     World.SEED = 9
@@ -69,7 +71,7 @@ def test_every_tile_shape_gives_the_same_bits(statics, periodic):
     s = np.full((nx, ny), 0.01, np.float32)
     results = []
     for shape in range(5):
-        w = World(nx, ny, periodic=periodic, layers=2, window=1, abrasion=1e-3)
+        w = World(nx, ny, periodic=periodic, layers=2, window=1, abrasion=1e-3, cascade_order=cascade_order)
         w.set_tune(capi.SWE_TUNE_SHAPE, shape)
         assert w.get_tune(capi.SWE_TUNE_SHAPE) == shape
         w.upload(h, s)
@@ -85,7 +87,8 @@ def test_every_tile_shape_gives_the_same_bits(statics, periodic):
     from simplewinderosion_amd.world import build_params
 
     st = O.OracleState(nx, ny, h, sediment=s, params=O.params_from(
-        build_params(periodic=periodic, layers=2, window=1, abrasion=1e-3)))
+        build_params(periodic=periodic, layers=2, window=1, abrasion=1e-3, cascade_order=cascade_order)))
+    assert O.load().or_colour_mode(st.p) == cascade_order
     st.erode(30000, 2)
     assert np.array_equal(results[0][0][0].ravel(), st.zm)
     assert np.array_equal(results[0][0][1].ravel(), st.sm)
