@@ -328,7 +328,8 @@ def main():
     ap.add_argument("--sur-pushforward-blocks", type=int, default=1,
                     help="surrogate: longest own rollout (in blocks of K cycles) a pushforward batch starts from")
     ap.add_argument("--timing-stride", type=int, default=16,
-                    help="HIP events around every n-th kernel launch in the timed region")
+                    help="HIP events around every n-th step launch of the sampling cycles run after the timed "
+                         "region, for the roofline (0: no sampling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     ap.add_argument("--calibrate-snap", type=int, default=0, metavar="SIDE",
@@ -379,7 +380,9 @@ def main():
           file=sys.stderr, flush=True)
     world.erode(N, n_cycles=args.warmup)
     world.synchronize()
-    world.set_timing(args.timing_stride)
+    # the timed region runs the library's default path (a map-cycle replayed
+    # as one HIP graph; no events inside)
+    world.set_timing(0)
     world.reset_stats()
     world.synchronize()
     t0 = time.perf_counter()
@@ -387,7 +390,6 @@ def main():
     world.synchronize()
     t1 = time.perf_counter()
     st = world.stats()
-    audit = world.audit()
     elapsed = t1 - t0
     total_steps = st["particle_steps"]  # counted on the device
     if total_steps != steps_per_cycle * args.steps:
@@ -395,10 +397,31 @@ def main():
                          % (total_steps, N, p.max_age + 1, args.steps))
     value = total_steps / elapsed
 
+    # step kernel launch durations: HIP events on the library's stream around
+    # every timing_stride-th step launch, over cycles run right after the
+    # timed region (events need individual launches, so these cycles run
+    # outside the graph); their own rate is reported beside the timed one
+    sample_cycles = max(2, min(args.steps, 20))
+    sampled_ms = None
+    avg_launch_ms = None
+    if args.timing_stride > 0:
+        world.set_timing(args.timing_stride)
+        world.reset_stats()
+        world.synchronize()
+        ts0 = time.perf_counter()
+        world.erode(N, n_cycles=sample_cycles)
+        world.synchronize()
+        sampled_ms = 1e3 * (time.perf_counter() - ts0) / sample_cycles
+        st2 = world.stats()
+        avg_launch_ms = st2["march_ms"] / st2["march_launches"] if st2["march_launches"] else None
+        world.set_timing(0)
+    else:
+        st2 = {"march_launches": 0}
+    audit = world.audit()
+
     # step kernel: algorithmic bytes per launch / HIP-event launch time
     launches_per_cycle = (p.max_age + 2) + 1 if R == 1 else None
     cyc_bytes, per_step, per_cell = algorithmic_bytes_per_cycle(p, steps_per_cycle, world.nx * world.ny)
-    avg_launch_ms = st["march_ms"] / st["march_launches"] if st["march_launches"] else None
     achieved = None
     bytes_per_launch = None
     if avg_launch_ms and launches_per_cycle:
@@ -410,7 +433,9 @@ def main():
                 "kernel": "k_step", "algorithmic_bytes_per_launch": bytes_per_launch,
                 "bytes_per_step": per_step, "bytes_per_cell_cycle": per_cell,
                 "launches_per_cycle": launches_per_cycle, "avg_launch_ms": avg_launch_ms,
-                "timed_launches": st["march_launches"], "timing_stride": args.timing_stride}
+                "timed_launches": st2["march_launches"], "timing_stride": args.timing_stride,
+                "sampled_cycles": sample_cycles if args.timing_stride > 0 else 0,
+                "sampled_ms_per_cycle": sampled_ms}
     line = {
         "metric": "particle-steps/s",
         "value": value,
