@@ -135,17 +135,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   const uint32_t ls = RELAX_ONLY ? 0u : (uint32_t)a.light_slots;
   const uint32_t nb = (uint32_t)a.default_slots, b = blockIdx.x - (uint32_t)a.heavy_slots;
   const uint32_t xcd = b % 8;
+  // a single map's terrain record does not depend on the tile: its load goes
+  // out with the slot's list or stamp loads below, not after them
+  const Terrain T1 = a.terr[0];
   int32_t u;  // global tile
   bool listed = false;
   if (blockIdx.x < (uint32_t)a.heavy_slots) {
     const uint32_t k = blockIdx.x / 8;  // heavy slot k of this XCD (blockIdx.x % 8 == xcd)
-    if (hs == 0 || k >= a.hcnt_cur[xcd]) return;
-    u = (int32_t)a.heavy_cur[xcd * (hs / 8) + k];
+    if (hs == 0) return;
+    // the list entry is read together with the list length (k < hs / 8, so
+    // it lies inside the list whatever the length; unused when k >= length)
+    const uint32_t cnt = a.hcnt_cur[xcd];
+    const uint32_t cand = a.heavy_cur[xcd * (hs / 8) + k];
+    if (k >= cnt) return;
+    u = (int32_t)cand;
     listed = true;
   } else if (b >= nb) {
-    const uint32_t bl = b - nb, k = bl / 8;
-    if (ls == 0 || k >= a.hcnt_cur[8 + xcd]) return;
-    u = (int32_t)a.heavy_cur[kHeavySlotsMax + xcd * (ls / 8) + k];
+    const uint32_t bl = b - nb, k = bl / 8;  // k < ls / 8
+    if (ls == 0) return;
+    const uint32_t cnt = a.hcnt_cur[8 + xcd];
+    const uint32_t cand = a.heavy_cur[kHeavySlotsMax + xcd * (ls / 8) + k];
+    if (k >= cnt) return;
+    u = (int32_t)cand;
     listed = true;
   } else if (a.tile_order <= 0) {
     const uint32_t q8 = nb / 8, r8 = nb % 8;
@@ -169,7 +180,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   // before it loads anything
   if ((hs | ls) > 0 && !listed && a.hstamp[u] > (uint32_t)a.window) return;
   // the terrain this tile belongs to; all map indices below are terrain-local
-  const Terrain T = a.terr[a.nterr == 1 ? 0 : a.tile_terrain[u]];
+  Terrain T = T1;
+  if (a.nterr != 1) T = a.terr[a.tile_terrain[u]];
   const int32_t nx = T.nx, ny = T.ny;
   const int32_t tiles_x = T.tiles_x, tiles_y = T.tiles_y;
   const int32_t t0 = T.tile0;
