@@ -83,6 +83,13 @@ typedef struct {
  * a device array of nseg segments in order of off, total = sum of n. */
 int swe_nn_adam_multi(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, float lr, float b1, float b2,
                       float eps, int32_t step, void* stream);
+/* The same with lr and the bias corrections read from device memory at
+ * hyper[0..2] (for a step captured once in a graph); swe_nn_adam_hyper
+ * fills a host array with (lr, 1 - b1^step, 1 - b2^step) exactly as
+ * swe_nn_adam_multi computes them. */
+int swe_nn_adam_multi_dev(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, const float* hyper, float b1,
+                          float b2, float eps, void* stream);
+void swe_nn_adam_hyper(float lr, float b1, float b2, int32_t step, float* out);
 /* Surrogate input features x [N][H][W][8] bf16 from float maps h, s [N][H][W]
  * (s may be NULL), per-map mean heights and wind azimuths [N]: (h - mean) /
  * h_scale, s / s_scale, cos az, sin az, periodic central differences along

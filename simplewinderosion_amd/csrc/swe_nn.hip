@@ -875,10 +875,17 @@ __global__ void k_adam(float* __restrict__ w, bf16* __restrict__ w16, const floa
 // search over the nseg offsets).  The arithmetic is k_adam's; a segment
 // with wt16 is a convolution's [Cout][9][Cin] weights whose flipped bf16
 // copy (k_flip's layout) is written beside w16.
+// hyper (optional): lr, c1, c2 read from device memory instead of the
+// arguments, so that a captured graph serves every step
 __global__ void k_adam_multi(const swe_nn_adam_seg* __restrict__ seg, int32_t nseg, int64_t total, float lr,
-                             float b1, float b2, float eps, float c1, float c2) {
+                             float b1, float b2, float eps, float c1, float c2, const float* __restrict__ hyper) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= total) return;
+  if (hyper) {
+    lr = hyper[0];
+    c1 = hyper[1];
+    c2 = hyper[2];
+  }
   int32_t lo = 0, hi = nseg - 1;
   while (lo < hi) {
     const int32_t mid = (lo + hi + 1) >> 1;
@@ -1252,7 +1259,23 @@ int swe_nn_adam_multi(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, f
   if (!seg || nseg < 1 || total < 0 || step < 1) return -1;
   if (total == 0) return 0;
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_adam_multi, dim3(G(total)), dim3(256), 0, S(stream), seg, nseg, total, lr, b1, b2, eps, c1, c2);
+  hipLaunchKernelGGL(k_adam_multi, dim3(G(total)), dim3(256), 0, S(stream), seg, nseg, total, lr, b1, b2, eps, c1, c2,
+                     (const float*)nullptr);
+  SWENN_LAUNCH_CHECK();
+}
+
+void swe_nn_adam_hyper(float lr, float b1, float b2, int32_t step, float* out) {
+  out[0] = lr;
+  out[1] = 1.0f - powf(b1, (float)step);
+  out[2] = 1.0f - powf(b2, (float)step);
+}
+
+int swe_nn_adam_multi_dev(const swe_nn_adam_seg* seg, int32_t nseg, int64_t total, const float* hyper, float b1,
+                          float b2, float eps, void* stream) {
+  if (!seg || !hyper || nseg < 1 || total < 0) return -1;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(k_adam_multi, dim3(G(total)), dim3(256), 0, S(stream), seg, nseg, total, 0.0f, b1, b2, eps, 0.0f,
+                     0.0f, hyper);
   SWENN_LAUNCH_CHECK();
 }
 

@@ -32,6 +32,8 @@ SIGNATURES = {
     "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
     "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
     "swe_nn_adam_multi": (ctypes.c_int, [_P, _I32, _I64, _F, _F, _F, _F, _I32, _P]),
+    "swe_nn_adam_multi_dev": (ctypes.c_int, [_P, _I32, _I64, _P, _F, _F, _F, _P]),
+    "swe_nn_adam_hyper": (None, [_F, _F, _F, _I32, _P]),
     "swe_nn_features": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _F, _F, _F, _F, _P, _P]),
     "swe_nn_project_mass": (ctypes.c_int, [_P, _I64, _F, _I64, _P, _P, _P]),
 }
@@ -246,6 +248,20 @@ def adam_table(segs, device):
 def adam_multi(table, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
     buf, nseg, total = table
     _ck(load().swe_nn_adam_multi(_p(buf), nseg, total, lr, b1, b2, eps, int(step), _stream()), "swe_nn_adam_multi")
+
+
+def adam_multi_dev(table, hyper, b1=0.9, b2=0.999, eps=1e-8):
+    """adam_multi with (lr, c1, c2) read from the device tensor `hyper`."""
+    buf, nseg, total = table
+    _ck(load().swe_nn_adam_multi_dev(_p(buf), nseg, total, _p(hyper), b1, b2, eps, _stream()),
+        "swe_nn_adam_multi_dev")
+
+
+def adam_hyper(lr, step, b1=0.9, b2=0.999):
+    """(lr, 1 - b1^step, 1 - b2^step) as the library computes them (float32)."""
+    out = (ctypes.c_float * 3)()
+    load().swe_nn_adam_hyper(lr, b1, b2, int(step), ctypes.cast(out, ctypes.c_void_p))
+    return [out[0], out[1], out[2]]
 
 
 def project_mass(d, scale, target, master, work):

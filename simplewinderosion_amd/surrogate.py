@@ -225,15 +225,7 @@ class Surrogate:
         """One Adam step of every weight and bias in one launch (which also
         keeps the bf16 weights and their flipped copies)."""
         self.step += 1
-        if self._adam is None:
-            segs = []
-            for name, cin, cout, _ in LAYERS:
-                segs.append((self.w32[name], self.w16[name], self.wt16[name], self.gw[name], self.m[name],
-                             self.v[name], cin, cout))
-                segs.append((self.b32[name], None, None, self.gb[name], self.m[name + ".b"], self.v[name + ".b"],
-                             0, 0))
-            self._adam = nn.adam_table(segs, self.dev)
-        nn.adam_multi(self._adam, self.step, lr=lr)
+        nn.adam_multi(self._adam_table(), self.step, lr=lr)
 
     def train_step(self, h, s, az, dh, ds, lr):
         """One Adam step on a batch: heights h, sediment s [N,H,W], wind
@@ -250,10 +242,86 @@ class Surrogate:
         self.adam(lr)
         return part.sum()
 
+    def _adam_table(self):
+        if self._adam is None:
+            segs = []
+            for name, cin, cout, _ in LAYERS:
+                segs.append((self.w32[name], self.w16[name], self.wt16[name], self.gw[name], self.m[name],
+                             self.v[name], cin, cout))
+                segs.append((self.b32[name], None, None, self.gb[name], self.m[name + ".b"], self.v[name + ".b"],
+                             0, 0))
+            self._adam = nn.adam_table(segs, self.dev)
+        return self._adam
+
     def predict(self, h, s, az):
         """K-cycle height and sediment changes [N,H,W] (device float)."""
         out, _ = self.forward(self.features(h, s, az))
         return out[..., 0] * self.d_scale, out[..., 1] * self.ds_scale
+
+
+class TrainGraph:
+    """Surrogate.train_step for a fixed batch shape, captured once as a HIP
+    graph and replayed: the batch goes into static device buffers, the mean
+    heights are computed before the replay (a torch reduction stays outside
+    the graph), and Adam reads its learning rate and bias corrections from
+    device memory.  The first step runs eagerly (it allocates the work
+    buffers); every step computes the same bits as train_step."""
+
+    def __init__(self, model, N, H, W):
+        torch = _torch()
+        dev = model.dev
+        self.model = model
+        self.shape = (N, H, W)
+        f = dict(device=dev, dtype=torch.float32)
+        self.h, self.s = torch.empty((N, H, W), **f), torch.empty((N, H, W), **f)
+        self.dh, self.ds = torch.empty((N, H, W), **f), torch.empty((N, H, W), **f)
+        self.mean, self.az = torch.empty((N,), **f), torch.empty((N,), **f)
+        self.hyper = torch.empty((3,), **f)
+        self.hyper_host = torch.empty((3,), dtype=torch.float32, pin_memory=True)
+        self.graph = None
+        self.part = None
+
+    def _kernels(self):
+        """The captured work: features, forward, loss gradient, backward, Adam."""
+        torch = _torch()
+        m = self.model
+        x0 = nn.features(self.h, self.s, self.mean, self.az, m.h_scale, m.s_scale, m.g_scale, m.l_scale)
+        out, A = m.forward(x0)
+        N, H, W, C = out.shape
+        target = torch.stack((self.dh / m.d_scale, self.ds / m.ds_scale), dim=-1).reshape(N * H * W, N_OUT)
+        target = target.contiguous()
+        grad = torch.empty_like(out)
+        part = torch.empty(((N * H * W * C + 255) // 256,), device=m.dev)
+        nn.mse(out.view(N * H * W, C), target, grad.view(N * H * W, C), part)
+        m.backward(A, grad)
+        nn.adam_multi_dev(m._adam_table(), self.hyper)
+        return part
+
+    def step(self, h, s, az, dh, ds, lr):
+        """One Adam step on a batch (as Surrogate.train_step); returns the
+        loss tensor (device)."""
+        torch = _torch()
+        m = self.model
+        if tuple(h.shape) != self.shape:
+            raise ValueError("batch shape %s, graph captured for %s" % (tuple(h.shape), self.shape))
+        if self.graph is None:
+            loss = m.train_step(h, s, az, dh, ds, lr)  # eager: allocates what the graph will reuse
+            torch.cuda.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph):
+                self.part = self._kernels()
+            return loss
+        self.h.copy_(h)
+        self.s.copy_(s)
+        self.dh.copy_(dh)
+        self.ds.copy_(ds)
+        self.mean.copy_(self.h.mean(dim=(1, 2)))
+        self.az.copy_(torch.as_tensor(np.asarray(az, np.float32).reshape(-1)))
+        m.step += 1
+        self.hyper_host.copy_(torch.tensor(nn.adam_hyper(lr, m.step), dtype=torch.float32))
+        self.hyper.copy_(self.hyper_host, non_blocking=True)
+        self.graph.replay()
+        return self.part.sum()
 
 
 # -----------------------------------------------------------------------------
@@ -342,7 +410,7 @@ def symmetry(x, az, k):
 
 
 def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_every=100, augment=True,
-          pushforward=0.0, pushforward_from=0.5, pushforward_blocks=1):
+          pushforward=0.0, pushforward_from=0.5, pushforward_blocks=1, graph=False):
     """Adam on random (terrain, t) pairs of `states` with a cosine-decayed
     learning rate; with `augment`, each batch under a random one of the
     map's eight symmetries (square maps).  With `pushforward` p > 0, after
@@ -352,7 +420,9 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
     in 1 .. pushforward_blocks, as many as the data allows) -- the state a
     rollout reaches after b surrogate blocks -- and its target the plain
     state (b + 1) K cycles on, so that the network learns to correct its own
-    drift.  Returns the loss curve and the time of the steps."""
+    drift.  With `graph`, the steps replay one captured HIP graph
+    (TrainGraph: the same bits, less host time).  Returns the loss curve and
+    the time of the steps."""
     torch = _torch()
     T, n = states.shape[0] - K, states.shape[1]
     T2 = states.shape[0] - 2 * K  # pushforward needs two blocks of data after t
@@ -360,6 +430,7 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
     az = np.asarray(az, np.float64)
     augment = augment and states.shape[-1] == states.shape[-2]
     curve = []
+    tg = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for st in range(steps):
@@ -387,8 +458,14 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
             pair, a = symmetry(torch.stack((x0, x1)), a, sym)
             x0, x1 = pair[0], pair[1]
         rate = lr * 0.5 * (1.0 + math.cos(math.pi * st / max(1, steps)))
-        loss = model.train_step(x0[:, 0].contiguous(), x0[:, 1].contiguous(), a.astype(np.float32),
-                                (x1[:, 0] - x0[:, 0]).contiguous(), (x1[:, 1] - x0[:, 1]).contiguous(), rate)
+        args = (x0[:, 0].contiguous(), x0[:, 1].contiguous(), a.astype(np.float32),
+                (x1[:, 0] - x0[:, 0]).contiguous(), (x1[:, 1] - x0[:, 1]).contiguous(), rate)
+        if graph:
+            if tg is None:
+                tg = TrainGraph(model, *args[0].shape)
+            loss = tg.step(*args)
+        else:
+            loss = model.train_step(*args)
         if st % log_every == 0 or st == steps - 1:
             curve.append((st, float(loss)))
             if log:

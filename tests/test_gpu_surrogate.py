@@ -76,6 +76,36 @@ def test_backward_matches_autograd():
         assert cb > 0.98, (name, cb)
 
 
+def test_graph_training_steps_match_eager():
+    """Training steps replayed from one captured HIP graph (TrainGraph) give
+    the eager steps' bits -- weights, bf16 and flipped copies, Adam moments,
+    loss -- on every replay, with other eager GPU work between replays."""
+    print("case train graph", flush=True)
+    a = S.Surrogate(seed=3)
+    b = S.Surrogate(seed=3)
+    tg = S.TrainGraph(b, 2, 64, 64)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    az = np.array([0.3, 1.9], np.float32)
+    for step in range(5):
+        h = (torch.rand((2, 64, 64), generator=g) * 0.5).to("cuda")
+        s = (torch.rand((2, 64, 64), generator=g) * 0.02).to("cuda")
+        dh = (torch.randn((2, 64, 64), generator=g) * 0.01).to("cuda")
+        ds = (torch.randn((2, 64, 64), generator=g) * 0.01).to("cuda")
+        lr = 1e-3 * (1.0 - 0.1 * step)
+        la = a.train_step(h, s, az, dh, ds, lr)
+        lb = tg.step(h, s, az, dh, ds, lr)
+        b.predict(h, s, az)  # eager work between replays
+        torch.cuda.synchronize()
+        assert float(la) == float(lb), step
+        assert a.step == b.step
+        for n in a.w32:
+            assert torch.equal(a.w32[n], b.w32[n]) and torch.equal(a.b32[n], b.b32[n]), (step, n)
+            assert torch.equal(a.w16[n], b.w16[n]) and torch.equal(a.wt16[n], b.wt16[n]), (step, n)
+            for k in (n, n + ".b"):
+                assert torch.equal(a.m[k], b.m[k]) and torch.equal(a.v[k], b.v[k]), (step, k)
+    assert tg.graph is not None
+
+
 def test_training_fits_a_batch():
     print("case surrogate fit", flush=True)
     m = S.Surrogate(seed=4)
