@@ -5,15 +5,15 @@ A step is one map-cycle -- World::erode(N) (world.hpp:36-77) -- of the
 sediment), 1,000,000 particles per cycle, step-synchronous windows (one
 fly() call per particle per window, 514 windows + a closing relax per
 cycle), flow tracking on.  Inputs are resident in HBM before the timed
-region; the timed region is K whole cycles bracketed by device
-synchronisation.
+region; the timed region is K whole cycles of the library's default path
+(each cycle one HIP-graph launch) bracketed by device synchronisation.
 
 Prints one JSON line with value = particle-steps/s, the roofline of the
 tile step kernel (k_step) -- algorithmic bytes per launch over its average
 launch duration, measured with HIP events on the library's stream around
-every 16th launch inside the timed region -- and the CPU baseline (the
-reference's sequential algorithm, one core) timed on a bounded sample of
-the same workload.
+every 16th step launch of up to 20 cycles run right after the timed
+region -- and the CPU baseline (the reference's sequential algorithm, one
+core) timed on a bounded sample of the same workload.
 """
 import argparse
 import json
