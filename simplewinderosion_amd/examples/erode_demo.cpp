@@ -2,19 +2,24 @@
 // written against include/swe_world.hpp: World::init, then World::erode(N)
 // per frame (SimpleWindErosion.cpp:36-38,212-217).  Prints the exact mass
 // sum and a height checksum so that tests can compare with other callers.
-//   erode_demo NX NY SEED PARTICLES FRAMES
+//   erode_demo NX NY SEED PARTICLES FRAMES [reference]
+// With "reference" every static keeps the reference's value (bounded map,
+// settling 1.0, suspension 0.05, ...); otherwise the map is periodic and
+// settling 0.05.
 #include <cinttypes>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "swe_world.hpp"
 
 int main(int argc, char** argv) {
   if (argc < 6) {
-    std::fprintf(stderr, "usage: %s NX NY SEED PARTICLES FRAMES\n", argv[0]);
+    std::fprintf(stderr, "usage: %s NX NY SEED PARTICLES FRAMES [reference]\n", argv[0]);
     return 2;
   }
+  const bool reference = argc > 6 && std::string(argv[6]) == "reference";
   const int nx = std::atoi(argv[1]), ny = std::atoi(argv[2]);
   const unsigned seed = (unsigned)std::strtoul(argv[3], nullptr, 10);
   const int particles = std::atoi(argv[4]), frames = std::atoi(argv[5]);
@@ -24,8 +29,10 @@ int main(int argc, char** argv) {
   swe::World::nx = nx;
   swe::World::ny = ny;
   swe::World::SEED = seed;
-  swe::World::settling = 0.05f;
-  swe::World::periodic = 1;
+  if (!reference) {
+    swe::World::settling = 0.05f;
+    swe::World::periodic = 1;
+  }
   try {
     swe::World::init(h.data());
     for (int f = 0; f < frames; f++) swe::World::erode(particles);

@@ -16,19 +16,28 @@ from simplewinderosion_amd.build import DEMO
 pytestmark = pytest.mark.gpu
 
 
-def test_cpp_world_mirror_matches_python_and_oracle(statics):
-    nx, ny, seed, particles, frames = 48, 40, 7, 300, 3
-    print("case cpp mirror", nx, ny, seed, particles, frames, flush=True)
+@pytest.mark.parametrize("reference", [False, True])
+def test_cpp_world_mirror_matches_python_and_oracle(statics, reference):
+    """Periodic with settling 0.05, and every static at the reference's value
+    (bounded map, settling 1.0: the colour-ordered cascade) at the
+    reference's density (4096 particles on 512^2 -> 64 on 64^2)."""
+    nx, ny, seed, particles, frames = (64, 64, 7, 64, 4) if reference else (48, 40, 7, 300, 3)
+    print("case cpp mirror", nx, ny, seed, particles, frames, reference, flush=True)
     assert os.path.exists(DEMO), "C++ example not built"
-    out = subprocess.run([DEMO, str(nx), str(ny), str(seed), str(particles), str(frames)],
-                         capture_output=True, text=True, timeout=120)
+    cmd = [DEMO, str(nx), str(ny), str(seed), str(particles), str(frames)] + (["reference"] if reference else [])
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     cpp = json.loads(out.stdout.strip().splitlines()[-1])
 
     h = (generate_terrain(nx, ny, seed) * np.float32(0.5)).astype(np.float32)
     World.SEED = seed
-    World.settling = 0.05
-    w = World(nx, ny, periodic=True)
+    if reference:
+        assert World.settling == 1.0
+        w = World(nx, ny)
+        assert w.params().periodic == 0 and O.load().or_colour_mode(O.params_from(w.params())) == 1
+    else:
+        World.settling = 0.05
+        w = World(nx, ny, periodic=True)
     w.upload(h)
     for _ in range(frames):
         w.erode(particles)
