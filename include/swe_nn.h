@@ -26,6 +26,16 @@ int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, 
  * into y32 followed by swe_nn_relu_grad. */
 int swe_nn_conv3x3_masked(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w,
                           const float* bias, int32_t Cout, const void* mask, void* y16, void* stream);
+/* The same convolution with a caller's workspace: act 0 none, 1 ReLU, 2
+ * zero where mask <= 0.  Layers with few pixel blocks and a long reduction
+ * (low resolutions, many channels) split the reduction over up to 8
+ * workgroups per block, whose partial sums (work: swe_nn_conv3x3_work
+ * floats, 0 when the layer is not split) are added in a fixed order; with
+ * less work than that the layer runs unsplit. */
+int64_t swe_nn_conv3x3_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout);
+int swe_nn_conv3x3_ws(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
+                      int32_t Cout, int32_t act, const void* mask, void* y16, float* y32, float* work,
+                      int64_t work_len, void* stream);
 /* wt[ci][t][co] = w[co][8-t][ci]: the weights whose forward convolution of
  * dy is the data gradient dx (Cin and Cout swap). */
 int swe_nn_flip(const void* w, int32_t Cin, int32_t Cout, void* wt, void* stream);

@@ -42,11 +42,16 @@ __device__ __forceinline__ int32_t wrapi(int32_t v, int32_t n) { return v < 0 ? 
 // ---------------------------------------------------------------------------
 // ACT 0: none, 1: ReLU, 2: zero where mask <= 0 (the ReLU gradient of a
 // data gradient through the layer below, mask = that layer's output).
+// With part, workgroup z of gridDim.z takes the k-steps [z K / Z, (z + 1) K /
+// Z) and stores its raw sums in part[z][pixel][channel] (k_conv_reduce adds
+// the Z partials in order and applies the epilogue): split-K for layers with
+// few pixel blocks, whose workgroups would otherwise each walk every k-step.
 template <int NT, int ACT>
 __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, const bf16* __restrict__ w,
                                                  const float* __restrict__ bias, const bf16* __restrict__ mask,
                                                  bf16* __restrict__ y16, float* __restrict__ y32, int32_t N,
-                                                 int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
+                                                 int32_t H, int32_t W, int32_t Cin, int32_t Cout,
+                                                 float* __restrict__ part) {
   constexpr int WM = NT >= 64 ? 2 : 4;  // waves along the pixels
   constexpr int WN = 4 / WM;            // ... and the channels
   constexpr int TM = 128 / WM, TN = NT / WN;
@@ -114,13 +119,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
 #pragma unroll
     for (int j = 0; j < BN; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 
-  load(0);
+  const int32_t kz = (int32_t)gridDim.z, z = (int32_t)blockIdx.z;
+  const int32_t s0 = (int32_t)((int64_t)z * ksteps / kz), s1 = (int32_t)((int64_t)(z + 1) * ksteps / kz);
+  load(s0);
   store(0);
   __syncthreads();
   const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
-  for (int32_t s = 0; s < ksteps; s++) {
-    const int32_t buf = s & 1;
-    if (s + 1 < ksteps) load(s + 1);
+  for (int32_t s = s0; s < s1; s++) {
+    const int32_t buf = (s - s0) & 1;
+    if (s + 1 < s1) load(s + 1);
     bf16x8 a[BM], b[BN];
 #pragma unroll
     for (int i = 0; i < BM; i++) a[i] = *(const bf16x8*)&sA[buf][(wm * TM + 16 * i + rl) * LD + kq];
@@ -130,8 +137,24 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
     for (int i = 0; i < BM; i++)
 #pragma unroll
       for (int j = 0; j < BN; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-    if (s + 1 < ksteps) store(buf ^ 1);
+    if (s + 1 < s1) store(buf ^ 1);
     __syncthreads();
+  }
+  if (part) {
+    float* pz = part + (int64_t)z * P * Cout;
+#pragma unroll
+    for (int j = 0; j < BN; j++) {
+      const int32_t co = co0 + wn * TN + 16 * j + rl;
+      if (co >= Cout) continue;
+#pragma unroll
+      for (int i = 0; i < BM; i++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int64_t p = p0 + wm * TM + 16 * i + 4 * (lane >> 4) + e;
+          if (p < P) pz[p * Cout + co] = acc[i][j][e];
+        }
+    }
+    return;
   }
 #pragma unroll
   for (int j = 0; j < BN; j++) {
@@ -284,6 +307,24 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, c
     step(s, ra, rb);
     if (s + 1 < rows) step(s + 1, rb, ra);
   }
+}
+
+// y = the Z split-K partials added in order + bias, then ACT (as k_conv3x3's
+// epilogue); n = pixels x Cout
+template <int ACT>
+__global__ void k_conv_reduce(const float* __restrict__ part, int32_t kz, int64_t n, int32_t Cout,
+                              const float* __restrict__ bias, const bf16* __restrict__ mask, bf16* __restrict__ y16,
+                              float* __restrict__ y32) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float a = part[i];
+  for (int32_t z = 1; z < kz; z++) a += part[(int64_t)z * n + i];
+  const int32_t co = (int32_t)(i % Cout);
+  float v = a + (bias ? bias[co] : 0.0f);
+  if (ACT == 1) v = v > 0.0f ? v : 0.0f;
+  if (ACT == 2) v = bf2f(mask[i]) > 0.0f ? v : 0.0f;
+  if (y16) y16[i] = f2bf(v);
+  if (y32) y32[i] = v;
 }
 
 // wt[ci][t'][co] = w[co][8 - t][ci]: the data gradient of a periodic 3x3
@@ -1014,15 +1055,39 @@ extern "C" {
 #ifndef SWENN_CONV_MINROWS
 #define SWENN_CONV_MINROWS 8
 #endif
+// split-K factor of k_conv3x3 for a grid of mt x ncob workgroups: layers
+// with fewer workgroups than the chip has CUs and a long reduction are
+// split so that ~768 workgroups take >= 8 k-steps each (at most 8 parts)
+static int32_t conv_split(int64_t mt, int64_t ncob, int32_t ksteps) {
+  const int64_t wgs = mt * ncob;
+  if (wgs >= 256 || ksteps < 24) return 1;
+  int64_t z = (768 + wgs - 1) / wgs;
+  if (z > ksteps / 8) z = ksteps / 8;
+  if (z > 8) z = 8;
+  return z < 2 ? 1 : (int32_t)z;
+}
+
+static bool conv_rows_ok(int32_t W, int32_t Cin, int32_t Cout, int64_t P);
+
+// k_conv3x3's grid: pixel blocks, output-channel blocks of nt, split parts
+static void conv_grid(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout, int64_t* mt, int32_t* nt,
+                      int32_t* kz) {
+  const int64_t P = (int64_t)N * H * W;
+  *mt = (P + 127) / 128;
+  *nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || *mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
+  *kz = conv_split(*mt, (Cout + *nt - 1) / *nt, 9 * ((Cin + 31) / 32));
+}
+
 static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
-                       int32_t Cout, int act, const void* mask, void* y16, float* y32, void* stream) {
+                       int32_t Cout, int act, const void* mask, void* y16, float* y32, void* stream,
+                       float* work = nullptr, int64_t work_len = 0) {
   if (!x || !w || N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (!y16 && !y32)) return -1;
   const int64_t P = (int64_t)N * H * W;
   const bf16* xx = (const bf16*)x;
   const bf16* ww = (const bf16*)w;
   const bf16* mm = (const bf16*)mask;
   bf16* yy = (bf16*)y16;
-  if (SWENN_CONV_ROWS && W % 32 == 0 && (Cin <= 32 || (Cin <= 64 && (int64_t)Cout * P <= ((int64_t)1 << 22)))) {
+  if (conv_rows_ok(W, Cin, Cout, P)) {
     const int64_t strips = (int64_t)N * (W / 32), cob = (Cout + 31) / 32;
     int64_t cps = (SWENN_CONV_WGS + strips * cob - 1) / (strips * cob);
     const int64_t cmax = (H + SWENN_CONV_MINROWS - 1) / SWENN_CONV_MINROWS;
@@ -1045,16 +1110,21 @@ static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t C
 #undef SWENN_CONV_R
     SWENN_LAUNCH_CHECK();
   }
-  const int64_t mt = (P + 127) / 128;
+  int64_t mt;
+  int32_t nt, kz;
+  conv_grid(N, H, W, Cin, Cout, &mt, &nt, &kz);
   if (mt > 0x7fffffffLL) return -1;
-  const int32_t nt = Cout <= 16 ? 16 : (Cout <= 32 ? 32 : (Cout <= 64 || mt * ((Cout + 127) / 128) < 512 ? 64 : 128));
-  const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt));
+  if (kz > 1 && (!work || work_len < (int64_t)kz * P * Cout)) kz = 1;  // no room for the partials: unsplit
+  float* part = kz > 1 ? work : nullptr;
+  const dim3 grid((unsigned)mt, (unsigned)((Cout + nt - 1) / nt), (unsigned)kz);
+  // split: the parts hold raw sums (the kernel's epilogue is not run)
+  const int ka = kz > 1 ? 0 : act;
 #define SWENN_CONV(NT_, AC)                                                                                       \
   hipLaunchKernelGGL((k_conv3x3<NT_, AC>), grid, dim3(256), 0, S(stream), xx, ww, bias, mm, yy, y32, N, H, W, Cin, \
-                     Cout)
+                     Cout, part)
 #define SWENN_CONV_NT(NT_)            \
-  if (act == 1) SWENN_CONV(NT_, 1);  \
-  else if (act == 2) SWENN_CONV(NT_, 2); \
+  if (ka == 1) SWENN_CONV(NT_, 1);  \
+  else if (ka == 2) SWENN_CONV(NT_, 2); \
   else SWENN_CONV(NT_, 0)
   if (nt == 16) { SWENN_CONV_NT(16); }
   else if (nt == 32) { SWENN_CONV_NT(32); }
@@ -1062,7 +1132,37 @@ static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t C
   else { SWENN_CONV_NT(128); }
 #undef SWENN_CONV_NT
 #undef SWENN_CONV
+  if (kz > 1) {
+    const int64_t n = P * Cout;
+    if (act == 1)
+      hipLaunchKernelGGL(k_conv_reduce<1>, dim3(G(n)), dim3(256), 0, S(stream), part, kz, n, Cout, bias, mm, yy, y32);
+    else if (act == 2)
+      hipLaunchKernelGGL(k_conv_reduce<2>, dim3(G(n)), dim3(256), 0, S(stream), part, kz, n, Cout, bias, mm, yy, y32);
+    else
+      hipLaunchKernelGGL(k_conv_reduce<0>, dim3(G(n)), dim3(256), 0, S(stream), part, kz, n, Cout, bias, mm, yy, y32);
+  }
   SWENN_LAUNCH_CHECK();
+}
+
+static bool conv_rows_ok(int32_t W, int32_t Cin, int32_t Cout, int64_t P) {
+  return SWENN_CONV_ROWS && W % 32 == 0 && (Cin <= 32 || (Cin <= 64 && (int64_t)Cout * P <= ((int64_t)1 << 22)));
+}
+
+int64_t swe_nn_conv3x3_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
+  if (N < 1 || H < 1 || W < 1 || Cin < 1 || Cout < 1) return 0;
+  const int64_t P = (int64_t)N * H * W;
+  if (conv_rows_ok(W, Cin, Cout, P)) return 0;
+  int64_t mt;
+  int32_t nt, kz;
+  conv_grid(N, H, W, Cin, Cout, &mt, &nt, &kz);
+  return kz > 1 ? (int64_t)kz * P * Cout : 0;
+}
+
+int swe_nn_conv3x3_ws(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
+                      int32_t Cout, int32_t act, const void* mask, void* y16, float* y32, float* work,
+                      int64_t work_len, void* stream) {
+  if (act < 0 || act > 2 || (act == 2 && !mask)) return -1;
+  return conv_launch(x, N, H, W, Cin, w, bias, Cout, act, mask, y16, y32, stream, work, work_len);
 }
 
 int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,

@@ -16,6 +16,8 @@ _F = ctypes.c_float
 SIGNATURES = {
     "swe_nn_conv3x3": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _I32, _P, _P, _P]),
     "swe_nn_conv3x3_masked": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _P, _P, _P]),
+    "swe_nn_conv3x3_work": (_I64, [_I32, _I32, _I32, _I32, _I32]),
+    "swe_nn_conv3x3_ws": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _P, _P, _I32, _I32, _P, _P, _P, _P, _I64, _P]),
     "swe_nn_flip": (ctypes.c_int, [_P, _I32, _I32, _P, _P]),
     "swe_nn_wgrad_work": (_I64, [_I32, _I32, _I32, _I32, _I32]),
     "swe_nn_wgrad": (ctypes.c_int, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _P, _I64, _P]),
@@ -88,9 +90,16 @@ def _dense(t, dtype, what):
     return t
 
 
-def conv3x3(x, w, bias=None, relu=False, out16=None, out32=None):
+def conv_work(N, H, W, Cin, Cout):
+    """Floats of workspace a split convolution of this shape uses (0: unsplit)."""
+    return int(load().swe_nn_conv3x3_work(N, H, W, Cin, Cout))
+
+
+def conv3x3(x, w, bias=None, relu=False, out16=None, out32=None, work=None):
     """x [N,H,W,Cin] bf16, w [Cout,9,Cin] bf16, bias [Cout] float -> out16
-    [N,H,W,Cout] bf16 and/or out32 float (allocated when neither is given)."""
+    [N,H,W,Cout] bf16 and/or out32 float (allocated when neither is given).
+    With `work` (float device tensor of conv_work(...) floats), layers with
+    few pixel blocks split their reduction over several workgroups."""
     import torch
 
     _dense(x, torch.bfloat16, "x")
@@ -101,12 +110,16 @@ def conv3x3(x, w, bias=None, relu=False, out16=None, out32=None):
         raise ValueError("weights must be [Cout, 9, Cin]")
     if out16 is None and out32 is None:
         out16 = torch.empty((N, H, W, Cout), device=x.device, dtype=torch.bfloat16)
-    _ck(load().swe_nn_conv3x3(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, int(bool(relu)), _p(out16), _p(out32),
-                              _stream()), "swe_nn_conv3x3")
+    if work is not None:
+        _ck(load().swe_nn_conv3x3_ws(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, int(bool(relu)), None, _p(out16),
+                                     _p(out32), _p(work), work.numel(), _stream()), "swe_nn_conv3x3_ws")
+    else:
+        _ck(load().swe_nn_conv3x3(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, int(bool(relu)), _p(out16),
+                                  _p(out32), _stream()), "swe_nn_conv3x3")
     return out16 if out16 is not None else out32
 
 
-def conv3x3_masked(x, w, mask, out16, bias=None):
+def conv3x3_masked(x, w, mask, out16, bias=None, work=None):
     """out16 [N,H,W,Cout] bf16 = bf16(conv3x3(x, w) + bias), zero where mask
     [N,H,W,Cout] bf16 <= 0 (mask optional): a data gradient through a ReLU."""
     import torch
@@ -120,8 +133,13 @@ def conv3x3_masked(x, w, mask, out16, bias=None):
         raise ValueError("weights must be [Cout, 9, Cin] and out16 [N, H, W, Cout]")
     if mask is not None and (mask.shape != out16.shape or mask.dtype != torch.bfloat16):
         raise ValueError("mask must be a bf16 tensor shaped like out16")
-    _ck(load().swe_nn_conv3x3_masked(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, _p(mask), _p(out16), _stream()),
-        "swe_nn_conv3x3_masked")
+    if work is not None:
+        _ck(load().swe_nn_conv3x3_ws(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, 2 if mask is not None else 0,
+                                     _p(mask), _p(out16), None, _p(work), work.numel(), _stream()),
+            "swe_nn_conv3x3_ws")
+    else:
+        _ck(load().swe_nn_conv3x3_masked(_p(x), N, H, W, Cin, _p(w), _p(bias), Cout, _p(mask), _p(out16),
+                                         _stream()), "swe_nn_conv3x3_masked")
     return out16
 
 

@@ -94,6 +94,7 @@ class Surrogate:
         self.step = 0
         self._work = None
         self._adam = None  # swe_nn_adam_multi table over every parameter tensor
+        self._cw = None    # split-K partials of the low-resolution convolutions
 
     # -- bookkeeping ----------------------------------------------------------
     def n_params(self):
@@ -146,6 +147,17 @@ class Surrogate:
         azt = torch.as_tensor(np.asarray(az, np.float32).reshape(N)).to(self.dev).contiguous()
         return nn.features(h, s, mean, azt, self.h_scale, self.s_scale, self.g_scale, self.l_scale)
 
+    def _conv_work(self, shape, cout):
+        """The workspace a convolution of input `shape` (NHWC) to `cout`
+        channels splits its reduction into, or None."""
+        torch = _torch()
+        need = nn.conv_work(shape[0], shape[1], shape[2], shape[3], cout)
+        if need == 0:
+            return None
+        if self._cw is None or self._cw.numel() < need:
+            self._cw = torch.empty(need, device=self.dev, dtype=torch.float32)
+        return self._cw
+
     def _buf(self, shape, dtype=None):
         torch = _torch()
         return torch.empty(shape, device=self.dev, dtype=dtype or torch.bfloat16)
@@ -161,10 +173,11 @@ class Surrogate:
         def conv(name, x, out32=False):
             cout = self.w16[name].shape[0]
             y = self._buf(x.shape[:3] + (cout,), torch.float32 if out32 else None)
+            ws = self._conv_work(x.shape, cout)
             if out32:
-                nn.conv3x3(x, self.w16[name], self.b32[name], relu=False, out32=y)
+                nn.conv3x3(x, self.w16[name], self.b32[name], relu=False, out32=y, work=ws)
             else:
-                nn.conv3x3(x, self.w16[name], self.b32[name], relu=True, out16=y)
+                nn.conv3x3(x, self.w16[name], self.b32[name], relu=True, out16=y, work=ws)
             A[name] = y
             return y
 
@@ -196,7 +209,8 @@ class Surrogate:
             if not need_dx:
                 return None
             dx = self._buf(x_in.shape[:3] + (self.w16[name].shape[2],))
-            return nn.conv3x3_masked(g16, self.wt16[name], mask, dx)
+            return nn.conv3x3_masked(g16, self.wt16[name], mask, dx,
+                                     work=self._conv_work(g16.shape, self.w16[name].shape[2]))
 
         skip = {}
         g = nn.relu_grad(dout, None, self._buf(dout.shape))  # the last layer is linear

@@ -144,6 +144,38 @@ def test_masked_conv_matches_unfused(Cin, Cout, N, H, W):
         assert torch.equal(got, want)
 
 
+@pytest.mark.parametrize("N,H,W,Cin,Cout", [(8, 16, 16, 256, 192), (8, 32, 32, 192, 192), (1, 4, 4, 96, 40)])
+def test_conv3x3_split_k(N, H, W, Cin, Cout):
+    """Layers with few pixel blocks split their reduction over several
+    workgroups (partials added in a fixed order): against float64, the same
+    bits on a second call, the masked data-gradient form equal to the float
+    output passed through the ReLU gradient, and unsplit without workspace."""
+    print("case split-k", N, H, W, Cin, Cout, flush=True)
+    need = nn.conv_work(N, H, W, Cin, Cout)
+    assert need > 0
+    work = torch.empty(need, device=DEV)
+    x = rnd((N, H, W, Cin), seed=21)
+    w = rnd((Cout, 9, Cin), seed=22, scale=0.1)
+    b = rnd((Cout,), torch.float32, seed=23)
+    mask = rnd((N, H, W, Cout), seed=24)
+    y32 = torch.empty((N, H, W, Cout), device=DEV)
+    y32b = torch.empty_like(y32)
+    nn.conv3x3(x, w, b, out32=y32, work=work)
+    nn.conv3x3(x, w, b, out32=y32b, work=work)
+    plain = torch.empty_like(y32)
+    nn.conv3x3(x, w, b, out32=plain)
+    g16 = nn.conv3x3_masked(x, w, mask, torch.empty((N, H, W, Cout), device=DEV, dtype=torch.bfloat16), bias=b,
+                            work=work)
+    torch.cuda.synchronize()
+    assert torch.equal(y32, y32b)
+    ref = ref_conv(x, w, b)
+    assert torch.allclose(y32, ref, atol=1e-4 * (9 * Cin) ** 0.5, rtol=1e-4)
+    assert torch.allclose(y32, plain, atol=1e-4 * (9 * Cin) ** 0.5, rtol=1e-4)
+    want = nn.relu_grad(y32, mask, torch.empty((N, H, W, Cout), device=DEV, dtype=torch.bfloat16))
+    torch.cuda.synchronize()
+    assert torch.equal(g16, want)
+
+
 def test_fused_skip_passes_match_unfused():
     """up2 + add, pool2_grad + add + ReLU mask and up2_grad + ReLU mask in
     one pass each: the same bits as the separate kernels."""
