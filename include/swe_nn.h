@@ -17,7 +17,8 @@ extern "C" {
 #endif
 
 /* y = conv3x3(x, w) + bias, optional ReLU; y16 (bf16) and/or y32 (float).
- * Cin % 8 == 0; BF16 MFMA with FP32 accumulation. */
+ * Cin % 8 == 0, x and w each under 2 GiB (else -1); BF16 MFMA with FP32
+ * accumulation. */
 int swe_nn_conv3x3(const void* x, int32_t N, int32_t H, int32_t W, int32_t Cin, const void* w, const float* bias,
                    int32_t Cout, int32_t relu, void* y16, float* y32, void* stream);
 /* y16 = bf16(conv3x3(x, w) + bias), zero where mask <= 0 (mask [N][H][W][Cout]

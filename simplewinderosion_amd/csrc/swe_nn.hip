@@ -26,6 +26,12 @@ __device__ __forceinline__ bf16 f2bf(float v) { return (bf16)v; }  // round to n
 
 __device__ __forceinline__ int32_t wrapi(int32_t v, int32_t n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// three offsets picked by a wave-uniform index in 0..2
+struct Off3 {
+  uint32_t v0, v1, v2;
+  __device__ __forceinline__ uint32_t at(int32_t i) const { return i == 0 ? v0 : (i == 1 ? v1 : v2); }
+};
+
 // ---------------------------------------------------------------------------
 // 3x3 convolution, implicit GEMM: rows = pixels (the batch's pixels
 // flattened, n-major), columns = output channels, reduction k = tap * Cin +
@@ -39,6 +45,8 @@ __device__ __forceinline__ int32_t wrapi(int32_t v, int32_t n) { return v < 0 ? 
 // per step).  Waves tile the block 2 x 2 (NT >= 64) or 4 x 1; per
 // v_mfma_f32_16x16x32_bf16, lane l supplies A row l%16 and B row l%16, both
 // at k = 8*(l/16) .. +7, and holds D rows 4*(l/16) .. +3 of column l%16.
+// A step issues only 8 MFMAs a wave, so its loop, address and barrier work
+// is what the step costs: no step divides, and addresses are 32-bit.
 // ---------------------------------------------------------------------------
 // ACT 0: none, 1: ReLU, 2: zero where mask <= 0 (the ReLU gradient of a
 // data gradient through the layer below, mask = that layer's output).
@@ -56,8 +64,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
   constexpr int WN = 4 / WM;            // ... and the channels
   constexpr int TM = 128 / WM, TN = NT / WN;
   constexpr int BM = TM / 16, BN = TN / 16;
-  constexpr int LD = 40;
-  constexpr int BROWS = NT < 64 ? NT : 64;  // B rows a pass of 64 threads' row index covers
+  constexpr int KC = 32, LD = KC + 8;
   __shared__ __attribute__((aligned(16))) bf16 sA[2][128 * LD];
   __shared__ __attribute__((aligned(16))) bf16 sB[2][NT * LD];
   const int32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -65,41 +72,75 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
   const int64_t P = (int64_t)N * H * W;
   const int64_t p0 = (int64_t)blockIdx.x * 128;
   const int32_t co0 = blockIdx.y * NT;
-  const int32_t cchunks = (Cin + 31) / 32, ksteps = 9 * cchunks;
+  const int32_t cchunks = (Cin + KC - 1) / KC, ksteps = 9 * cchunks;
   const int32_t g = tid & 3, r0 = tid >> 2;  // loader: row r0 (+64), channel group g
 
-  // the two A rows of this thread: image, row, column
-  int32_t ai[2], ay[2], ax[2];
-  bool aok[2];
+  // Loads are raw buffer loads at 32-bit byte offsets (the host keeps x and
+  // w under 2 GiB): an offset at or above kBad lies beyond the buffer and
+  // reads zero, which stands for pixels past P, output channels past Cout and
+  // input channels past Cin.  Each thread keeps the byte offsets of its two
+  // A pixels' three (periodic) rows and three columns, so a step's address
+  // is two selects and an add rather than 64-bit index arithmetic.
+  constexpr uint32_t kBad = 0x80000000u;
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)(P * Cin * 2), 0x00020000);
+  const __amdgpu_buffer_rsrc_t wr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)w, (short)0, (int)((int64_t)Cout * 9 * Cin * 2), 0x00020000);
+  Off3 rowo[2], colo[2];  // named fields, not arrays: a select between them stays in registers
+  uint32_t wo[2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int64_t p = p0 + r0 + 64 * h;
-    aok[h] = p < P;
-    const int64_t pc = aok[h] ? p : 0;
-    ax[h] = (int32_t)(pc % W);
-    ay[h] = (int32_t)((pc / W) % H);
-    ai[h] = (int32_t)(pc / ((int64_t)W * H));
+    const bool ok = p < P;
+    const int64_t pc = ok ? p : 0;
+    const int32_t ax = (int32_t)(pc % W), ay = (int32_t)((pc / W) % H), ai = (int32_t)(pc / ((int64_t)W * H));
+    auto row = [&](int32_t d) { return ok ? (uint32_t)(((int64_t)ai * H + wrapi(ay + d, H)) * W * Cin * 2) : kBad; };
+    auto col = [&](int32_t d) { return (uint32_t)(wrapi(ax + d, W) * Cin * 2 + 16 * g); };
+    rowo[h] = Off3{row(-1), row(0), row(1)};
+    colo[h] = Off3{col(-1), col(0), col(1)};
+    const int32_t co = co0 + r0 + 64 * h;
+    wo[h] = co < Cout ? (uint32_t)((int64_t)co * 9 * Cin * 2 + 16 * g) : kBad;
   }
-  const bf16x8 zero = {};
+  const int32_t kz = (int32_t)gridDim.z, z = (int32_t)blockIdx.z;
+  const int32_t s0 = (int32_t)((int64_t)z * ksteps / kz), s1 = (int32_t)((int64_t)(z + 1) * ksteps / kz);
+  // the next step to load, kept as tap (row ty, column tx) and chunk c so
+  // that no step divides; abase: the A byte offsets of that tap, recomputed
+  // only when the tap changes, so a step adds the chunk and selects kBad for
+  // channels past Cin
+  int32_t ltap = s0 / cchunks, lc = s0 - ltap * cchunks;
+  int32_t lty = ltap / 3, ltx = ltap - 3 * lty;
+  uint32_t abase[2];
+  auto settap = [&]() __attribute__((always_inline)) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) abase[h] = rowo[h].at(lty) + colo[h].at(ltx);
+  };
+  settap();
+  const int32_t cin_g = Cin - 8 * g;  // a lane's channels at k are valid while k < cin_g
   bf16x8 ra[2], rb[2];
-  auto load = [&](int32_t s) __attribute__((always_inline)) {
-    const int32_t tap = s / cchunks, ci = (s - tap * cchunks) * 32 + 8 * g;
-    const int32_t dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const bool cok = ci < Cin;
+  auto load = [&]() __attribute__((always_inline)) {
+    const int32_t k = lc * KC;
+    const bool cok = k < cin_g;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      ra[h] = zero;
-      if (aok[h] && cok) {
-        const int32_t yy = wrapi(ay[h] + dy, H), xx = wrapi(ax[h] + dx, W);
-        ra[h] = *(const bf16x8*)(x + (((int64_t)ai[h] * H + yy) * W + xx) * Cin + ci);
-      }
+      const uint32_t o = abase[h] + (uint32_t)(k * 2);
+      ra[h] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(xr, (int)(cok ? o : kBad), 0, 0));
     }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-      rb[h] = zero;
       const int32_t row = r0 + 64 * h;
-      const int32_t co = co0 + row;
-      if (row < NT && (h == 0 || NT > 64) && co < Cout && cok) rb[h] = *(const bf16x8*)(w + ((int64_t)co * 9 + tap) * Cin + ci);
+      if (row < NT && (h == 0 || NT > 64)) {
+        const uint32_t o = wo[h] + (uint32_t)((ltap * Cin + k) * 2);
+        rb[h] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(wr, (int)(cok ? o : kBad), 0, 0));
+      }
+    }
+    if (++lc == cchunks) {
+      lc = 0;
+      ++ltap;
+      if (++ltx == 3) {
+        ltx = 0;
+        ++lty;
+      }
+      settap();
     }
   };
   auto store = [&](int32_t buf) __attribute__((always_inline)) {
@@ -111,7 +152,6 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
       if (row < NT && (h == 0 || NT > 64)) *(bf16x8*)&sB[buf][row * LD + 8 * g] = rb[h];
     }
   };
-  (void)BROWS;
 
   f32x4 acc[BM][BN];
 #pragma unroll
@@ -119,15 +159,13 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
 #pragma unroll
     for (int j = 0; j < BN; j++) acc[i][j] = (f32x4){0.0f, 0.0f, 0.0f, 0.0f};
 
-  const int32_t kz = (int32_t)gridDim.z, z = (int32_t)blockIdx.z;
-  const int32_t s0 = (int32_t)((int64_t)z * ksteps / kz), s1 = (int32_t)((int64_t)(z + 1) * ksteps / kz);
-  load(s0);
+  load();
   store(0);
   __syncthreads();
   const int32_t kq = 8 * (lane >> 4), rl = lane & 15;
   for (int32_t s = s0; s < s1; s++) {
     const int32_t buf = (s - s0) & 1;
-    if (s + 1 < s1) load(s + 1);
+    if (s + 1 < s1) load();
     bf16x8 a[BM], b[BN];
 #pragma unroll
     for (int i = 0; i < BM; i++) a[i] = *(const bf16x8*)&sA[buf][(wm * TM + 16 * i + rl) * LD + kq];
@@ -1056,8 +1094,8 @@ extern "C" {
 #define SWENN_CONV_MINROWS 8
 #endif
 // split-K factor of k_conv3x3 for a grid of mt x ncob workgroups: layers
-// with fewer workgroups than the chip has CUs and a long reduction are
-// split so that ~768 workgroups take >= 8 k-steps each (at most 8 parts)
+// with fewer workgroups than the chip has CUs and a long reduction are split
+// so that ~768 workgroups take >= 8 k-steps each (at most 8 parts)
 static int32_t conv_split(int64_t mt, int64_t ncob, int32_t ksteps) {
   const int64_t wgs = mt * ncob;
   if (wgs >= 256 || ksteps < 24) return 1;
@@ -1083,6 +1121,8 @@ static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t C
                        float* work = nullptr, int64_t work_len = 0) {
   if (!x || !w || N < 1 || H < 1 || W < 1 || Cin < 8 || Cin % 8 || Cout < 1 || (!y16 && !y32)) return -1;
   const int64_t P = (int64_t)N * H * W;
+  // k_conv3x3's buffer loads take 32-bit byte offsets below 2^31
+  if (P * Cin * 2 >= ((int64_t)1 << 31) || (int64_t)Cout * 9 * Cin * 2 >= ((int64_t)1 << 31)) return -1;
   const bf16* xx = (const bf16*)x;
   const bf16* ww = (const bf16*)w;
   const bf16* mm = (const bf16*)mask;

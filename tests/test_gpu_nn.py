@@ -176,6 +176,24 @@ def test_conv3x3_split_k(N, H, W, Cin, Cout):
     assert torch.equal(g16, want)
 
 
+def test_conv3x3_refuses_tensors_of_2_gib():
+    """The convolution addresses x and w with 32-bit byte offsets: shapes
+    whose input or weights reach 2 GiB are refused (-1) before any launch."""
+    x = rnd((1, 4, 4, 32), seed=25)
+    w = rnd((32, 9, 32), seed=26, scale=0.1)
+    y = torch.empty((1, 4, 4, 32), device=DEV, dtype=torch.bfloat16)
+    lib = nn.load()
+    # 2^14 x 2^8 x 2^8 pixels x 32 channels x 2 bytes = 2^36 bytes of input
+    assert lib.swe_nn_conv3x3(x.data_ptr(), 1 << 14, 256, 256, 32, w.data_ptr(), None, 32, 0, y.data_ptr(), None,
+                              None) == -1
+    # 2^18 output channels x 9 x 512 x 2 bytes of weights > 2 GiB
+    assert lib.swe_nn_conv3x3(x.data_ptr(), 1, 4, 4, 512, w.data_ptr(), None, 1 << 18, 0, y.data_ptr(), None,
+                              None) == -1
+    assert lib.swe_nn_conv3x3(x.data_ptr(), 1, 4, 4, 32, w.data_ptr(), None, 32, 0, y.data_ptr(), None, None) == 0
+    torch.cuda.synchronize()
+    assert torch.allclose(y.float(), ref_conv(x, w), atol=1e-2, rtol=1e-2)
+
+
 def test_fused_skip_passes_match_unfused():
     """up2 + add, pool2_grad + add + ReLU mask and up2_grad + ReLU mask in
     one pass each: the same bits as the separate kernels."""
