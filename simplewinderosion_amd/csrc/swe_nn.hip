@@ -302,11 +302,25 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, c
   const int32_t co_b = 16 * ct + rl;            // B row (output channel within the block)
   const int32_t co = co0 + co_b;
   const float bv = (bias && co < Cout) ? bias[co] : 0.0f;
+  // With 32 input channels a wave's nine B fragments (its 16 output
+  // channels, every tap) stay in registers for the whole strip, so a row
+  // step reads only A from LDS; with 64 they are read per row.
+  constexpr int BREG = CI == 32 ? 9 : 1;
+  bf16x8 breg[BREG];
+#pragma unroll
+  for (int t = 0; t < BREG; t++) breg[t] = *(const bf16x8*)(s_w + co_b * LDW + t * CI + kq);
   // step s: LDS holds rows s .. s + 2, `held` row s + 3 (needed from step
   // s + 1 on); row s + 4 is loaded into `next`, and `held` is stored into
   // the slot of row s - 1, free since the barrier that ended step s - 1
   auto step = [&](int32_t s, Row& held, Row& next) __attribute__((always_inline)) {
     if (s + 2 < rows) next = load(s + 4);
+    // D rows (pixels) 4 (lane / 16) + e of the wave's 16, column = channel co
+    const int64_t prow = (img + y0 + s) * W + x0 + 16 * pt + 4 * (lane >> 4);
+    float mk[4] = {1.0f, 1.0f, 1.0f, 1.0f};  // the mask is requested before the MFMAs
+    if (ACT == 2 && co < Cout) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) mk[e] = bf2f(mask[(prow + e) * Cout + co]);
+    }
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int dyi = 0; dyi < 3; dyi++) {
@@ -320,20 +334,18 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, c
 #pragma unroll
         for (int kc = 0; kc < CI / 32; kc++) {
           const bf16x8 a = *(const bf16x8*)(ap + 32 * kc);
-          const bf16x8 b = *(const bf16x8*)(bp + 32 * kc);
+          const bf16x8 b = CI == 32 ? breg[t % BREG] : *(const bf16x8*)(bp + 32 * kc);
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
       }
     }
-    // D rows (pixels) 4 (lane / 16) + e of the wave's 16, column = channel co
     if (co < Cout) {
-      const int64_t prow = (img + y0 + s) * W + x0 + 16 * pt;
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int64_t p = prow + 4 * (lane >> 4) + e;
+        const int64_t p = prow + e;
         float v = acc[e] + bv;
         if (ACT == 1) v = v > 0.0f ? v : 0.0f;
-        if (ACT == 2) v = bf2f(mask[p * Cout + co]) > 0.0f ? v : 0.0f;
+        if (ACT == 2) v = mk[e] > 0.0f ? v : 0.0f;
         if (y16) y16[p * Cout + co] = f2bf(v);
         if (y32) y32[p * Cout + co] = v;
       }
