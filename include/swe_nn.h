@@ -70,9 +70,11 @@ int swe_nn_pool2_grad_add(const void* dy, const void* skip, const void* mask, vo
 int swe_nn_up2_grad_mask(const void* dy, const void* mask, void* dx, int32_t N, int32_t h, int32_t w, int32_t C,
                          void* stream);
 /* mean squared error over the first `used` of C channels (target [npix][used]);
- * grad [npix][C]; part[ceil(npix*C/256)] partial loss sums */
-int swe_nn_mse(const float* pred, const float* target, float* grad, float* part, int64_t npix, int32_t C,
-               int32_t used, void* stream);
+ * its gradient by pred [npix][C] as float (grad) and/or bf16 (grad16, the
+ * bf16 rounding of grad), at least one of them; part[ceil(npix*C/256)]
+ * partial loss sums */
+int swe_nn_mse(const float* pred, const float* target, float* grad, void* grad16, float* part, int64_t npix,
+               int32_t C, int32_t used, void* stream);
 /* Adam on float master weights w (+ bf16 copy w16, optional); step >= 1 */
 int swe_nn_adam(float* w, void* w16, const float* g, float* m, float* v, int64_t n, float lr, float b1, float b2,
                 float eps, int32_t step, void* stream);

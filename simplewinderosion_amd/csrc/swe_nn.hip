@@ -369,7 +369,8 @@ __global__ void k_conv_reduce(const float* __restrict__ part, int32_t kz, int64_
   if (i >= n) return;
   float a = part[i];
   for (int32_t z = 1; z < kz; z++) a += part[(int64_t)z * n + i];
-  const int32_t co = (int32_t)(i % Cout);
+  // 32-bit remainder where n allows (a 64-bit one per element is slower than the loads)
+  const int32_t co = n <= 0x7fffffffLL ? (int32_t)((uint32_t)i % (uint32_t)Cout) : (int32_t)(i % Cout);
   float v = a + (bias ? bias[co] : 0.0f);
   if (ACT == 1) v = v > 0.0f ? v : 0.0f;
   if (ACT == 2) v = bf2f(mask[i]) > 0.0f ? v : 0.0f;
@@ -923,27 +924,41 @@ __global__ void k_pool2_v8(const bf16* __restrict__ x, bf16* __restrict__ y, int
 // (pred - target)^2 / count; grad = 2 (pred - target) / count (0 elsewhere).
 // Per-block partial sums of the loss.
 __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ target, float* __restrict__ grad,
-                      float* __restrict__ part, int64_t npix, int32_t C, int32_t used, float inv_count) {
+                      bf16* __restrict__ grad16, float* __restrict__ part, int64_t npix, int32_t C, int32_t used,
+                      float inv_count) {
   __shared__ float s[256];
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int32_t t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * 256;
+  const int64_t i = base + t;
+  // pixel and channel of i from the block's (one wave-uniform 64-bit divide)
+  // and 32-bit arithmetic
+  const int64_t pb = base / C;
+  const int32_t r = (int32_t)(base - pb * C) + t;
   float l = 0.0f;
   if (i < npix * C) {
-    const int32_t c = (int32_t)(i % C);
+    const int32_t c = r % C;
     float g = 0.0f;
     if (c < used) {
-      const float d = pred[i] - target[(i / C) * used + c];
+      const float d = pred[i] - target[(pb + r / C) * used + c];
       l = d * d * inv_count;
       g = 2.0f * d * inv_count;
     }
-    grad[i] = g;
+    if (grad) grad[i] = g;
+    if (grad16) grad16[i] = f2bf(g);
   }
-  s[threadIdx.x] = l;
+  // the block's sum as a binary tree over its 256 threads (thread t adds
+  // t + k for k = 128, 64, ..., 1); the levels below 64 run in wave 0 with
+  // shuffles, which pair the same lanes, so the sum is the same bits
+  s[t] = l;
   __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
-    __syncthreads();
+  if (t < 128) s[t] += s[t + 128];
+  __syncthreads();
+  if (t < 64) {
+    float v = s[t] + s[t + 64];
+#pragma unroll
+    for (int k = 32; k > 0; k >>= 1) v += __shfl_down(v, k, 64);
+    if (t == 0) part[blockIdx.x] = v;
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
 // Adam on FP32 master weights, with the BF16 copy the convolutions read
@@ -977,12 +992,16 @@ __global__ void k_adam_multi(const swe_nn_adam_seg* __restrict__ seg, int32_t ns
     c1 = hyper[1];
     c2 = hyper[2];
   }
+  // the segment of the block's first element (a wave-uniform search), then
+  // advanced for lanes past its end (a block meets few segments)
+  const int64_t i0 = (int64_t)blockIdx.x * 256;
   int32_t lo = 0, hi = nseg - 1;
   while (lo < hi) {
     const int32_t mid = (lo + hi + 1) >> 1;
-    if (seg[mid].off <= i) lo = mid;
+    if (seg[mid].off <= i0) lo = mid;
     else hi = mid - 1;
   }
+  while (lo + 1 < nseg && seg[lo + 1].off <= i) ++lo;
   const swe_nn_adam_seg& sg = seg[lo];
   const int64_t j = i - sg.off;
   const float gi = sg.g[j];
@@ -994,7 +1013,19 @@ __global__ void k_adam_multi(const swe_nn_adam_seg* __restrict__ seg, int32_t ns
   sg.w[j] = wi;
   if (sg.w16) ((bf16*)sg.w16)[j] = f2bf(wi);
   if (sg.wt16) {
-    const int32_t ci = (int32_t)(j % sg.cin), t = (int32_t)((j / sg.cin) % 9), co = (int32_t)(j / (9 * (int64_t)sg.cin));
+    // j = (co * 9 + t) * cin + ci; 32-bit division where the segment allows
+    // (a 64-bit divide per element cost more than the update's memory traffic)
+    int32_t ci, t, co;
+    if (sg.n <= 0x7fffffffLL) {
+      const uint32_t ju = (uint32_t)j, cin = (uint32_t)sg.cin, q = ju / cin;
+      ci = (int32_t)(ju - q * cin);
+      t = (int32_t)(q % 9u);
+      co = (int32_t)(q / 9u);
+    } else {
+      ci = (int32_t)(j % sg.cin);
+      t = (int32_t)((j / sg.cin) % 9);
+      co = (int32_t)(j / (9 * (int64_t)sg.cin));
+    }
     ((bf16*)sg.wt16)[((int64_t)ci * 9 + (8 - t)) * sg.cout + co] = f2bf(wi);
   }
 }
@@ -1390,11 +1421,12 @@ int swe_nn_up2_grad(const void* dy, void* dx, int32_t N, int32_t h, int32_t w, i
   SWENN_LAUNCH_CHECK();
 }
 
-int swe_nn_mse(const float* pred, const float* target, float* grad, float* part, int64_t npix, int32_t C,
-               int32_t used, void* stream) {
-  if (used < 1 || used > C) return -1;
+int swe_nn_mse(const float* pred, const float* target, float* grad, void* grad16, float* part, int64_t npix,
+               int32_t C, int32_t used, void* stream) {
+  if (!pred || !target || !part || (!grad && !grad16) || npix < 1 || used < 1 || used > C) return -1;
   const float inv = 1.0f / (float)((double)npix * used);
-  hipLaunchKernelGGL(k_mse, dim3(G(npix * C)), dim3(256), 0, S(stream), pred, target, grad, part, npix, C, used, inv);
+  hipLaunchKernelGGL(k_mse, dim3(G(npix * C)), dim3(256), 0, S(stream), pred, target, grad, (bf16*)grad16, part, npix,
+                     C, used, inv);
   SWENN_LAUNCH_CHECK();
 }
 

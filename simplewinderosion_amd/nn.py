@@ -31,7 +31,7 @@ SIGNATURES = {
     "swe_nn_up2_add": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_pool2_grad_add": (ctypes.c_int, [_P, _P, _P, _P, _I32, _I32, _I32, _I32, _P]),
     "swe_nn_up2_grad_mask": (ctypes.c_int, [_P, _P, _P, _I32, _I32, _I32, _I32, _P]),
-    "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _I64, _I32, _I32, _P]),
+    "swe_nn_mse": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _I32, _I32, _P]),
     "swe_nn_adam": (ctypes.c_int, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I32, _P]),
     "swe_nn_adam_multi": (ctypes.c_int, [_P, _I32, _I64, _F, _F, _F, _F, _I32, _P]),
     "swe_nn_adam_multi_dev": (ctypes.c_int, [_P, _I32, _I64, _P, _F, _F, _F, _P]),
@@ -228,11 +228,13 @@ def up2_grad_mask(dy, mask, dx):
     return dx
 
 
-def mse(pred, target, grad, part):
-    """pred [npix, C] float, target [npix, used] float -> loss partials; grad [npix, C]."""
+def mse(pred, target, grad, part, grad16=None):
+    """pred [npix, C] float, target [npix, used] float -> loss partials; the
+    gradient by pred into grad [npix, C] float and/or grad16 (bf16)."""
     npix, C = pred.shape[0], pred.shape[-1]
     used = target.shape[-1]
-    _ck(load().swe_nn_mse(_p(pred), _p(target), _p(grad), _p(part), npix, C, used, _stream()), "swe_nn_mse")
+    _ck(load().swe_nn_mse(_p(pred), _p(target), _p(grad), _p(grad16), _p(part), npix, C, used, _stream()),
+        "swe_nn_mse")
     return part
 
 

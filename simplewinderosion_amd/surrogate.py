@@ -195,8 +195,8 @@ class Surrogate:
         return b, A
 
     def backward(self, A, dout):
-        """dout [N,H,W,16] float: gradient of the loss by the output; fills
-        self.gw / self.gb."""
+        """dout [N,H,W,16] float or bf16: gradient of the loss by the output;
+        fills self.gw / self.gb."""
         torch = _torch()
         N, H, W, _ = dout.shape
         L = len(CH)
@@ -213,7 +213,8 @@ class Surrogate:
                                      work=self._conv_work(g16.shape, self.w16[name].shape[2]))
 
         skip = {}
-        g = nn.relu_grad(dout, None, self._buf(dout.shape))  # the last layer is linear
+        # the last layer is linear: its gradient is dout in bf16
+        g = dout if dout.dtype == torch.bfloat16 else nn.relu_grad(dout, None, self._buf(dout.shape))
         for l in range(L - 1):
             # dec_l_b (its input is dec_l_a's ReLU output), dec_l_a (input cat_l)
             g = layer("dec%db" % l, A["dec%da" % l], g, mask=A["dec%da" % l])
@@ -249,9 +250,9 @@ class Surrogate:
         out, A = self.forward(self.features(h, s, az))
         N, H, W, C = out.shape
         target = torch.stack((dh / self.d_scale, ds / self.ds_scale), dim=-1).reshape(N * H * W, N_OUT).contiguous()
-        grad = torch.empty_like(out)
+        grad = self._buf(out.shape)
         part = torch.empty(((N * H * W * C + 255) // 256,), device=self.dev)
-        nn.mse(out.view(N * H * W, C), target, grad.view(N * H * W, C), part)
+        nn.mse(out.view(N * H * W, C), target, None, part, grad16=grad.view(N * H * W, C))
         self.backward(A, grad)
         self.adam(lr)
         return part.sum()
@@ -304,9 +305,9 @@ class TrainGraph:
         N, H, W, C = out.shape
         target = torch.stack((self.dh / m.d_scale, self.ds / m.ds_scale), dim=-1).reshape(N * H * W, N_OUT)
         target = target.contiguous()
-        grad = torch.empty_like(out)
+        grad = m._buf(out.shape)
         part = torch.empty(((N * H * W * C + 255) // 256,), device=m.dev)
-        nn.mse(out.view(N * H * W, C), target, grad.view(N * H * W, C), part)
+        nn.mse(out.view(N * H * W, C), target, None, part, grad16=grad.view(N * H * W, C))
         m.backward(A, grad)
         nn.adam_multi_dev(m._adam_table(), self.hyper)
         return part

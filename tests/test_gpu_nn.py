@@ -264,14 +264,29 @@ def test_mse_and_adam():
     pred = rnd((500, 16), torch.float32, seed=9)
     target = rnd((500, 2), torch.float32, seed=10)
     grad = torch.empty_like(pred)
+    grad16 = torch.empty(pred.shape, device=DEV, dtype=torch.bfloat16)
     part = torch.empty(((500 * 16 + 255) // 256,), device=DEV)
+    part2 = torch.empty_like(part)
     nn.mse(pred, target, grad, part)
+    nn.mse(pred, target, None, part2, grad16=grad16)
     torch.cuda.synchronize()
     d = pred[:, :2] - target
     assert abs(float(part.sum()) - float((d * d).mean())) < 1e-5
+    # the block sums against a serial binary tree over each block's 256 terms
+    ld = torch.zeros(part.numel() * 256, dtype=torch.float64)
+    pad = torch.zeros_like(pred)
+    pad[:, :2] = (d * d) * (1.0 / d.numel())
+    ld[:pred.numel()] = pad.reshape(-1).cpu().double()
+    tree = ld.float().view(-1, 256)
+    k = 128
+    while k:
+        tree = tree[:, :k] + tree[:, k:2 * k]
+        k //= 2
+    assert torch.equal(part.cpu(), tree[:, 0])
     ref = torch.zeros_like(pred)
     ref[:, :2] = 2 * d / d.numel()
     assert torch.allclose(grad, ref, atol=1e-8, rtol=1e-5)
+    assert torch.equal(grad16, grad.to(torch.bfloat16)) and torch.equal(part2, part)
     w = rnd((1000,), torch.float32, seed=11)
     g = rnd((1000,), torch.float32, seed=12)
     m = torch.zeros_like(w)
