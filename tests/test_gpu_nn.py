@@ -66,7 +66,9 @@ def test_conv3x3_forward(N, H, W, Cin, Cout, relu):
 
 
 @pytest.mark.parametrize("Cin,Cout,N,H,W", [(16, 32, 2, 10, 70), (24, 384, 2, 10, 70), (64, 72, 2, 60, 70),
-                                             (128, 8, 1, 33, 40)])
+                                             (128, 8, 1, 33, 40),
+                                             (48, 40, 3, 5, 12),      # a 32-pixel step spans rows and images
+                                             (64, 64, 2, 16, 16)])    # the 16^2 bottom level's shape
 def test_conv3x3_gradients(Cin, Cout, N, H, W):
     """Data gradient (forward on flipped weights) and the weight and bias
     gradients, over several pixel chunks with a partial last one and
