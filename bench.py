@@ -258,6 +258,69 @@ def bench_surrogate(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_surrogate_layers(args):
+    """The surrogate's convolution kernels layer by layer at the training
+    shape (batch 8, 256^2 network): for every layer the forward (bias +
+    ReLU), the data gradient (flipped weights, the ReLU mask of the layer
+    below) and the weight and bias gradients, each timed over 20 calls with
+    events on the stream the kernels run on, from seeded synthetic tensors.
+    Prints one JSON line: TFLOP/s of the three passes over all layers
+    (3 x sum 2*Cin*Cout*9*pixels) against the BF16 dense MFMA peak."""
+    import torch
+    from simplewinderosion_amd import nn, surrogate as S
+
+    reps = 20
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) * 1000.0 / reps
+
+    layers, flops, total = [], 0.0, 0.0
+    for name, ci, co, _ in S.LAYERS:
+        N, H = 8, 256 >> S.LEVEL[name]
+        g = torch.Generator(device="cpu").manual_seed(args.seed)
+        x = torch.randn((N, H, H, ci), generator=g).to("cuda").to(torch.bfloat16)
+        w = (torch.randn((co, 9, ci), generator=g) * 0.1).to("cuda").to(torch.bfloat16)
+        b = torch.randn((co,), generator=g).to("cuda")
+        dy = (torch.randn((N, H, H, co), generator=g) * 0.01).to("cuda").to(torch.bfloat16)
+        y = torch.empty((N, H, H, co), device="cuda", dtype=torch.bfloat16)
+        dx = torch.empty((N, H, H, ci), device="cuda", dtype=torch.bfloat16)
+        wt = nn.flip(w)
+        nf, nd = nn.conv_work(N, H, H, ci, co), nn.conv_work(N, H, H, co, ci)
+        wsf = torch.empty(nf, device="cuda") if nf else None
+        wsd = torch.empty(nd, device="cuda") if nd else None
+        dw = torch.empty((co, 9, ci), device="cuda")
+        db = torch.empty((co,), device="cuda")
+        work = nn.wgrad(x, dy, dw, db)
+        t_f = timeit(lambda: nn.conv3x3(x, w, b, relu=True, out16=y, work=wsf))
+        # the first layer's input gradient is never needed in training
+        t_d = timeit(lambda: nn.conv3x3_masked(dy, wt, x, dx, work=wsd)) if name != "enc0a" else 0.0
+        t_w = timeit(lambda: nn.wgrad(x, dy, dw, db, work))
+        f = 2.0 * ci * co * 9 * N * H * H
+        flops += f * (3 if name != "enc0a" else 2)
+        total += t_f + t_d + t_w
+        layers.append({"layer": name, "cin": ci, "cout": co, "side": H, "forward_us": round(t_f, 1),
+                       "data_grad_us": round(t_d, 1), "weight_grad_us": round(t_w, 1)})
+    tflops = flops / (total * 1e-6) / 1e12
+    print(json.dumps({
+        "metric": "TFLOP/s", "value": tflops, "unit": "TFLOP/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": total / 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 MFMA (fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": "surrogate layers, batch 8 at 256x256", "layers": len(layers),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": tflops / 2500.0, "traffic": None},
+        "layers": layers}))
+
+
 def bench_batch(args):
     """The multi-terrain workload: 64 terrains (sides log-uniform in
     128..1024, own seed, wind azimuth uniform at 45 degrees elevation,
@@ -357,6 +420,9 @@ def main():
         return
     if args.workload == "surrogate":
         bench_surrogate(args)
+        return
+    if args.workload == "surrogate-layers":
+        bench_surrogate_layers(args)
         return
 
     from simplewinderosion_amd import workloads
