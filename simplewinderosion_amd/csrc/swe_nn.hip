@@ -302,13 +302,15 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, c
   const int32_t co_b = 16 * ct + rl;            // B row (output channel within the block)
   const int32_t co = co0 + co_b;
   const float bv = (bias && co < Cout) ? bias[co] : 0.0f;
-  // With 32 input channels a wave's nine B fragments (its 16 output
-  // channels, every tap) stay in registers for the whole strip, so a row
-  // step reads only A from LDS; with 64 they are read per row.
-  constexpr int BREG = CI == 32 ? 9 : 1;
+  // A wave's B fragments (its 16 output channels, every tap and 32-channel
+  // slice: 9 or 18) stay in registers for the whole strip, so a row step
+  // reads only A from LDS (measured faster than reading B per row also at
+  // 64 input channels, where they take 72 registers)
+  constexpr int BREG = 9 * (CI / 32);
   bf16x8 breg[BREG];
 #pragma unroll
-  for (int t = 0; t < BREG; t++) breg[t] = *(const bf16x8*)(s_w + co_b * LDW + t * CI + kq);
+  for (int u = 0; u < BREG; u++)
+    breg[u] = *(const bf16x8*)(s_w + co_b * LDW + (u / (CI / 32)) * CI + 32 * (u % (CI / 32)) + kq);
   // step s: LDS holds rows s .. s + 2, `held` row s + 3 (needed from step
   // s + 1 on); row s + 4 is loaded into `next`, and `held` is stored into
   // the slot of row s - 1, free since the barrier that ended step s - 1
@@ -330,11 +332,10 @@ __global__ __launch_bounds__(256) void k_conv_rows(const bf16* __restrict__ x, c
         const int32_t t = 3 * dyi + dxi;
         // A: pixel 16 pt + rl of the row, shifted by dx (column + dxi)
         const bf16* ap = xr + (16 * pt + rl + dxi) * LDX + kq;
-        const bf16* bp = s_w + co_b * LDW + t * CI + kq;
 #pragma unroll
         for (int kc = 0; kc < CI / 32; kc++) {
           const bf16x8 a = *(const bf16x8*)(ap + 32 * kc);
-          const bf16x8 b = CI == 32 ? breg[t % BREG] : *(const bf16x8*)(bp + 32 * kc);
+          const bf16x8 b = breg[t * (CI / 32) + kc];
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
         }
       }
