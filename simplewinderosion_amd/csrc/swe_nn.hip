@@ -1119,10 +1119,11 @@ extern "C" {
 
 // Maps whose rows are whole 32-pixel strips take k_conv_rows for layers of
 // at most 32 input channels, and of at most 64 when the output has at most
-// 2^22 values (measured on the surrogate's layers, MI355X: 32-channel layers
-// at 256^2 45 -> 30 us; with 64 input channels the 57 KB of LDS leave two
-// workgroups per CU and every 32-channel output block re-reads the rows, so
-// a 64 -> 64 layer at 128^2 is faster on k_conv3x3): 32 output channels a
+// 2^SWENN_CONV_ROWS64_LOG2 values (measured on the surrogate's layers,
+// MI355X: 32-channel layers at 256^2 45 -> 30 us; with 64 input channels
+// the 57 KB of LDS leave two workgroups per CU and every 32-channel output
+// block re-reads the rows, which the register-held weights now outweigh up
+// to the 64 -> 64 layers at 128^2, 2^23 values): 32 output channels a
 // workgroup, each strip cut into chunks of R rows, enough chunks for
 // ~SWENN_CONV_WGS workgroups but at most ceil(H / SWENN_CONV_MINROWS) a
 // strip.  Others take k_conv3x3, whose output channels per workgroup are the
@@ -1228,8 +1229,12 @@ static int conv_launch(const void* x, int32_t N, int32_t H, int32_t W, int32_t C
   SWENN_LAUNCH_CHECK();
 }
 
+#ifndef SWENN_CONV_ROWS64_LOG2
+#define SWENN_CONV_ROWS64_LOG2 23
+#endif
 static bool conv_rows_ok(int32_t W, int32_t Cin, int32_t Cout, int64_t P) {
-  return SWENN_CONV_ROWS && W % 32 == 0 && (Cin <= 32 || (Cin <= 64 && (int64_t)Cout * P <= ((int64_t)1 << 22)));
+  return SWENN_CONV_ROWS && W % 32 == 0 &&
+         (Cin <= 32 || (Cin <= 64 && (int64_t)Cout * P <= ((int64_t)1 << SWENN_CONV_ROWS64_LOG2)));
 }
 
 int64_t swe_nn_conv3x3_work(int32_t N, int32_t H, int32_t W, int32_t Cin, int32_t Cout) {
