@@ -81,11 +81,15 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16* __restrict__ x, con
   // input channels past Cin.  Each thread keeps the byte offsets of its two
   // A pixels' three (periodic) rows and three columns, so a step's address
   // is two selects and an add rather than 64-bit index arithmetic.
+  // Descriptors: stride 0 (raw: offsets are bytes, checked against
+  // num_records = the buffer's size), flags word 0x00020000 = the 32-bit
+  // data format the gfx9 family's untyped buffer loads need.
   constexpr uint32_t kBad = 0x80000000u;
+  constexpr int kRawFlags = 0x00020000;
   const __amdgpu_buffer_rsrc_t xr =
-      __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)(P * Cin * 2), 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)(P * Cin * 2), kRawFlags);
   const __amdgpu_buffer_rsrc_t wr =
-      __builtin_amdgcn_make_buffer_rsrc((void*)w, (short)0, (int)((int64_t)Cout * 9 * Cin * 2), 0x00020000);
+      __builtin_amdgcn_make_buffer_rsrc((void*)w, (short)0, (int)((int64_t)Cout * 9 * Cin * 2), kRawFlags);
   Off3 rowo[2], colo[2];  // named fields, not arrays: a select between them stays in registers
   uint32_t wo[2];
 #pragma unroll
