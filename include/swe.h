@@ -340,6 +340,39 @@ int swe_normals(swe_ctx* ctx, float* out);
 int swe_normals_noalloc(swe_ctx* ctx, float* out);
 
 /*
+ * Known-answer entry points (not on the hot path; a map context only).
+ *
+ * swe_debug_step: one Wind::fly() call (wind.hpp:28-117, with the cell
+ * lookup of :30-38) of each of n caller-given particles against the current
+ * snapshot, by the device code the step kernels run.  Host arrays:
+ *   state  n x 7 floats in/out: pos x, y, z, speed x, y, z, sediment
+ *   age    n in/out: calls made (0 = fresh: pos.y snaps to the ground);
+ *          -1 once the particle died
+ *   key    n RNG keys: the turbulence draws of call `age` (wind.hpp:83)
+ *   alive  n out: 1 for a particle-step, 0 when the particle died (left a
+ *          bounded map or passed max_age)
+ *   pickup n out: the fixed-point (2^-32) amount the call took from the
+ *          particle's cell (wind.hpp:110-112; negative: deposited)
+ *
+ * swe_debug_cascade: one World::cascade (world.hpp:79-153) at cell (x, y)
+ * against the current snapshot, with the settling factor of `events`
+ * cascade events (1 - (1 - settling)^events): count = in-map neighbours,
+ * nb_cell[8] their cells in the reference's offset order (world.hpp:83-92)
+ * and nb_fx[8] the fixed-point amount each receives (the centre loses the
+ * sum).
+ */
+int swe_debug_step(swe_ctx* ctx, int32_t n, float* state, int32_t* age, const uint64_t* key, int32_t* alive,
+                   int64_t* pickup);
+int swe_debug_cascade(swe_ctx* ctx, int32_t x, int32_t y, uint32_t events, int32_t* count, int64_t* nb_cell,
+                      int64_t* nb_fx);
+
+/*
+ * The particles' RNG seed (World::SEED, SimpleWindErosion.cpp:27-34, which
+ * seeds srand): the same as swe_set_params with params.seed = seed.
+ */
+int swe_set_rng(swe_ctx* ctx, uint64_t seed);
+
+/*
  * Terrain generator for the benchmark inputs: seeded fBm gradient noise
  * (hashed lattice gradients, quintic fade, `octaves` octaves with lacunarity
  * 2 and gain 0.5, base frequency `base_cycles` periods across the map,

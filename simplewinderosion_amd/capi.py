@@ -152,6 +152,11 @@ SIGNATURES = {
     "swe_reset_stats": (ctypes.c_int, [_P]),
     "swe_normals": (ctypes.c_int, [_P, _FP]),
     "swe_normals_noalloc": (ctypes.c_int, [_P, _FP]),
+    "swe_debug_step": (ctypes.c_int, [_P, ctypes.c_int32, _FP, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_int32), _I64P]),
+    "swe_debug_cascade": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                         ctypes.POINTER(ctypes.c_int32), _I64P, _I64P]),
+    "swe_set_rng": (ctypes.c_int, [_P, ctypes.c_uint64]),
     "swe_windmap": (ctypes.c_int, [_P, _FP]),
     "swe_mesh": (ctypes.c_int, [_P, _FP]),
     "swe_generate_terrain": (

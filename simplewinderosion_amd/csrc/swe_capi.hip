@@ -1481,6 +1481,107 @@ int swe_normals(swe_ctx* c, float* out) {
   return swe_normals_terrain(c, 0, out);
 }
 
+// Device memory of a call that may return early (HIPCHK): freed on scope exit
+struct DevScratch {
+  void* p = nullptr;
+  ~DevScratch() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int swe_debug_step(swe_ctx* c, int32_t n, float* state, int32_t* age, const uint64_t* key, int32_t* alive,
+                   int64_t* pickup) {
+  if (!c || n < 0 || (n > 0 && (!state || !age || !key || !alive || !pickup))) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "swe_debug_step: a map context only");
+  if (n == 0) return 0;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r;
+  if ((r = ensure_layers(c))) return r;
+  if (!c->zs[c->cur()] || (c->p.layers == 2 && !c->ss[c->cur()]))
+    return fail(c, SWE_ESTATE, "swe_debug_step: no snapshot");
+  // one allocation: state (7 floats), age, alive (int32), key, pick (64-bit) per particle
+  const size_t b_st = sizeof(float) * 7 * (size_t)n, b_i = sizeof(int32_t) * (size_t)n,
+               b_q = sizeof(uint64_t) * (size_t)n;
+  const size_t o_age = (b_st + 15) / 16 * 16, o_alive = o_age + (b_i + 15) / 16 * 16,
+               o_key = o_alive + (b_i + 15) / 16 * 16, o_pick = o_key + b_q, total = o_pick + b_q;
+  DevScratch buf;
+  if (hipMalloc(&buf.p, total) != hipSuccess) {
+    buf.p = nullptr;
+    return fail(c, SWE_ENOMEM, "swe_debug_step: hipMalloc failed");
+  }
+  char* base = (char*)buf.p;
+  swe::DebugFlyArgs a{};
+  a.S = c->zs[c->cur()];
+  a.SS = c->p.layers == 2 ? c->ss[c->cur()] : nullptr;
+  a.st = (float*)base;
+  a.age = (int32_t*)(base + o_age);
+  a.alive = (int32_t*)(base + o_alive);
+  a.key = (const uint64_t*)(base + o_key);
+  a.pick = (int64_t*)(base + o_pick);
+  a.n = n;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.rcp_nx = 1.0f / (float)c->nx;
+  a.rcp_ny = 1.0f / (float)c->ny;
+  a.mapscale = c->p.mapscale;
+  fill_consts(c, a.ps, a.nps);
+  a.p = c->p;
+  HIPCHK(c, hipMemcpyAsync(a.st, state, b_st, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(a.age, age, b_i, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void*)a.key, key, b_q, hipMemcpyHostToDevice, c->stream));
+  swe::launch_debug_fly(a, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(state, a.st, b_st, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(age, a.age, b_i, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(alive, a.alive, b_i, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pickup, a.pick, b_q, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int swe_debug_cascade(swe_ctx* c, int32_t x, int32_t y, uint32_t events, int32_t* count, int64_t* nb_cell,
+                      int64_t* nb_fx) {
+  if (!c || !count || !nb_cell || !nb_fx) return SWE_EINVAL;
+  if (c->nterr != 1) return fail(c, SWE_EINVAL, "swe_debug_cascade: a map context only");
+  if (x < 0 || y < 0 || x >= c->nx || y >= c->ny) return fail(c, SWE_EINVAL, "swe_debug_cascade: cell out of the map");
+  HIPCHK(c, hipSetDevice(c->device));
+  int r;
+  if ((r = ensure_layers(c))) return r;
+  if (!c->zs[c->cur()] || (c->p.layers == 2 && !c->ss[c->cur()]))
+    return fail(c, SWE_ESTATE, "swe_debug_cascade: no snapshot");
+  DevScratch buf;
+  if (hipMalloc(&buf.p, sizeof(int64_t) * 17) != hipSuccess) {
+    buf.p = nullptr;
+    return fail(c, SWE_ENOMEM, "swe_debug_cascade: hipMalloc failed");
+  }
+  swe::DebugCascadeArgs a{};
+  a.S = c->zs[c->cur()];
+  a.SS = c->p.layers == 2 ? c->ss[c->cur()] : nullptr;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.x = x;
+  a.y = y;
+  a.events = events;
+  a.cell = (int64_t*)buf.p;
+  a.fx = a.cell + 8;
+  a.count = (int32_t*)(a.cell + 16);
+  a.p = c->p;
+  swe::launch_debug_cascade(a, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(nb_cell, a.cell, sizeof(int64_t) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(nb_fx, a.fx, sizeof(int64_t) * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(count, a.count, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int swe_set_rng(swe_ctx* c, uint64_t seed) {
+  if (!c) return SWE_EINVAL;
+  swe_params q = c->p;
+  q.seed = seed;
+  return swe_set_params(c, &q);
+}
+
 int swe_normals_noalloc(swe_ctx* c, float* out) {
   if (!c || !out) return SWE_EINVAL;
   if (c->nterr != 1) return fail(c, SWE_EINVAL, "not available on a batch context");

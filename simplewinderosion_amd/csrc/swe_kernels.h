@@ -108,4 +108,34 @@ void launch_mesh(const float* S, float* out, int32_t nx, int32_t ny, float mapsc
 void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float mapscale, bool periodic,
                     hipStream_t s);
 
+// Known-answer entry points (not on the hot path): one fly() call of each
+// of n caller-given particles, and one cell's cascade, against a snapshot,
+// with the device functions the step kernels run.
+struct DebugFlyArgs {
+  const float* S;       // height snapshot
+  const float* SS;      // sediment snapshot (layers == 2)
+  float* st;            // n x 7: pos xyz, speed xyz, sediment (in/out)
+  int32_t* age;         // n, in/out
+  const uint64_t* key;  // n particle RNG keys
+  int32_t* alive;       // n out: 1 stepped, 0 died
+  int64_t* pick;        // n out: fixed-point amount taken from the cell
+  int32_t n, nx, ny;
+  float rcp_nx, rcp_ny, mapscale;
+  float ps[3], nps[3];
+  swe_params p;
+};
+void launch_debug_fly(const DebugFlyArgs& a, hipStream_t s);
+
+struct DebugCascadeArgs {
+  const float* S;
+  const float* SS;
+  int32_t nx, ny, x, y;
+  uint32_t events;
+  int32_t* count;  // out: in-map neighbours
+  int64_t* cell;   // out [8]: their cells in offset order
+  int64_t* fx;     // out [8]: the fixed-point amount each receives
+  swe_params p;
+};
+void launch_debug_cascade(const DebugCascadeArgs& a, hipStream_t s);
+
 }  // namespace swe

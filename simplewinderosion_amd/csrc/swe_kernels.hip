@@ -723,4 +723,120 @@ void launch_normals(const float* S, float* out, int32_t nx, int32_t ny, float ma
   else hipLaunchKernelGGL((k_normals<false>), dim3(grid_for(n, 256)), dim3(256), 0, s, S, out, nx, ny, mapscale);
 }
 
+// ---------------------------------------------------------------------------
+// Known-answer entry points
+// ---------------------------------------------------------------------------
+// fly()'s writes discarded: the call's pickup is the particle's ledger
+struct NullAcc {
+  __device__ void add_pickup(int64_t, int64_t, bool) {}
+  __device__ void add_event(int64_t) {}
+  __device__ void add_track(int64_t, float, f3) {}
+};
+
+template <bool PERIODIC, int LAYERS>
+__global__ __launch_bounds__(64) void k_debug_fly(DebugFlyArgs a) {
+  const int32_t i = (int32_t)(blockIdx.x * 64 + threadIdx.x);
+  if (i >= a.n) return;
+  FlyConsts k;
+  k.nx = a.nx;
+  k.ny = a.ny;
+  k.rcp_nx = a.rcp_nx;
+  k.rcp_ny = a.rcp_ny;
+  k.mapscale = a.mapscale;
+  k.max_age = a.p.max_age;
+  set_boundarylayer(k, a.p.boundarylayer);
+  k.suspension = a.p.suspension;
+  k.gravity = a.p.gravity;
+  k.abrasion = a.p.abrasion;
+  k.ps = mk(a.ps[0], a.ps[1], a.ps[2]);
+  k.nps = mk(a.nps[0], a.nps[1], a.nps[2]);
+  float* st = a.st + (int64_t)7 * i;
+  Particle w;
+  w.px = st[0];
+  w.py = st[1];
+  w.pz = st[2];
+  w.sx = st[3];
+  w.sy = st[4];
+  w.sz = st[5];
+  w.sed = st[6];
+  w.age = a.age[i];
+  w.ledger = 0;
+  w.lastc = 0;
+  int32_t cx, cy;
+  bool ok = cell_of<PERIODIC>(w.px, w.pz, k, cx, cy);  // wind.hpp:30-38
+  if (ok) {
+    GlobalMap<PERIODIC> m{a.S, a.SS, cx, cy, a.nx, a.ny};
+    NullAcc acc;
+    ok = fly<PERIODIC, false, LAYERS>(w, a.key[i], cx, cy, k, m, acc) == kStepped;
+  }
+  if (!ok) w.age = -1;  // the march's mark of a dead particle
+  st[0] = w.px;
+  st[1] = w.py;
+  st[2] = w.pz;
+  st[3] = w.sx;
+  st[4] = w.sy;
+  st[5] = w.sz;
+  st[6] = w.sed;
+  a.age[i] = w.age;
+  a.alive[i] = ok ? 1 : 0;
+  a.pick[i] = w.ledger;
+}
+
+template <bool PERIODIC, int LAYERS>
+__global__ void k_debug_cascade(DebugCascadeArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t c = (int64_t)a.x * a.ny + a.y;
+  float hn[8], sn[8];
+  int64_t q[8];
+  uint32_t valid = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    bool ok;
+    q[k] = colour_neighbour<PERIODIC>(a.x, a.y, k, a.nx, a.ny, ok);
+    hn[k] = ok ? a.S[q[k]] : 0.0f;
+    sn[k] = (LAYERS == 2 && ok) ? a.SS[q[k]] : 0.0f;
+    valid |= (ok ? 1u : 0u) << k;
+  }
+  const float hc = a.S[c];
+  const float sc = LAYERS == 2 ? a.SS[c] : 0.0f;
+  int64_t d[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const auto sed_of = [&](int32_t k) { return sn[k]; };
+  cascade_cell_dev<LAYERS>(hn, sed_of, hc, sc, valid, settle_factor_dev(a.p.settling, a.events), a.p.maxdiff,
+                           [&](int32_t k, int64_t amount) {
+#pragma unroll
+                             for (int j = 0; j < 8; j++)
+                               if (j == k) d[j] += amount;
+                           });
+  int32_t n = 0;
+  for (int k = 0; k < 8; k++) {
+    if (!((valid >> k) & 1u)) continue;
+    a.cell[n] = q[k];
+    a.fx[n] = d[k];
+    n++;
+  }
+  *a.count = n;
+}
+
+void launch_debug_fly(const DebugFlyArgs& a, hipStream_t s) {
+  if (a.n <= 0) return;
+  const dim3 g((unsigned)((a.n + 63) / 64));
+  if (a.p.periodic) {
+    if (a.p.layers == 2) hipLaunchKernelGGL((k_debug_fly<true, 2>), g, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_debug_fly<true, 1>), g, dim3(64), 0, s, a);
+  } else {
+    if (a.p.layers == 2) hipLaunchKernelGGL((k_debug_fly<false, 2>), g, dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_debug_fly<false, 1>), g, dim3(64), 0, s, a);
+  }
+}
+
+void launch_debug_cascade(const DebugCascadeArgs& a, hipStream_t s) {
+  if (a.p.periodic) {
+    if (a.p.layers == 2) hipLaunchKernelGGL((k_debug_cascade<true, 2>), dim3(1), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_debug_cascade<true, 1>), dim3(1), dim3(64), 0, s, a);
+  } else {
+    if (a.p.layers == 2) hipLaunchKernelGGL((k_debug_cascade<false, 2>), dim3(1), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_debug_cascade<false, 1>), dim3(1), dim3(64), 0, s, a);
+  }
+}
+
 }  // namespace swe

@@ -179,6 +179,43 @@ class World:
         self._check(self.lib.swe_normals(self.ctx, _fp(out)), "swe_normals")
         return out.reshape(3, self.nx, self.ny)
 
+    # -- known-answer entry points -------------------------------------------
+    def debug_step(self, state, age, key):
+        """One fly() call (wind.hpp:28-117) of each particle against the
+        current snapshot: state [n, 7] (pos xyz, speed xyz, sediment), age [n]
+        calls made, key [n] RNG keys.  Returns (state, age, alive, pickup)."""
+        st = np.ascontiguousarray(state, np.float32).reshape(-1, 7).copy()
+        n = st.shape[0]
+        ag = np.ascontiguousarray(age, np.int32).reshape(-1).copy()
+        ky = np.ascontiguousarray(key, np.uint64).reshape(-1)
+        if ag.size != n or ky.size != n:
+            raise ValueError("state, age and key describe different particle counts")
+        alive = np.zeros(n, np.int32)
+        pick = np.zeros(n, np.int64)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        self._check(self.lib.swe_debug_step(self.ctx, n, _fp(st), ag.ctypes.data_as(i32),
+                                            ky.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                            alive.ctypes.data_as(i32), _ip(pick)), "swe_debug_step")
+        return st, ag, alive, pick
+
+    def debug_cascade(self, x, y, events=1):
+        """One World::cascade (world.hpp:79-153) at (x, y) with `events`
+        events against the current snapshot: [(neighbour cell, fixed-point
+        amount received)] for the in-map neighbours in offset order."""
+        cnt = ctypes.c_int32()
+        cell = np.zeros(8, np.int64)
+        fx = np.zeros(8, np.int64)
+        self._check(self.lib.swe_debug_cascade(self.ctx, int(x), int(y), int(events), ctypes.byref(cnt),
+                                               _ip(cell), _ip(fx)), "swe_debug_cascade")
+        return [(int(cell[i]), int(fx[i])) for i in range(cnt.value)]
+
+    def set_rng(self, seed):
+        """World::SEED of the particles' counter RNG (swe_set_rng).  SEED is
+        a static of World, as in the reference, and erode() pushes the
+        statics: the class attribute is set too."""
+        World.SEED = int(seed)
+        self._check(self.lib.swe_set_rng(self.ctx, int(seed)), "swe_set_rng")
+
     # -- audit, counters, timing -------------------------------------------------
     def audit(self):
         a = capi.SweAudit()
