@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from simplewinderosion_amd import World, Wind, capi
+from simplewinderosion_amd import Batch, World, Wind, capi
 
 pytestmark = pytest.mark.gpu
 
@@ -136,3 +136,30 @@ def test_set_rng_is_the_seed():
     ha, hb, hc = a.download()["height"], b.download()["height"], c.download()["height"]
     assert np.array_equal(ha, hb)
     assert not np.array_equal(ha, hc)
+
+
+def test_debug_entries_refuse_bad_calls():
+    """A batch context and a cell outside the map are refused before any
+    launch, with the reason in swe_last_error."""
+    say("case debug refusals")
+    lib = capi.load()
+    w = World(NX, NY)
+    w.upload(terrain(1))
+    cnt = ctypes.c_int32()
+    cell = np.zeros(8, np.int64)
+    fx = np.zeros(8, np.int64)
+    i64 = ctypes.POINTER(ctypes.c_int64)
+    for (x, y) in ((-1, 0), (0, NY), (NX, 3)):
+        rc = lib.swe_debug_cascade(w.ctx, x, y, 1, ctypes.byref(cnt), cell.ctypes.data_as(i64),
+                                   fx.ctypes.data_as(i64))
+        assert rc == -1 and b"out of the map" in lib.swe_last_error(w.ctx)
+    b = Batch([(32, 32), (48, 40)])
+    st, age, key = particles(4, 1)
+    i32 = ctypes.POINTER(ctypes.c_int32)
+    alive = np.zeros(4, np.int32)
+    pick = np.zeros(4, np.int64)
+    rc = lib.swe_debug_step(b.ctx, 4, st.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), age.ctypes.data_as(i32),
+                            key.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), alive.ctypes.data_as(i32),
+                            pick.ctypes.data_as(i64))
+    assert rc == -1 and b"map context only" in lib.swe_last_error(b.ctx)
+    assert w.debug_step(st[:0], age[:0], key[:0])[2].size == 0  # no particles: nothing to do
