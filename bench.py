@@ -212,13 +212,15 @@ def bench_surrogate(args):
         print("bench: " + msg, file=sys.stderr, flush=True)
 
     K = 50
-    states, az, t_gen = S.generate_states(args.sur_terrains, args.sur_cycles, seed=args.seed, log=log)
+    states, az, t_gen = S.generate_states(args.sur_terrains, args.sur_cycles, seed=args.seed,
+                                          workload=args.sur_train_workload, log=log)
     model = S.Surrogate(seed=args.seed)
     S.fit_scales(model, states, K)
     curve, t_train = S.train(model, states, az, K, args.sur_steps, batch=8, lr=1e-3, log=log, log_every=250,
                              pushforward=args.sur_pushforward, pushforward_blocks=args.sur_pushforward_blocks,
-                             graph=args.sur_graph)
-    fps = 3 * S.Surrogate.flops_forward(8, 256, 256)
+                             graph=args.sur_graph, crop=args.sur_crop, border=args.sur_border)
+    side = args.sur_crop if args.sur_crop else states.shape[-1]
+    fps = 3 * S.Surrogate.flops_forward(8, side, side)
     del states
     torch.cuda.empty_cache()
     world = workloads.make_world("surr1024", seed=7, deposit_on_death=True)
@@ -250,7 +252,8 @@ def bench_surrogate(args):
             "tolerance_rms": [round(j["tol_rms"], 6) for j in judged],
             "noise_floor_at": "each judged state" if not args.sur_first_floor else "the first judged state",
             "mass_drift_fx": rec["mass_drift_fx"], "mass_exact": rec["mass_exact"],
-            "training": {"maps": args.sur_terrains, "cycles_per_map": args.sur_cycles, "steps": args.sur_steps,
+            "training": {"workload": args.sur_train_workload, "crop": args.sur_crop, "border": args.sur_border,
+                         "maps": args.sur_terrains, "cycles_per_map": args.sur_cycles, "steps": args.sur_steps,
                          "batch": 8, "graph": args.sur_graph, "pushforward": args.sur_pushforward,
                          "pushforward_blocks": args.sur_pushforward_blocks,
                          "final_loss": curve[-1][1], "tflops": fps * args.sur_steps / t_train / 1e12,
@@ -383,6 +386,12 @@ def main():
     ap.add_argument("--sur-terrains", type=int, default=64, help="surrogate: training maps")
     ap.add_argument("--sur-cycles", type=int, default=400, help="surrogate: plain cycles per training map")
     ap.add_argument("--sur-steps", type=int, default=12000, help="surrogate: Adam steps")
+    ap.add_argument("--sur-train-workload", default="surr256",
+                    help="surrogate: workload of the training maps (surr256, or surr1024 with --sur-crop)")
+    ap.add_argument("--sur-crop", type=int, default=0,
+                    help="surrogate: train on crops of this side from larger maps (0: whole maps)")
+    ap.add_argument("--sur-border", type=int, default=0,
+                    help="surrogate: band along a crop's edges left out of the loss gradient")
     ap.add_argument("--sur-blocks", type=int, default=40, help="surrogate: rollout blocks of K = 50 cycles")
     ap.add_argument("--sur-graph", action="store_true",
                     help="surrogate: replay one captured HIP graph per training step instead of launching its "

@@ -242,9 +242,11 @@ class Surrogate:
         self.step += 1
         nn.adam_multi(self._adam_table(), self.step, lr=lr)
 
-    def train_step(self, h, s, az, dh, ds, lr):
+    def train_step(self, h, s, az, dh, ds, lr, border=0):
         """One Adam step on a batch: heights h, sediment s [N,H,W], wind
         azimuths az [N], and their K-cycle changes dh, ds (device float).
+        With `border` > 0 the loss gradient is zero on that band along the
+        edges (crops of a larger map, whose own edges are not periodic).
         Returns the loss tensor (device)."""
         torch = _torch()
         out, A = self.forward(self.features(h, s, az))
@@ -253,6 +255,11 @@ class Surrogate:
         grad = self._buf(out.shape)
         part = torch.empty(((N * H * W * C + 255) // 256,), device=self.dev)
         nn.mse(out.view(N * H * W, C), target, None, part, grad16=grad.view(N * H * W, C))
+        if border > 0:
+            grad[:, :border] = 0
+            grad[:, H - border:] = 0
+            grad[:, :, :border] = 0
+            grad[:, :, W - border:] = 0
         self.backward(A, grad)
         self.adam(lr)
         return part.sum()
@@ -425,7 +432,7 @@ def symmetry(x, az, k):
 
 
 def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_every=100, augment=True,
-          pushforward=0.0, pushforward_from=0.5, pushforward_blocks=1, graph=False):
+          pushforward=0.0, pushforward_from=0.5, pushforward_blocks=1, graph=False, crop=0, border=0):
     """Adam on random (terrain, t) pairs of `states` with a cosine-decayed
     learning rate; with `augment`, each batch under a random one of the
     map's eight symmetries (square maps).  With `pushforward` p > 0, after
@@ -436,11 +443,34 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
     rollout reaches after b surrogate blocks -- and its target the plain
     state (b + 1) K cycles on, so that the network learns to correct its own
     drift.  With `graph`, the steps replay one captured HIP graph
-    (TrainGraph: the same bits, less host time).  Returns the loss curve and
-    the time of the steps."""
+    (TrainGraph: the same bits, less host time).  With `crop` > 0 each
+    pair is a crop x crop window at a random place of its (periodic, larger)
+    map, the loss gradient zero on a `border` band along the crop's edges,
+    which are not periodic.  Returns the loss curve and the time of the
+    steps."""
     torch = _torch()
     T, n = states.shape[0] - K, states.shape[1]
     T2 = states.shape[0] - 2 * K  # pushforward needs two blocks of data after t
+    MX, MY = states.shape[-2], states.shape[-1]
+    crop = int(crop) if crop and crop < min(MX, MY) else 0
+    if crop and (pushforward > 0 or graph):
+        raise ValueError("crops are not combined with pushforward or graph training")
+    dev = states.device
+
+    def crops(t, k):
+        """[batch, 2, crop, crop] windows of states[t, k] at random places (wrapping)."""
+        ox = torch.randint(0, MX, (batch,), generator=g)
+        oy = torch.randint(0, MY, (batch,), generator=g)
+        ar = torch.arange(crop)
+        ix = ((ox[:, None] + ar[None, :]) % MX).to(dev)
+        iy = ((oy[:, None] + ar[None, :]) % MY).to(dev)
+        ch = torch.arange(2, device=dev)[None, :, None, None]
+
+        def take(tt):
+            return states[tt.to(dev)[:, None, None, None], k.to(dev)[:, None, None, None], ch,
+                          ix[:, None, :, None], iy[:, None, None, :]]
+        return take(t), take(t + K)
+
     g = torch.Generator(device="cpu").manual_seed(seed)
     az = np.asarray(az, np.float64)
     augment = augment and states.shape[-1] == states.shape[-2]
@@ -466,6 +496,8 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
                 h, sd = (h + dh).contiguous(), (sd + ds).clamp_min(0.0).contiguous()
             x0 = torch.stack((h, sd), dim=1)
             x1 = states[t + (nb + 1) * K, k]
+        elif crop:
+            x0, x1 = crops(t, k)
         else:
             x0, x1 = states[t, k], states[t + K, k]
         if augment:
@@ -480,7 +512,7 @@ def train(model, states, az, K, steps, batch=8, lr=1e-3, seed=0, log=None, log_e
                 tg = TrainGraph(model, *args[0].shape)
             loss = tg.step(*args)
         else:
-            loss = model.train_step(*args)
+            loss = model.train_step(*args, border=border if crop else 0)
         if st % log_every == 0 or st == steps - 1:
             curve.append((st, float(loss)))
             if log:

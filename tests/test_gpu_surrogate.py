@@ -106,6 +106,30 @@ def test_graph_training_steps_match_eager():
     assert tg.graph is not None
 
 
+def test_crop_training_masks_the_border():
+    """Training on crops of a larger periodic map: with the border band
+    covering the whole crop the loss gradient is zero and Adam leaves every
+    weight bit for bit; with a narrow band the weights move."""
+    print("case crop training", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    states = (torch.rand((6, 2, 2, 48, 48), generator=g) * 0.1).to("cuda")
+    az = np.array([0.3, 1.7])
+
+    def run(border):
+        m = S.Surrogate(seed=0)
+        S.fit_scales(m, states, 2)
+        before = {k: v.clone() for k, v in m.w32.items()}
+        curve, _ = S.train(m, states, az, 2, 2, batch=4, crop=32, border=border, seed=1)
+        torch.cuda.synchronize()
+        assert all(np.isfinite(l) for _, l in curve)
+        return sum(int(not torch.equal(before[k], m.w32[k])) for k in before), len(before)
+
+    changed, total = run(16)  # bands of 16 on both sides of 32 cover the crop
+    assert changed == 0
+    changed, total = run(4)
+    assert changed == total
+
+
 def test_training_fits_a_batch():
     print("case surrogate fit", flush=True)
     m = S.Surrogate(seed=4)
